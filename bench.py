@@ -1,0 +1,264 @@
+#!/usr/bin/env python3
+"""Throughput of the hot path on one MI355X, as one JSON line.
+
+Default workload (the metric's long-FIR configuration): 2^24 complex64 samples (complex
+Gaussian noise of unit power plus 5 tones of amplitude sqrt(10) at random frequencies and
+phases, seeded), filtered by a 1025-tap Kaiser (beta 8) low-pass with cutoff 0.1 fs and unit DC
+gain. One step is one rf_fir call over the whole capture with inputs already in HBM.
+
+  python bench.py [--steps K] [--warmup W] [--impl fft_ola] [--workload fir_long|fir_plain|
+                   channelize] [--gpus 1]
+
+Input/output buffers rotate over three sets (768 MiB for the long FIR) so that the 256 MB
+Infinity Cache cannot hold a step's input from the previous visit. Timing: device events on
+the stream the kernels run on, bracketing all K steps, after a device synchronisation. The
+roofline figure uses the algorithmic bytes of one step (input + output + taps) over the
+measured step time. The CPU baseline is the float64 oracle (single thread) on a bounded
+prefix of the same input.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+HBM_PEAK_GBS = 8000.0
+METRIC = "Msamples/s + %HBM roofline (FIR, channelizer); GCC-PHAT pairs/s at fixed error"
+
+
+def load_rfhip():
+    pkg = ROOT / "gpu-rf-pipeline_amd"
+    if not (pkg / "librfhip.so").exists():
+        import subprocess
+        subprocess.run(["make", "-s", "-C", str(pkg), "-j8", "librfhip.so"], check=True)
+    spec = importlib.util.spec_from_file_location("rfhip", pkg / "__init__.py")
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["rfhip"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def load_oracle():
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle  # the checker and the CPU baseline
+    return oracle
+
+
+def kaiser_lowpass(num_taps: int, cutoff: float, beta: float) -> np.ndarray:
+    n = np.arange(num_taps) - (num_taps - 1) / 2
+    h = 2 * cutoff * np.sinc(2 * cutoff * n) * np.kaiser(num_taps, beta)
+    return (h / h.sum()).astype(np.float32)
+
+
+def hamming_lowpass(num_taps: int, cutoff: float) -> np.ndarray:
+    n = np.arange(num_taps) - (num_taps - 1) / 2
+    h = 2 * cutoff * np.sinc(2 * cutoff * n) * np.hamming(num_taps)
+    return (h / h.sum()).astype(np.float32)
+
+
+def fir_long_input(n: int = 1 << 24) -> tuple[np.ndarray, np.ndarray]:
+    """Complex Gaussian noise (I, Q ~ N(0, 1/2)) plus 5 tones at 10 dB SNR, seed 0xC2."""
+    rng = np.random.Generator(np.random.Philox(0xC2))
+    x = (rng.standard_normal(n, dtype=np.float32)
+         + 1j * rng.standard_normal(n, dtype=np.float32)) * np.float32(np.sqrt(0.5))
+    t = np.arange(n, dtype=np.float64)
+    freqs = rng.uniform(-0.5, 0.5, 5)
+    phases = rng.uniform(0, 2 * np.pi, 5)
+    tones = np.zeros(n, dtype=np.complex128)
+    for f, p in zip(freqs, phases):
+        tones += np.sqrt(10.0) * np.exp(1j * (2 * np.pi * f * t + p))
+    return (x + tones).astype(np.complex64), kaiser_lowpass(1025, 0.1, 8.0)
+
+
+def fir_plain_input(n: int = 1 << 20) -> tuple[np.ndarray, np.ndarray]:
+    """I, Q ~ N(0, 1) plus unit tones at 0.05, 0.17, 0.31 fs; 63-tap Hamming sinc 0.2 fs."""
+    rng = np.random.Generator(np.random.Philox(0xC1))
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    t = np.arange(n)
+    for f in (0.05, 0.17, 0.31):
+        x = x + np.exp(2j * np.pi * f * t)
+    return x.astype(np.complex64), hamming_lowpass(63, 0.2)
+
+
+def sampled_fir_error(x: np.ndarray, h: np.ndarray, y: np.ndarray, count: int = 2000):
+    """Max abs / rel error of `count` outputs against float64 dot products."""
+    rng = np.random.default_rng(11)
+    idx = np.concatenate([np.arange(min(h.size + 8, x.size)), rng.integers(0, x.size, count)])
+    k = np.arange(h.size)
+    src = idx[:, None] - k[None, :]
+    xv = np.where(src >= 0, x[np.clip(src, 0, None)].astype(np.complex128), 0)
+    want = xv @ h.astype(np.float64)
+    err = np.abs(y[idx].astype(np.complex128) - want)
+    return float(err.max()), float((err / np.maximum(np.abs(want), 1e-30)).max()), \
+        float((err - 1e-5 * np.abs(want)).max())
+
+
+def time_steps(rf, step, steps: int, warmup: int, stream) -> float:
+    for i in range(warmup):
+        step(i)
+    rf.synchronize()
+    start, stop = rf.Event(), rf.Event()
+    start.record(stream)
+    for i in range(steps):
+        step(warmup + i)
+    stop.record(stream)
+    stop.synchronize()
+    rf.synchronize()
+    return stop.ms_since(start)
+
+
+def bench_fir(rf, args, long_filter: bool) -> dict:
+    x, h = fir_long_input() if long_filter else fir_plain_input()
+    n, t = x.size, h.size
+    sets = 3
+    stream = rf.Stream()
+    d_h = rf.DeviceBuffer.from_array(h)
+    d_x = [rf.DeviceBuffer.from_array(x) for _ in range(sets)]
+    d_y = [rf.DeviceBuffer(n) for _ in range(sets)]
+    impl = rf.fir_impl_index(args.impl)
+
+    def step_for(impl_idx):
+        def step(i):
+            rf.fir_device(impl_idx, d_x[i % sets].ptr, d_h.ptr, d_y[i % sets].ptr, n, t, 0,
+                          stream.handle)
+        return step
+
+    ms = time_steps(rf, step_for(impl), args.steps, args.warmup, stream.handle)
+    ms_step = ms / args.steps
+    y = d_y[(args.warmup + args.steps - 1) % sets].download()
+    max_abs, max_rel, excess = sampled_fir_error(x, h, y)
+    bytes_step = 16 * n + 4 * t
+    achieved = bytes_step / (ms_step * 1e-3) / 1e9
+
+    variants = {}
+    for name in [v.name for v in rf.fir_impls()]:
+        if name == args.impl or name == "naive" and long_filter:
+            continue
+        vms = time_steps(rf, step_for(rf.fir_impl_index(name)), max(2, args.steps // 5), 1,
+                         stream.handle) / max(2, args.steps // 5)
+        variants[name] = {"ms_per_step": round(vms, 4), "msamples_per_s": round(n / vms / 1e3, 1)}
+
+    cpu = None
+    if not args.no_cpu:
+        oracle = load_oracle()
+        sample = min(n, 1 << 23) if long_filter else n
+        t0 = time.perf_counter()
+        oracle.fir(x[:sample], h)
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(sample / dt / 1e6, 4), "unit": "Msamples/s", "cores": 1,
+               "kind": "port",
+               "sample": f"float64 oracle (oracle/oracle.c orc_fir), one thread, first {sample} "
+                         f"of the {n} samples x {t} taps, {dt:.2f} s"}
+    workload = (f"FIR {n} complex64 samples x {t}-tap "
+                + ("Kaiser(beta=8) lowpass 0.1 fs" if long_filter else "Hamming lowpass 0.2 fs"))
+    return {
+        "metric": METRIC,
+        "value": round(n / ms_step / 1e3, 1),
+        "unit": "Msamples/s",
+        "ms_per_step": round(ms_step, 5),
+        "dtype": "f32",
+        "config": {"workload": workload, "impl": args.impl, "samples": n, "taps": t,
+                   "buffer_sets": sets, "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles(f"fir_{args.impl}")},
+        "accuracy": {"max_abs_err": max_abs, "max_rel_err": max_rel,
+                     "within_rtol_atol_1e-5": excess <= 1e-5,
+                     "checked_outputs": "first taps+8 and 2000 random outputs vs float64"},
+        "variants": variants,
+        "cpu_baseline": cpu,
+    }
+
+
+def bench_channelize(rf, args) -> dict:
+    n, m, t = 1 << 24, 64, 1024
+    rng = np.random.Generator(np.random.Philox(0xC3))
+    x = ((rng.standard_normal(n, dtype=np.float32) + 1j * rng.standard_normal(n, dtype=np.float32))
+         * np.float32(0.01)).astype(np.complex64)
+    h = kaiser_lowpass(t, 1.0 / 128, 8.0)
+    frames = rf.channelize_frames(n, m)
+    sets = 3
+    stream = rf.Stream()
+    d_h = rf.DeviceBuffer.from_array(h)
+    d_x = [rf.DeviceBuffer.from_array(x) for _ in range(sets)]
+    d_z = [rf.DeviceBuffer(frames * m) for _ in range(sets)]
+
+    def step(i):
+        rf.channelize_device(None, d_x[i % sets].ptr, d_h.ptr, d_z[i % sets].ptr, n, m, t,
+                             stream.handle)
+
+    ms = time_steps(rf, step, args.steps, args.warmup, stream.handle)
+    ms_step = ms / args.steps
+    bytes_step = 16 * n + 4 * t
+    achieved = bytes_step / (ms_step * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        oracle = load_oracle()
+        sample = 1 << 22
+        t0 = time.perf_counter()
+        oracle.channelize(x[:sample], h, m)
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(sample / dt / 1e6, 4), "unit": "Msamples/s", "cores": 1,
+               "kind": "port", "sample": f"float64 oracle, one thread, first {sample} samples"}
+    return {
+        "metric": METRIC, "value": round(n / ms_step / 1e3, 1), "unit": "Msamples/s",
+        "ms_per_step": round(ms_step, 5), "dtype": "f32",
+        "config": {"workload": f"polyphase channelizer {n} complex64 samples, M={m}, "
+                               f"{t}-tap Kaiser prototype", "samples": n, "channels": m,
+                   "taps": t, "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles("channelize_fused")},
+        "cpu_baseline": cpu,
+    }
+
+
+def traffic_from_profiles(name: str):
+    """HBM bytes per launch from a committed rocprofv3 counter summary, if one exists."""
+    path = ROOT / "profiles" / f"{name}_traffic.json"
+    if path.exists():
+        try:
+            return json.loads(path.read_text()).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            return None
+    return None
+
+
+def main() -> int:
+    p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--impl", default="fft_ola")
+    p.add_argument("--workload", default="fir_long", choices=["fir_long", "fir_plain", "channelize"])
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = p.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: the hot path runs on one GPU; multi-GPU runs are not part of this "
+              "library. Use --gpus 1.", file=sys.stderr)
+        return 2
+    rf = load_rfhip()
+    if args.workload == "channelize":
+        out = bench_channelize(rf, args)
+    else:
+        out = bench_fir(rf, args, args.workload == "fir_long")
+    line = {"metric": out.pop("metric"), "value": out.pop("value"), "unit": out.pop("unit"),
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": out.pop("ms_per_step"), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": out.pop("dtype"), "data": "synthetic",
+            "config": out.pop("config")}
+    line.update(out)
+    line["gpu"] = rf.device_name()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

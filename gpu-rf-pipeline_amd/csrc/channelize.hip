@@ -1,0 +1,293 @@
+// Polyphase channelizer for gfx950 (rf_channelize).
+//
+// Definition (src/channelize.cu:3-11, tools/reference.py:24-30,128-146):
+//   u[t][m] = sum_p h[pM + m] x[tM - pM - m]               (branch sums, x[<0] = 0)
+//   z[t][k] = sum_m u[t][m] exp(+2 pi i k m / M)          (unnormalised inverse DFT)
+//
+// Fused path (M a power of two, 8 <= M <= 4096): a workgroup of 512 lanes owns 4096 / M
+// consecutive frames. It stages the samples those frames reach back to in LDS, computes the
+// branch sums (for M <= 512 each lane owns one branch and eight consecutive frames and slides a
+// register window along the branch, reading each staged sample once), writes u[t][.] into the
+// same LDS, runs the M-point inverse FFTs there and stores z with coalesced writes. x is read
+// from HBM once (plus the (P-1)M-sample history per workgroup) and z written once.
+// Other M: a branch-sum kernel and the plan's batched inverse FFT, as the reference does.
+
+#include "fft_plan.hpp"
+#include "twiddle.hpp"
+
+#include <string>
+
+namespace rf {
+namespace {
+
+__device__ __forceinline__ float2 load_sample(const float2* __restrict__ x, long long m, int n)
+{
+    return (m >= 0 && m < n) ? x[m] : make_float2(0.0f, 0.0f);
+}
+
+// One lane per (frame, branch), as src/channelize.cu:25-49.
+__global__ void channelize_branch_kernel(const float2* __restrict__ x,
+                                         const float* __restrict__ taps, float2* __restrict__ u,
+                                         int channels, int num_taps, long long frames)
+{
+    const long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= frames * channels) {
+        return;
+    }
+    const long long t = idx / channels;
+    const int m = static_cast<int>(idx - t * channels);
+    const long long newest = t * channels;
+    float re = 0.0f;
+    float im = 0.0f;
+    for (int l = m; l < num_taps; l += channels) {
+        const long long s = newest - l;
+        if (s < 0) {
+            break;
+        }
+        const float h = taps[l];
+        const float2 v = x[s];
+        re = fmaf(h, v.x, re);
+        im = fmaf(h, v.y, im);
+    }
+    u[idx] = make_float2(re, im);
+}
+
+constexpr int kChanThreads = 512;
+constexpr int kChanOutputs = 4096;  // frames * M per workgroup
+constexpr int kChanMaxWindowTaps = 16;
+
+// Branch sums of one lane for M <= 512: branch m, frames 8g .. 8g+7 of the workgroup.
+// PB (1..16) taps per branch as a compile-time window; lds holds
+// x[t0 M - pb M + 1 + i].
+template <int PB>
+__device__ __forceinline__ void branch_window(const float2* lds, const float* __restrict__ taps,
+                                              int num_taps, int M, int m, int g, float2* acc)
+{
+    float h[PB];
+#pragma unroll
+    for (int p = 0; p < PB; ++p) {
+        const int l = p * M + m;
+        h[p] = l < num_taps ? taps[l] : 0.0f;
+    }
+    constexpr int W = 8 + PB - 1;
+    float2 w[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+        w[q] = lds[(8 * g + 1 + q) * M - 1 - m];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        float re = 0.0f;
+        float im = 0.0f;
+#pragma unroll
+        for (int p = 0; p < PB; ++p) {
+            const float2 v = w[j - p + PB - 1];
+            re = fmaf(h[p], v.x, re);
+            im = fmaf(h[p], v.y, im);
+        }
+        acc[j] = make_float2(re, im);
+    }
+}
+
+// Generic branch sum from the staged samples: output (f, m) of the workgroup.
+__device__ __forceinline__ float2 branch_generic(const float2* lds, const float* __restrict__ taps,
+                                                 int num_taps, int M, int pb, int f, int m)
+{
+    float re = 0.0f;
+    float im = 0.0f;
+    for (int p = 0; p < pb; ++p) {
+        const int l = p * M + m;
+        const float h = l < num_taps ? taps[l] : 0.0f;
+        const float2 v = lds[(f - p + pb) * M - 1 - m];
+        re = fmaf(h, v.x, re);
+        im = fmaf(h, v.y, im);
+    }
+    return make_float2(re, im);
+}
+
+template <int PB>
+__global__ __launch_bounds__(kChanThreads) void channelize_fused_kernel(
+    const float2* __restrict__ x, const float* __restrict__ taps, float2* __restrict__ out,
+    int n, int log2M, int num_taps, long long frames, const float2* __restrict__ table)
+{
+    extern __shared__ float2 lds[];
+    const int M = 1 << log2M;
+    const int F = kChanOutputs >> log2M;
+    const int pb = PB ? PB : (num_taps + M - 1) / M;
+    const long long t0 = static_cast<long long>(blockIdx.x) * F;
+    const long long base = t0 * M - static_cast<long long>(pb) * M + 1;
+    const int len = (F + pb - 1) * M;
+    for (int i = threadIdx.x; i < len; i += kChanThreads) {
+        lds[i] = load_sample(x, base + i, n);
+    }
+    __syncthreads();
+
+    float2 acc[8];
+    const bool windowed = PB != 0 && M <= kChanThreads;
+    if (windowed) {
+        const int m = threadIdx.x & (M - 1);
+        const int g = threadIdx.x >> log2M;
+        branch_window<PB ? PB : 1>(lds, taps, num_taps, M, m, g, acc);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int e = threadIdx.x + j * kChanThreads;
+            acc[j] = branch_generic(lds, taps, num_taps, M, pb, e >> log2M, e & (M - 1));
+        }
+    }
+    __syncthreads();  // the staged samples are dead; the frames reuse the LDS
+
+    const int slots = lds_slots(M);
+    if (windowed) {
+        const int m = threadIdx.x & (M - 1);
+        const int g = threadIdx.x >> log2M;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            lds[(8 * g + j) * slots + lds_pad(m)] = acc[j];
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int e = threadIdx.x + j * kChanThreads;
+            lds[(e >> log2M) * slots + lds_pad(e & (M - 1))] = acc[j];
+        }
+    }
+    __syncthreads();
+    const long long remaining = frames - t0;
+    const int count = remaining < F ? static_cast<int>(remaining) : F;
+    const int per = M >> 3;
+    const int f = threadIdx.x / per;
+    const int t = threadIdx.x - f * per;
+    lds_fft<true>(lds + f * slots, log2M, t, f < count, table);
+    const int total = count * M;
+    float2* dst = out + t0 * M;
+    for (int e = threadIdx.x; e < total; e += kChanThreads) {
+        dst[e] = lds[(e >> log2M) * slots + lds_pad(e & (M - 1))];
+    }
+}
+
+std::size_t fused_lds_bytes(int log2M, int pb)
+{
+    const int M = 1 << log2M;
+    const int F = kChanOutputs >> log2M;
+    const std::size_t stage = static_cast<std::size_t>(F + pb - 1) * M;
+    const std::size_t frames = static_cast<std::size_t>(F) * lds_slots(M);
+    return (stage > frames ? stage : frames) * sizeof(float2);
+}
+
+template <int PB>
+void launch_fused(const float2* x, const float* taps, float2* out, int n, int log2M,
+                  int num_taps, long long frames, std::size_t bytes, hipStream_t stream)
+{
+    const int F = kChanOutputs >> log2M;
+    const unsigned grid = static_cast<unsigned>(ceil_div(frames, F));
+    auto kernel = channelize_fused_kernel<PB>;
+    if (bytes > 64 * 1024
+        && hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               static_cast<int>(bytes)) != hipSuccess) {
+        (void)hipGetLastError();  // not needed by this runtime; the launch reports problems
+    }
+    kernel<<<grid, kChanThreads, bytes, stream>>>(x, taps, out, n, log2M, num_taps, frames,
+                                                  twiddle_table_4096());
+    check_launch("channelize_fused_kernel");
+}
+
+// Dispatch on taps per branch: a compile-time register window for 1..16, generic otherwise.
+bool try_fused(const float2* x, const float* taps, float2* out, int n, int M, int num_taps,
+               long long frames, hipStream_t stream)
+{
+    if (!is_pow2(M) || M < 8 || M > kMaxLdsFft) {
+        return false;
+    }
+    const int log2M = ilog2(M);
+    const int pb = (num_taps + M - 1) / M;
+    const std::size_t bytes = fused_lds_bytes(log2M, pb);
+    if (bytes > static_cast<std::size_t>(max_lds_per_block())) {
+        return false;
+    }
+    const bool windowed = pb <= kChanMaxWindowTaps && M <= kChanThreads;
+    if (!windowed) {
+        launch_fused<0>(x, taps, out, n, log2M, num_taps, frames, bytes, stream);
+        return true;
+    }
+    switch (pb) {
+#define RF_CHAN_CASE(P)                                                                        \
+    case P:                                                                                    \
+        launch_fused<P>(x, taps, out, n, log2M, num_taps, frames, bytes, stream);              \
+        break;
+        RF_CHAN_CASE(1) RF_CHAN_CASE(2) RF_CHAN_CASE(3) RF_CHAN_CASE(4)
+        RF_CHAN_CASE(5) RF_CHAN_CASE(6) RF_CHAN_CASE(7) RF_CHAN_CASE(8)
+        RF_CHAN_CASE(9) RF_CHAN_CASE(10) RF_CHAN_CASE(11) RF_CHAN_CASE(12)
+        RF_CHAN_CASE(13) RF_CHAN_CASE(14) RF_CHAN_CASE(15) RF_CHAN_CASE(16)
+#undef RF_CHAN_CASE
+    default:
+        launch_fused<0>(x, taps, out, n, log2M, num_taps, frames, bytes, stream);
+    }
+    return true;
+}
+
+}  // namespace
+}  // namespace rf
+
+using rf::guarded;
+
+extern "C" {
+
+int rf_channelize_frames(int num_samples, int channels)
+{
+    if (channels < 1 || num_samples < 0) {
+        return 0;
+    }
+    return static_cast<int>((static_cast<long long>(num_samples) + channels - 1) / channels);
+}
+
+rf_status rf_channelize(const rf_fft_plan* plan, const rf_cf32* x, const float* taps,
+                        rf_cf32* out, int num_samples, int channels, int num_taps,
+                        rf_stream stream)
+{
+    return guarded([&] {
+        if (channels < 1) rf::fail(RF_INVALID_ARG, "channelize: need at least one channel");
+        if (num_taps < 1) rf::fail(RF_INVALID_ARG, "channelize: need at least one tap");
+        if (num_samples < 0) rf::fail(RF_INVALID_ARG, "channelize: negative sample count");
+        const long long frames = rf_channelize_frames(num_samples, channels);
+        if (frames == 0) {
+            return;
+        }
+        if (x == nullptr || taps == nullptr || out == nullptr) {
+            rf::fail(RF_INVALID_ARG, "channelize: NULL buffer");
+        }
+        if (plan != nullptr && (plan->nfft != channels || plan->batch != frames)) {
+            rf::fail(RF_INVALID_ARG, "channelize: the plan must be channels points x frames");
+        }
+        hipStream_t s = rf::as_stream(stream);
+        const float2* fx = reinterpret_cast<const float2*>(x);
+        float2* fo = reinterpret_cast<float2*>(out);
+        if (rf::try_fused(fx, taps, fo, num_samples, channels, num_taps, frames, s)) {
+            return;
+        }
+        const long long total = frames * channels;
+        rf::channelize_branch_kernel<<<static_cast<unsigned>(rf::ceil_div(total, 256)), 256, 0,
+                                       s>>>(fx, taps, fo, channels, num_taps, frames);
+        rf::check_launch("channelize_branch_kernel");
+        if (plan != nullptr) {
+            rf::fft_exec(*plan, fo, fo, RF_FFT_INVERSE, s);
+            return;
+        }
+        rf_fft_plan* own = nullptr;
+        if (rf_fft_plan_create(channels, static_cast<int>(frames), &own) != RF_OK) {
+            rf::fail(RF_ALLOC, std::string("channelize: ") + rf_last_error());
+        }
+        try {
+            rf::fft_exec(*own, fo, fo, RF_FFT_INVERSE, s);
+        } catch (...) {
+            rf_fft_plan_destroy(own);
+            throw;
+        }
+        // The plan's buffers must outlive the enqueued transform.
+        RF_HIP_CHECK(hipStreamSynchronize(s));
+        rf_fft_plan_destroy(own);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,120 @@
+// Kernels shared by every user of the library's FFT: batched single-workgroup transforms
+// (P <= 4096) and the two passes of the four-step transform for longer ones.
+//
+// Four-step (Bailey 1990) with L = n1 * n2, n = j1 + n1 * j2, k = k2 + n2 * k1:
+//   column pass: for each j1, an n2-point FFT over j2 (stride n1), times w_L^(j1 k2),
+//                stored at j1 + n1 * k2;
+//   row pass:    for each k2, an n1-point FFT over j1 (contiguous), giving X[k2 + n2 k1] at
+//                k1 + n1 * k2 ("transposed order").
+// The inverse runs the same passes in the opposite order with conjugate twiddles, taking the
+// transposed order back to natural order. Cross-correlation never needs the spectrum in
+// natural order, so it multiplies spectra in transposed order and skips the transposes.
+#pragma once
+
+#include "fft_core.hpp"
+
+namespace rf {
+
+// Split of a power-of-two length 4096 < L <= 2^24: n2 = max(8, L / 4096) rows of n1 points.
+struct FourStep {
+    int log2L;
+    int log2n1;
+    int log2n2;
+    const float2* lo;  // exp(-2 pi i m / L), m < 4096
+    const float2* hi;  // exp(-2 pi i 4096 m / L), m < L / 4096 (at least 1 entry)
+};
+
+inline FourStep make_four_step_shape(int log2L)
+{
+    FourStep fs{};
+    fs.log2L = log2L;
+    fs.log2n2 = log2L - kTwiddleLog2 > 3 ? log2L - kTwiddleLog2 : 3;
+    fs.log2n1 = log2L - fs.log2n2;
+    return fs;
+}
+
+// w_L^(+-m) for m < L.
+template <bool INV>
+__device__ __forceinline__ float2 four_step_twiddle(const FourStep& fs, long long m)
+{
+    const float2 w = cmul(fs.lo[m & (kTwiddleSize - 1)], fs.hi[m >> kTwiddleLog2]);
+    return INV ? make_float2(w.x, -w.y) : w;
+}
+
+// Columns per workgroup of the column pass: C * n2 = 4096 values (512 lanes), or all n1
+// columns if there are fewer.
+__host__ __device__ __forceinline__ int column_tile(const FourStep& fs)
+{
+    const int c = 1 << (kTwiddleLog2 - fs.log2n2);
+    const int n1 = 1 << fs.log2n1;
+    return c < n1 ? c : n1;
+}
+
+// Column pass over one matrix: workgroup handles columns [c0, c0 + C). `load(j1, j2)` gives
+// the input at n = j1 + n1 j2; `store(j1, k2, v)` takes the twiddled result. 512 lanes.
+template <bool INV, class Load, class Store>
+__device__ __forceinline__ void column_pass_block(const FourStep& fs, long long c0, Load load,
+                                                  Store store, float2* lds,
+                                                  const float2* __restrict__ table)
+{
+    const int n2 = 1 << fs.log2n2;
+    const int C = column_tile(fs);
+    const int slots = lds_slots(n2);
+    const int total = C * n2;
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        const int row = e / C;
+        const int col = e - row * C;
+        lds[col * slots + lds_pad(row)] = load(c0 + col, row);
+    }
+    __syncthreads();
+    const int per = n2 >> 3;  // lanes per column transform
+    const int f = threadIdx.x / per;
+    const int t = threadIdx.x - f * per;
+    lds_fft<INV>(lds + (f < C ? f : 0) * slots, fs.log2n2, t, f < C, table);
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        const int k2 = e / C;
+        const int col = e - k2 * C;
+        float2 v = lds[col * slots + lds_pad(k2)];
+        v = cmul(v, four_step_twiddle<INV>(fs, (c0 + col) * static_cast<long long>(k2)));
+        store(c0 + col, k2, v);
+    }
+}
+
+// Batched single-workgroup transform: F = max(1, 512 / (P / 8)) transforms of P = 2^log2P
+// (8 <= P <= 4096) points per workgroup, contiguous in memory, 512 lanes at most.
+__host__ __device__ __forceinline__ int lds_batch_per_block(int log2P)
+{
+    const int per = 1 << (log2P - 3);
+    return per >= 512 ? 1 : 512 / per;
+}
+
+template <bool INV>
+__global__ __launch_bounds__(512) void fft_lds_batch_kernel(const float2* in, float2* out,
+                                                            int log2P, long long batch,
+                                                            float scale,
+                                                            const float2* __restrict__ table)
+{
+    extern __shared__ float2 lds[];
+    const int P = 1 << log2P;
+    const int F = lds_batch_per_block(log2P);
+    const int slots = lds_slots(P);
+    const long long first = static_cast<long long>(blockIdx.x) * F;
+    const int count = static_cast<int>(batch - first < F ? batch - first : F);
+    const long long base = first * P;
+    const int total = count * P;
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        const int f = e >> log2P;
+        lds[f * slots + lds_pad(e & (P - 1))] = in[base + e];
+    }
+    __syncthreads();
+    const int per = P >> 3;
+    const int f = threadIdx.x / per;
+    const int t = threadIdx.x - f * per;
+    lds_fft<INV>(lds + (f < count ? f : 0) * slots, log2P, t, f < count, table);
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        const int fe = e >> log2P;
+        out[base + e] = cscale(lds[fe * slots + lds_pad(e & (P - 1))], scale);
+    }
+}
+
+}  // namespace rf

@@ -1,0 +1,794 @@
+// FIR filter kernels for gfx950 and the rf_fir / rf_fir_batched entry points.
+//
+// Definition (src/fir.cu:1-5, tools/reference.py:15-17,103-111):
+//   y[n] = sum_{k=0}^{T-1} taps[k] * x[n - k],  x[m] = 0 for m < 0,  output length N.
+//
+// The direct variants (naive, tiled, tiled_const, register) all accumulate
+//   acc = fma(taps[k], x[n-k], acc)  for k = 0, 1, ..., T-1, starting from 0
+// per output and per component, so they give identical bits (up to the sign of a zero). The
+// reference's variant table is kept (src/fir.cu:725-734) with its launch semantics; the
+// kernels themselves are laid out for 64-lane wavefronts:
+//   - tiles are filled with 16-byte loads (two samples per lane),
+//   - tap reads are wave-uniform and go through the scalar cache (s_load), which is what the
+//     reference's __constant__ bank is for, without any program-wide tap state,
+//   - the register variant keeps R = 4, 8 or 16 outputs per lane and walks the taps R at a
+//     time, reading one 8R-byte window per R taps from an LDS tile padded by 16 bytes per
+//     R samples (conflict-free 16-byte LDS reads).
+// fft_ola is new: overlap-save with a 4096-point LDS FFT for long filters.
+
+#include "fft_core.hpp"
+#include "rf_common.hpp"
+#include "twiddle.hpp"
+
+#include <cstring>
+#include <string>
+
+namespace rf {
+namespace {
+
+// Pointer into the constant address space: wave-uniform loads through it become s_load
+// (scalar data cache), the AMD counterpart of the reference's constant-memory taps.
+typedef const __attribute__((address_space(4))) float* const_taps_ptr;
+
+__device__ __forceinline__ const_taps_ptr as_const_taps(const float* p)
+{
+    return (const_taps_ptr)(p);
+}
+
+__device__ __forceinline__ float2 load_sample(const float2* __restrict__ x, long long m, int n)
+{
+    return (m >= 0 && m < n) ? x[m] : make_float2(0.0f, 0.0f);
+}
+
+// --- naive ------------------------------------------------------------------------------
+
+// One lane per output; taps and samples from global memory (src/fir.cu:20-43).
+__device__ __forceinline__ void fir_naive_one(const float2* __restrict__ x,
+                                              const float* __restrict__ taps,
+                                              float2* __restrict__ y, int n, int num_taps,
+                                              long long i)
+{
+    if (i >= n) {
+        return;
+    }
+    const int last = i < num_taps - 1 ? static_cast<int>(i) : num_taps - 1;
+    float re = 0.0f;
+    float im = 0.0f;
+    for (int k = 0; k <= last; ++k) {
+        const float h = taps[k];
+        const float2 s = x[i - k];
+        re = fmaf(h, s.x, re);
+        im = fmaf(h, s.y, im);
+    }
+    y[i] = make_float2(re, im);
+}
+
+__global__ void fir_naive_kernel(const float2* __restrict__ x, const float* __restrict__ taps,
+                                 float2* __restrict__ y, int n, int num_taps)
+{
+    fir_naive_one(x, taps, y, n, num_taps,
+                  static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x);
+}
+
+// --- tiled / tiled_const ------------------------------------------------------------------
+
+// Tile of halo + blockDim.x samples: tile[i] = x[block_start - halo + i], zero outside the
+// input (src/fir.cu:89-99). Filled two samples per lane per step.
+__device__ __forceinline__ void fill_tile(const float2* __restrict__ x, int n, long long start,
+                                          int len, float2* tile)
+{
+    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+        tile[i] = load_sample(x, start + i, n);
+    }
+}
+
+template <bool CONST_TAPS>
+__device__ __forceinline__ void fir_tiled_block(const float2* __restrict__ x,
+                                                const float* __restrict__ taps,
+                                                float2* __restrict__ y, int n, int num_taps,
+                                                float2* tile)
+{
+    const int halo = num_taps - 1;
+    const long long block_start = static_cast<long long>(blockIdx.x) * blockDim.x;
+    fill_tile(x, n, block_start - halo, halo + static_cast<int>(blockDim.x), tile);
+    __syncthreads();
+    const long long i = block_start + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const int newest = halo + threadIdx.x;
+    float re = 0.0f;
+    float im = 0.0f;
+    if (CONST_TAPS) {
+        const const_taps_ptr ct = as_const_taps(taps);
+        for (int k = 0; k < num_taps; ++k) {
+            const float h = ct[k];
+            const float2 s = tile[newest - k];
+            re = fmaf(h, s.x, re);
+            im = fmaf(h, s.y, im);
+        }
+    } else {
+        for (int k = 0; k < num_taps; ++k) {
+            const float h = taps[k];
+            const float2 s = tile[newest - k];
+            re = fmaf(h, s.x, re);
+            im = fmaf(h, s.y, im);
+        }
+    }
+    y[i] = make_float2(re, im);
+}
+
+template <bool CONST_TAPS>
+__global__ void fir_tiled_kernel(const float2* __restrict__ x, const float* __restrict__ taps,
+                                 float2* __restrict__ y, int n, int num_taps)
+{
+    extern __shared__ float2 tile[];
+    fir_tiled_block<CONST_TAPS>(x, taps, y, n, num_taps, tile);
+}
+
+// --- register -----------------------------------------------------------------------------
+
+// LDS slot of tile sample s: groups of R samples, each followed by two pad slots (16 bytes),
+// so group g starts at g * (R + 2) and stays 16-byte aligned.
+template <int R>
+__device__ __forceinline__ int reg_slot(int s)
+{
+    return (s / R) * (R + 2) + (s % R);
+}
+
+// R outputs per lane. The tile holds x[block_start - Tpad .. block_start + blockDim.x * R)
+// with Tpad = ceil(T / R) * R, so lane l's window for taps [cR, cR + R) is group
+// (Tpad/R + l - c) ("cur") and the group before it ("prev"):
+//   x[first + j - k] = (j >= i) ? cur[j - i] : prev[R + j - i],   k = cR + i.
+template <int R>
+__device__ __forceinline__ void fir_register_block(const float2* __restrict__ x,
+                                                   const float* __restrict__ taps,
+                                                   float2* __restrict__ y, int n, int num_taps,
+                                                   float2* tile)
+{
+    const int outputs = static_cast<int>(blockDim.x) * R;
+    const int chunks = (num_taps + R - 1) / R;
+    const int tpad = chunks * R;
+    const int len = tpad + outputs;  // a multiple of R, so even
+    const long long block_start = static_cast<long long>(blockIdx.x) * outputs;
+    const long long tile_start = block_start - tpad;
+
+    for (int s = 2 * threadIdx.x; s < len; s += 2 * blockDim.x) {
+        const long long m = tile_start + s;
+        float4 v;
+        if (m >= 0 && m + 1 < n) {
+            const float2 a = x[m];
+            const float2 b = x[m + 1];
+            v = make_float4(a.x, a.y, b.x, b.y);
+        } else {
+            const float2 a = load_sample(x, m, n);
+            const float2 b = load_sample(x, m + 1, n);
+            v = make_float4(a.x, a.y, b.x, b.y);
+        }
+        *reinterpret_cast<float4*>(&tile[reg_slot<R>(s)]) = v;
+    }
+    __syncthreads();
+
+    const long long first = block_start + static_cast<long long>(threadIdx.x) * R;
+    if (first >= n) {
+        return;
+    }
+    const const_taps_ptr ct = as_const_taps(taps);
+    float2 cur[R];
+    float2 prev[R];
+    float2 acc[R];
+    {
+        const float4* g = reinterpret_cast<const float4*>(tile + (chunks + threadIdx.x) * (R + 2));
+#pragma unroll
+        for (int q = 0; q < R / 2; ++q) {
+            const float4 v = g[q];
+            cur[2 * q] = make_float2(v.x, v.y);
+            cur[2 * q + 1] = make_float2(v.z, v.w);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        acc[j] = make_float2(0.0f, 0.0f);
+    }
+    const int full = num_taps / R;  // chunks whose R taps all exist
+    for (int c = 0; c < chunks; ++c) {
+        {
+            const float4* g =
+                reinterpret_cast<const float4*>(tile + (chunks + threadIdx.x - c - 1) * (R + 2));
+#pragma unroll
+            for (int q = 0; q < R / 2; ++q) {
+                const float4 v = g[q];
+                prev[2 * q] = make_float2(v.x, v.y);
+                prev[2 * q + 1] = make_float2(v.z, v.w);
+            }
+        }
+        float h[R];
+        if (c < full) {
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                h[i] = ct[c * R + i];
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                const int k = c * R + i;
+                h[i] = k < num_taps ? ct[k] : 0.0f;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const float2 v = (j >= i) ? cur[j - i] : prev[R + j - i];
+                acc[j].x = fmaf(h[i], v.x, acc[j].x);
+                acc[j].y = fmaf(h[i], v.y, acc[j].y);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            cur[j] = prev[j];
+        }
+    }
+    const bool aligned = (reinterpret_cast<uintptr_t>(y) & 15) == 0;
+    if (aligned && first + R <= n) {
+#pragma unroll
+        for (int q = 0; q < R / 2; ++q) {
+            *reinterpret_cast<float4*>(&y[first + 2 * q]) =
+                make_float4(acc[2 * q].x, acc[2 * q].y, acc[2 * q + 1].x, acc[2 * q + 1].y);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            if (first + j < n) {
+                y[first + j] = acc[j];
+            }
+        }
+    }
+}
+
+template <int R>
+__global__ void fir_register_kernel(const float2* __restrict__ x, const float* __restrict__ taps,
+                                    float2* __restrict__ y, int n, int num_taps)
+{
+    extern __shared__ float2 tile[];
+    fir_register_block<R>(x, taps, y, n, num_taps, tile);
+}
+
+// --- batched launches (src/fir_batch.hpp:9-13, src/fused.cu:36-74, src/fir.cu:562-721) ----
+
+struct FirBatch {
+    const float2* x[RF_MAX_FIR_BATCH];
+    float2* y[RF_MAX_FIR_BATCH];
+    int n[RF_MAX_FIR_BATCH];
+};
+
+__global__ void fir_naive_batched_kernel(FirBatch batch, const float* __restrict__ taps,
+                                         int num_taps)
+{
+    const int c = blockIdx.y;
+    fir_naive_one(batch.x[c], taps, batch.y[c], batch.n[c], num_taps,
+                  static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x);
+}
+
+template <bool CONST_TAPS>
+__global__ void fir_tiled_batched_kernel(FirBatch batch, const float* __restrict__ taps,
+                                         int num_taps)
+{
+    extern __shared__ float2 tile[];
+    const int c = blockIdx.y;
+    // A block wholly past its channel's end returns before the barrier; every lane of it
+    // does, so the barrier is never split.
+    if (static_cast<long long>(blockIdx.x) * blockDim.x >= batch.n[c]) {
+        return;
+    }
+    fir_tiled_block<CONST_TAPS>(batch.x[c], taps, batch.y[c], batch.n[c], num_taps, tile);
+}
+
+template <int R>
+__global__ void fir_register_batched_kernel(FirBatch batch, const float* __restrict__ taps,
+                                            int num_taps)
+{
+    extern __shared__ float2 tile[];
+    const int c = blockIdx.y;
+    if (static_cast<long long>(blockIdx.x) * blockDim.x * R >= batch.n[c]) {
+        return;
+    }
+    fir_register_block<R>(batch.x[c], taps, batch.y[c], batch.n[c], num_taps, tile);
+}
+
+// --- fft_ola --------------------------------------------------------------------------------
+
+// Overlap-save: with P = 4096 and Q = P - (T - 1) outputs per block,
+//   seg[i] = x[bQ - (T-1) + i],  i < P (zero outside the input)
+//   y[bQ + j] = IFFT(FFT(seg) * FFT(h)) [T - 1 + j] / P,  j < Q.
+// 512 lanes per block, one radix-8 butterfly per lane per pass (4096 = 8^4). The first forward
+// pass reads x straight from HBM and the last inverse pass writes y straight back; the
+// spectrum multiply sits between the last forward and the first inverse pass, in registers.
+constexpr int kOlaLog2 = 12;
+constexpr int kOlaP = 1 << kOlaLog2;
+constexpr int kOlaThreads = kOlaP / 8;
+constexpr int kOlaMinTaps = 65;   // at or below this, the direct register kernel is cheaper
+constexpr int kOlaMaxTaps = 2049; // keeps Q >= P / 2
+
+// A radix-8 Stockham pass of the 4096-point transform through LDS (butterfly t, span p).
+template <bool INV>
+__device__ __forceinline__ void ola_lds_pass(float2* buf, int t, int p,
+                                             const float2* __restrict__ table)
+{
+    float2 u[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        u[r] = buf[lds_pad(t + r * kOlaThreads)];
+    }
+    __syncthreads();
+    const int k = t & (p - 1);
+    twiddle_and_dft<INV>(u, 3, k, table, kTwiddleSize / (p * 8));
+    const int j = (t - k) * 8 + k;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        buf[lds_pad(j + r * p)] = u[r];
+    }
+    __syncthreads();
+}
+
+// The two ends of the 4096-point transform that never touch LDS: the first pass (span 1, no
+// twiddles) takes inputs t + 512 r and writes 8t + r; the last pass (span 512) reads t + 512 r
+// and leaves outputs t + 512 r in the lane's registers.
+template <bool INV>
+__device__ __forceinline__ void ola_first_pass(float2* buf, int t, float2* u)
+{
+    if (INV) {
+        dft8<true>(u);
+    } else {
+        dft8<false>(u);
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        buf[lds_pad(8 * t + r)] = u[r];
+    }
+    __syncthreads();
+}
+
+template <bool INV>
+__device__ __forceinline__ void ola_last_pass(const float2* buf, int t, float2* u,
+                                              const float2* __restrict__ table)
+{
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        u[r] = buf[lds_pad(t + r * kOlaThreads)];
+    }
+    twiddle_and_dft<INV>(u, 3, t, table, 1);
+}
+
+__global__ __launch_bounds__(kOlaThreads) void fir_ola_taps_kernel(
+    const float* __restrict__ taps, int num_taps, float2* __restrict__ spec,
+    const float2* __restrict__ table)
+{
+    __shared__ float2 buf[lds_slots(kOlaP)];
+    const int t = threadIdx.x;
+    const float scale = 1.0f / static_cast<float>(kOlaP);  // exact: a power of two
+    for (int i = t; i < kOlaP; i += kOlaThreads) {
+        buf[lds_pad(i)] = make_float2(i < num_taps ? taps[i] * scale : 0.0f, 0.0f);
+    }
+    __syncthreads();
+    lds_fft<false>(buf, kOlaLog2, t, true, table);
+    for (int i = t; i < kOlaP; i += kOlaThreads) {
+        spec[i] = buf[lds_pad(i)];
+    }
+}
+
+__device__ __forceinline__ void fir_ola_block(const float2* __restrict__ x,
+                                              const float2* __restrict__ spec,
+                                              float2* __restrict__ y, int n, int num_taps,
+                                              long long block, float2* buf,
+                                              const float2* __restrict__ table)
+{
+    const int halo = num_taps - 1;
+    const int q = kOlaP - halo;
+    const long long out0 = block * q;
+    const long long in0 = out0 - halo;
+    const int t = threadIdx.x;
+    float2 u[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        u[r] = load_sample(x, in0 + t + r * kOlaThreads, n);
+    }
+    ola_first_pass<false>(buf, t, u);
+    ola_lds_pass<false>(buf, t, 8, table);
+    ola_lds_pass<false>(buf, t, 64, table);
+    ola_last_pass<false>(buf, t, u, table);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        u[r] = cmul(u[r], spec[t + r * kOlaThreads]);
+    }
+    __syncthreads();  // every lane has read the forward result before it is overwritten
+    ola_first_pass<true>(buf, t, u);
+    ola_lds_pass<true>(buf, t, 8, table);
+    ola_lds_pass<true>(buf, t, 64, table);
+    ola_last_pass<true>(buf, t, u, table);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int pos = t + r * kOlaThreads;
+        const long long o = out0 + pos - halo;
+        if (pos >= halo && o < n) {
+            y[o] = u[r];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kOlaThreads) void fir_ola_kernel(const float2* __restrict__ x,
+                                                              const float2* __restrict__ spec,
+                                                              float2* __restrict__ y, int n,
+                                                              int num_taps,
+                                                              const float2* __restrict__ table)
+{
+    __shared__ float2 buf[lds_slots(kOlaP)];
+    fir_ola_block(x, spec, y, n, num_taps, blockIdx.x, buf, table);
+}
+
+__global__ __launch_bounds__(kOlaThreads) void fir_ola_batched_kernel(
+    FirBatch batch, const float2* __restrict__ spec, int num_taps,
+    const float2* __restrict__ table)
+{
+    __shared__ float2 buf[lds_slots(kOlaP)];
+    const int c = blockIdx.y;
+    const long long q = kOlaP - (num_taps - 1);
+    if (static_cast<long long>(blockIdx.x) * q >= batch.n[c]) {
+        return;
+    }
+    fir_ola_block(batch.x[c], spec, batch.y[c], batch.n[c], num_taps, blockIdx.x, buf, table);
+}
+
+// --- host side ------------------------------------------------------------------------------
+
+struct ImplInfo {
+    const char* name;
+    const char* summary;
+};
+
+const ImplInfo kFirImpls[RF_FIR_IMPL_COUNT] = {
+    {"naive", "one lane per output sample, global memory only"},
+    {"tiled", "samples staged in an LDS tile with a halo"},
+    {"tiled_const", "tiled, with the taps read through the scalar (constant) cache"},
+    {"register", "4, 8 or 16 outputs per lane from a register window over a padded LDS tile"},
+    {"fft_ola", "overlap-save with a 4096-point LDS FFT (register kernel for <= 65 taps)"},
+};
+
+void check_fir_args(int impl, const void* x, const float* taps, const void* y, int n,
+                    int num_taps, int block)
+{
+    if (impl < 0 || impl >= RF_FIR_IMPL_COUNT) {
+        fail(RF_INVALID_ARG, "fir: unknown implementation " + std::to_string(impl));
+    }
+    if (n < 0) {
+        fail(RF_INVALID_ARG, "fir: negative sample count");
+    }
+    if (num_taps < 1) {
+        fail(RF_INVALID_ARG, "fir: need at least one tap");
+    }
+    if (block < 32 || block > 1024 || block % 32 != 0) {
+        fail(RF_INVALID_ARG, "fir: --block needs a multiple of 32 up to 1024, got "
+                                 + std::to_string(block));
+    }
+    if (n > 0 && (x == nullptr || y == nullptr || taps == nullptr)) {
+        fail(RF_INVALID_ARG, "fir: NULL buffer");
+    }
+}
+
+void check_const_taps(int num_taps, const char* impl)
+{
+    if (num_taps > RF_MAX_CONST_TAPS) {
+        fail(RF_LIMIT, std::string("fir ") + impl + ": " + std::to_string(num_taps)
+                           + " taps, more than the " + std::to_string(RF_MAX_CONST_TAPS)
+                           + " the constant-cache variants accept; use --impl naive");
+    }
+}
+
+std::size_t tiled_bytes(int num_taps, int block, const char* impl)
+{
+    const std::size_t bytes = (static_cast<std::size_t>(num_taps) - 1 + block) * sizeof(float2);
+    const int limit = max_lds_per_block();
+    if (bytes > static_cast<std::size_t>(limit)) {
+        fail(RF_LIMIT, std::string("fir ") + impl + ": " + std::to_string(num_taps)
+                           + " taps need " + std::to_string(bytes)
+                           + " bytes of LDS per workgroup, more than the "
+                           + std::to_string(limit) + " available; use --impl naive");
+    }
+    return bytes;
+}
+
+std::size_t register_bytes_for(int r, int num_taps, int block)
+{
+    const long long tpad = ceil_div(num_taps, r) * r;
+    const long long len = tpad + static_cast<long long>(block) * r;
+    return static_cast<std::size_t>(len / r * (r + 2)) * sizeof(float2);
+}
+
+// Outputs per lane: 16 when the launch still has a few blocks per CU, else 8 or 4; and the
+// largest of those whose tile fits in LDS.
+int register_outputs(int num_taps, int block, long long longest, std::size_t* bytes)
+{
+    const int limit = max_lds_per_block();
+    const long long want_blocks = 2LL * cu_count();
+    const int candidates[3] = {16, 8, 4};
+    int chosen = 0;
+    for (int r : candidates) {
+        if (register_bytes_for(r, num_taps, block) > static_cast<std::size_t>(limit)) {
+            continue;
+        }
+        chosen = r;
+        if (ceil_div(longest, static_cast<long long>(block) * r) >= want_blocks) {
+            break;
+        }
+    }
+    if (chosen == 0) {
+        fail(RF_LIMIT, "fir register: " + std::to_string(num_taps) + " taps need "
+                           + std::to_string(register_bytes_for(4, num_taps, block))
+                           + " bytes of LDS per workgroup, more than the "
+                           + std::to_string(limit) + " available; use --impl naive");
+    }
+    *bytes = register_bytes_for(chosen, num_taps, block);
+    return chosen;
+}
+
+// Raises a kernel's dynamic-LDS limit above 64 KiB where the runtime wants that; a runtime
+// that does not need it may refuse the call, and the launch itself then reports any problem.
+template <class Kernel>
+void allow_lds(Kernel kernel, std::size_t bytes)
+{
+    if (bytes > 64 * 1024) {
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                static_cast<int>(bytes)) != hipSuccess) {
+            (void)hipGetLastError();
+        }
+    }
+}
+
+void launch_register(const float2* x, const float* taps, float2* y, int n, int num_taps,
+                     int block, hipStream_t stream)
+{
+    std::size_t bytes = 0;
+    const int r = register_outputs(num_taps, block, n, &bytes);
+    const unsigned grid = static_cast<unsigned>(ceil_div(n, static_cast<long long>(block) * r));
+    if (r == 16) {
+        allow_lds(fir_register_kernel<16>, bytes);
+        fir_register_kernel<16><<<grid, block, bytes, stream>>>(x, taps, y, n, num_taps);
+    } else if (r == 8) {
+        allow_lds(fir_register_kernel<8>, bytes);
+        fir_register_kernel<8><<<grid, block, bytes, stream>>>(x, taps, y, n, num_taps);
+    } else {
+        allow_lds(fir_register_kernel<4>, bytes);
+        fir_register_kernel<4><<<grid, block, bytes, stream>>>(x, taps, y, n, num_taps);
+    }
+    check_launch("fir_register_kernel");
+}
+
+bool ola_applies(int num_taps)
+{
+    return num_taps > kOlaMinTaps && num_taps <= kOlaMaxTaps;
+}
+
+// Stream-ordered workspace for the tap spectrum of fft_ola: allocated and released on the
+// caller's stream, so the entry point stays enqueue-only.
+struct StreamScratch {
+    void* ptr = nullptr;
+    hipStream_t stream = nullptr;
+    StreamScratch(std::size_t bytes, hipStream_t s) : stream(s)
+    {
+        RF_HIP_CHECK(hipMallocAsync(&ptr, bytes, stream));
+    }
+    ~StreamScratch()
+    {
+        if (ptr != nullptr) {
+            (void)hipFreeAsync(ptr, stream);
+        }
+    }
+    StreamScratch(const StreamScratch&) = delete;
+    StreamScratch& operator=(const StreamScratch&) = delete;
+};
+
+void launch_ola_taps(const float* taps, int num_taps, float2* spec, const float2* table,
+                     hipStream_t stream)
+{
+    fir_ola_taps_kernel<<<1, kOlaThreads, 0, stream>>>(taps, num_taps, spec, table);
+    check_launch("fir_ola_taps_kernel");
+}
+
+void fir_launch(int impl, const float2* x, const float* taps, float2* y, int n, int num_taps,
+                int block, hipStream_t stream)
+{
+    if (n == 0) {
+        return;
+    }
+    const unsigned grid = static_cast<unsigned>(ceil_div(n, block));
+    switch (impl) {
+    case RF_FIR_NAIVE:
+        fir_naive_kernel<<<grid, block, 0, stream>>>(x, taps, y, n, num_taps);
+        check_launch("fir_naive_kernel");
+        return;
+    case RF_FIR_TILED: {
+        const std::size_t bytes = tiled_bytes(num_taps, block, "tiled");
+        allow_lds(fir_tiled_kernel<false>, bytes);
+        fir_tiled_kernel<false><<<grid, block, bytes, stream>>>(x, taps, y, n, num_taps);
+        check_launch("fir_tiled_kernel");
+        return;
+    }
+    case RF_FIR_TILED_CONST: {
+        check_const_taps(num_taps, "tiled_const");
+        const std::size_t bytes = tiled_bytes(num_taps, block, "tiled_const");
+        allow_lds(fir_tiled_kernel<true>, bytes);
+        fir_tiled_kernel<true><<<grid, block, bytes, stream>>>(x, taps, y, n, num_taps);
+        check_launch("fir_tiled_const_kernel");
+        return;
+    }
+    case RF_FIR_REGISTER:
+        check_const_taps(num_taps, "register");
+        launch_register(x, taps, y, n, num_taps, block, stream);
+        return;
+    case RF_FIR_FFT_OLA: {
+        if (!ola_applies(num_taps)) {
+            launch_register(x, taps, y, n, num_taps, block, stream);
+            return;
+        }
+        const float2* table = twiddle_table_4096();
+        StreamScratch spec(kOlaP * sizeof(float2), stream);
+        launch_ola_taps(taps, num_taps, static_cast<float2*>(spec.ptr), table, stream);
+        const long long q = kOlaP - (num_taps - 1);
+        const unsigned blocks = static_cast<unsigned>(ceil_div(n, q));
+        fir_ola_kernel<<<blocks, kOlaThreads, 0, stream>>>(
+            x, static_cast<const float2*>(spec.ptr), y, n, num_taps, table);
+        check_launch("fir_ola_kernel");
+        return;
+    }
+    }
+}
+
+void fir_batched_launch(int impl, const FirBatch& batch, int count, int longest,
+                        const float* taps, int num_taps, hipStream_t stream)
+{
+    const int block = RF_DEFAULT_FIR_BLOCK;
+    switch (impl) {
+    case RF_FIR_NAIVE: {
+        const dim3 grid(static_cast<unsigned>(ceil_div(longest, block)), count);
+        fir_naive_batched_kernel<<<grid, block, 0, stream>>>(batch, taps, num_taps);
+        check_launch("fir_naive_batched_kernel");
+        return;
+    }
+    case RF_FIR_TILED:
+    case RF_FIR_TILED_CONST: {
+        const bool ct = impl == RF_FIR_TILED_CONST;
+        if (ct) {
+            check_const_taps(num_taps, "tiled_const");
+        }
+        const std::size_t bytes = tiled_bytes(num_taps, block, ct ? "tiled_const" : "tiled");
+        const dim3 grid(static_cast<unsigned>(ceil_div(longest, block)), count);
+        if (ct) {
+            allow_lds(fir_tiled_batched_kernel<true>, bytes);
+            fir_tiled_batched_kernel<true><<<grid, block, bytes, stream>>>(batch, taps, num_taps);
+        } else {
+            allow_lds(fir_tiled_batched_kernel<false>, bytes);
+            fir_tiled_batched_kernel<false><<<grid, block, bytes, stream>>>(batch, taps,
+                                                                           num_taps);
+        }
+        check_launch("fir_tiled_batched_kernel");
+        return;
+    }
+    case RF_FIR_REGISTER:
+    case RF_FIR_FFT_OLA: {
+        if (impl == RF_FIR_FFT_OLA && ola_applies(num_taps)) {
+            const float2* table = twiddle_table_4096();
+            StreamScratch spec(kOlaP * sizeof(float2), stream);
+            launch_ola_taps(taps, num_taps, static_cast<float2*>(spec.ptr), table, stream);
+            const long long q = kOlaP - (num_taps - 1);
+            const dim3 grid(static_cast<unsigned>(ceil_div(longest, q)), count);
+            fir_ola_batched_kernel<<<grid, kOlaThreads, 0, stream>>>(
+                batch, static_cast<const float2*>(spec.ptr), num_taps, table);
+            check_launch("fir_ola_batched_kernel");
+            return;
+        }
+        if (impl == RF_FIR_REGISTER) {
+            check_const_taps(num_taps, "register");
+        }
+        std::size_t bytes = 0;
+        const int r = register_outputs(num_taps, block, longest, &bytes);
+        const dim3 grid(static_cast<unsigned>(ceil_div(longest, static_cast<long long>(block) * r)),
+                        count);
+        if (r == 16) {
+            allow_lds(fir_register_batched_kernel<16>, bytes);
+            fir_register_batched_kernel<16><<<grid, block, bytes, stream>>>(batch, taps, num_taps);
+        } else if (r == 8) {
+            allow_lds(fir_register_batched_kernel<8>, bytes);
+            fir_register_batched_kernel<8><<<grid, block, bytes, stream>>>(batch, taps, num_taps);
+        } else {
+            allow_lds(fir_register_batched_kernel<4>, bytes);
+            fir_register_batched_kernel<4><<<grid, block, bytes, stream>>>(batch, taps, num_taps);
+        }
+        check_launch("fir_register_batched_kernel");
+        return;
+    }
+    }
+}
+
+}  // namespace
+}  // namespace rf
+
+using rf::guarded;
+
+extern "C" {
+
+int rf_fir_impl_count(void)
+{
+    return RF_FIR_IMPL_COUNT;
+}
+
+const char* rf_fir_impl_name(int impl)
+{
+    return (impl >= 0 && impl < RF_FIR_IMPL_COUNT) ? rf::kFirImpls[impl].name : nullptr;
+}
+
+const char* rf_fir_impl_summary(int impl)
+{
+    return (impl >= 0 && impl < RF_FIR_IMPL_COUNT) ? rf::kFirImpls[impl].summary : nullptr;
+}
+
+rf_status rf_fir_parse_impl(const char* name, int* impl)
+{
+    return guarded([&] {
+        if (name == nullptr || impl == nullptr) {
+            rf::fail(RF_INVALID_ARG, "fir: NULL implementation name");
+        }
+        for (int i = 0; i < RF_FIR_IMPL_COUNT; ++i) {
+            if (std::strcmp(name, rf::kFirImpls[i].name) == 0) {
+                *impl = i;
+                return;
+            }
+        }
+        rf::fail(RF_INVALID_ARG, std::string("unknown fir implementation '") + name + "'");
+    });
+}
+
+rf_status rf_fir(int impl, const rf_cf32* x, const float* taps, rf_cf32* y, int num_samples,
+                 int num_taps, int block, rf_stream stream)
+{
+    return guarded([&] {
+        const int b = block == 0 ? RF_DEFAULT_FIR_BLOCK : block;
+        rf::check_fir_args(impl, x, taps, y, num_samples, num_taps, b);
+        rf::fir_launch(impl, reinterpret_cast<const float2*>(x), taps,
+                       reinterpret_cast<float2*>(y), num_samples, num_taps, b,
+                       rf::as_stream(stream));
+    });
+}
+
+rf_status rf_fir_batched(int impl, const rf_cf32* const* x, rf_cf32* const* y,
+                         const int* num_samples, int count, const float* taps, int num_taps,
+                         rf_stream stream)
+{
+    return guarded([&] {
+        if (count < 1 || count > RF_MAX_FIR_BATCH) {
+            rf::fail(RF_LIMIT, "batched fir: " + std::to_string(count)
+                                   + " channels, expected 1 to "
+                                   + std::to_string(RF_MAX_FIR_BATCH));
+        }
+        if (x == nullptr || y == nullptr || num_samples == nullptr) {
+            rf::fail(RF_INVALID_ARG, "batched fir: NULL channel arrays");
+        }
+        rf::FirBatch batch{};
+        int longest = 0;
+        for (int c = 0; c < count; ++c) {
+            rf::check_fir_args(impl, x[c], taps, y[c], num_samples[c], num_taps,
+                               RF_DEFAULT_FIR_BLOCK);
+            batch.x[c] = reinterpret_cast<const float2*>(x[c]);
+            batch.y[c] = reinterpret_cast<float2*>(y[c]);
+            batch.n[c] = num_samples[c];
+            longest = num_samples[c] > longest ? num_samples[c] : longest;
+        }
+        if (longest == 0) {
+            return;
+        }
+        rf::fir_batched_launch(impl, batch, count, longest, taps, num_taps,
+                               rf::as_stream(stream));
+    });
+}
+
+}  // extern "C"

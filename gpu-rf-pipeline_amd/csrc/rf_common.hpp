@@ -1,0 +1,125 @@
+// Shared pieces of the HIP sources: status handling across the C ABI, launch helpers and the
+// complex arithmetic every kernel uses. Internal to the library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <exception>
+#include <new>
+#include <string>
+
+#include "rfhip.h"
+
+namespace rf {
+
+// Thrown inside the library and turned into an rf_status at the C boundary, so no exception
+// ever crosses the ABI. Mirrors the reference's throw-on-error style (src/check.hpp:42-66).
+struct Error {
+    rf_status status;
+    std::string message;
+};
+
+void set_last_error(const std::string& message);
+
+[[noreturn]] inline void fail(rf_status status, const std::string& message)
+{
+    throw Error{status, message};
+}
+
+[[noreturn]] void fail_hip(hipError_t err, const char* call, const char* file, int line);
+
+#define RF_HIP_CHECK(call)                                                                    \
+    do {                                                                                      \
+        const hipError_t rf_hip_err_ = (call);                                                \
+        if (rf_hip_err_ != hipSuccess) {                                                      \
+            ::rf::fail_hip(rf_hip_err_, #call, __FILE__, __LINE__);                           \
+        }                                                                                     \
+    } while (0)
+
+// Runs `body` and converts a thrown rf::Error (or anything else) into a status code.
+template <class F>
+rf_status guarded(F&& body)
+{
+    try {
+        body();
+        return RF_OK;
+    } catch (const Error& e) {
+        set_last_error(e.message);
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return RF_ALLOC;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return RF_INVALID_ARG;
+    } catch (...) {
+        set_last_error("unknown error");
+        return RF_INVALID_ARG;
+    }
+}
+
+inline hipStream_t as_stream(rf_stream s)
+{
+    return static_cast<hipStream_t>(s);
+}
+
+// Checks the launch that was just enqueued.
+inline void check_launch(const char* what)
+{
+    const hipError_t err = hipGetLastError();
+    if (err != hipSuccess) {
+        fail(RF_HIP_ERROR, std::string("HIP error: ") + hipGetErrorString(err) + " (code "
+                               + std::to_string(static_cast<int>(err)) + ")\n  in: launch of "
+                               + what);
+    }
+}
+
+// LDS bytes one workgroup may use on the current device.
+int max_lds_per_block();
+// Compute units of the current device.
+int cu_count();
+
+inline long long ceil_div(long long a, long long b)
+{
+    return (a + b - 1) / b;
+}
+
+inline bool is_pow2(long long v)
+{
+    return v > 0 && (v & (v - 1)) == 0;
+}
+
+inline int ilog2(long long v)
+{
+    int l = 0;
+    while ((1LL << l) < v) {
+        ++l;
+    }
+    return l;
+}
+
+// Device-side complex helpers on float2 (x = re, y = im).
+__device__ __forceinline__ float2 cadd(float2 a, float2 b)
+{
+    return make_float2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ float2 csub(float2 a, float2 b)
+{
+    return make_float2(a.x - b.x, a.y - b.y);
+}
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ float2 cmulconj(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s)
+{
+    return make_float2(a.x * s, a.y * s);
+}
+
+}  // namespace rf

@@ -1,0 +1,232 @@
+// Status codes, the thread-local error text, and the runtime utilities of the C ABI (device
+// memory, streams, events). These have no arithmetic; the reference calls the CUDA runtime for
+// the same jobs directly (src/device_buffer.hpp:11-76, src/bench.cpp:10-40).
+
+#include "rf_common.hpp"
+
+#include <cstring>
+
+namespace rf {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_last_error(const std::string& message)
+{
+    g_last_error = message;
+}
+
+void fail_hip(hipError_t err, const char* call, const char* file, int line)
+{
+    fail(err == hipErrorOutOfMemory || err == hipErrorMemoryAllocation ? RF_ALLOC : RF_HIP_ERROR,
+         std::string("HIP error: ") + hipGetErrorString(err) + " (code "
+             + std::to_string(static_cast<int>(err)) + ")\n  in: " + call + "\n  at: " + file
+             + ":" + std::to_string(line));
+}
+
+int max_lds_per_block()
+{
+    int device = 0;
+    RF_HIP_CHECK(hipGetDevice(&device));
+    int bytes = 0;
+    RF_HIP_CHECK(hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    return bytes;
+}
+
+int cu_count()
+{
+    int device = 0;
+    RF_HIP_CHECK(hipGetDevice(&device));
+    int cus = 0;
+    RF_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    return cus;
+}
+
+}  // namespace rf
+
+using rf::guarded;
+
+extern "C" {
+
+const char* rf_status_string(rf_status status)
+{
+    switch (status) {
+    case RF_OK:          return "RF_OK";
+    case RF_INVALID_ARG: return "RF_INVALID_ARG";
+    case RF_LIMIT:       return "RF_LIMIT";
+    case RF_HIP_ERROR:   return "RF_HIP_ERROR";
+    case RF_ALLOC:       return "RF_ALLOC";
+    }
+    return "RF_UNKNOWN_STATUS";
+}
+
+const char* rf_last_error(void)
+{
+    return rf::g_last_error.c_str();
+}
+
+int rf_abi_version(void)
+{
+    return 1;
+}
+
+rf_status rf_device_count(int* count)
+{
+    return guarded([&] {
+        if (count == nullptr) rf::fail(RF_INVALID_ARG, "rf_device_count: count is NULL");
+        *count = 0;
+        RF_HIP_CHECK(hipGetDeviceCount(count));
+    });
+}
+
+rf_status rf_device_name(char* buffer, int length)
+{
+    return guarded([&] {
+        if (buffer == nullptr || length < 1) {
+            rf::fail(RF_INVALID_ARG, "rf_device_name: need a buffer");
+        }
+        int device = 0;
+        RF_HIP_CHECK(hipGetDevice(&device));
+        hipDeviceProp_t prop{};
+        RF_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+        std::string name = prop.name;
+        if (name.empty()) name = prop.gcnArchName;
+        std::strncpy(buffer, name.c_str(), static_cast<size_t>(length) - 1);
+        buffer[length - 1] = '\0';
+    });
+}
+
+rf_status rf_device_memory(size_t* bytes)
+{
+    return guarded([&] {
+        int device = 0;
+        RF_HIP_CHECK(hipGetDevice(&device));
+        hipDeviceProp_t prop{};
+        RF_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+        *bytes = prop.totalGlobalMem;
+    });
+}
+
+rf_status rf_malloc(void** ptr, size_t bytes)
+{
+    return guarded([&] {
+        if (ptr == nullptr) rf::fail(RF_INVALID_ARG, "rf_malloc: ptr is NULL");
+        *ptr = nullptr;
+        if (bytes == 0) return;
+        RF_HIP_CHECK(hipMalloc(ptr, bytes));
+    });
+}
+
+rf_status rf_free(void* ptr)
+{
+    return guarded([&] { RF_HIP_CHECK(hipFree(ptr)); });
+}
+
+rf_status rf_host_alloc(void** ptr, size_t bytes)
+{
+    return guarded([&] {
+        if (ptr == nullptr) rf::fail(RF_INVALID_ARG, "rf_host_alloc: ptr is NULL");
+        *ptr = nullptr;
+        if (bytes == 0) return;
+        RF_HIP_CHECK(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+    });
+}
+
+rf_status rf_host_free(void* ptr)
+{
+    return guarded([&] { RF_HIP_CHECK(hipHostFree(ptr)); });
+}
+
+rf_status rf_memcpy_h2d(void* dst, const void* src, size_t bytes, rf_stream stream)
+{
+    return guarded([&] {
+        if (bytes == 0) return;
+        RF_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, rf::as_stream(stream)));
+    });
+}
+
+rf_status rf_memcpy_d2h(void* dst, const void* src, size_t bytes, rf_stream stream)
+{
+    return guarded([&] {
+        if (bytes == 0) return;
+        RF_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, rf::as_stream(stream)));
+    });
+}
+
+rf_status rf_memcpy_d2d(void* dst, const void* src, size_t bytes, rf_stream stream)
+{
+    return guarded([&] {
+        if (bytes == 0) return;
+        RF_HIP_CHECK(
+            hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, rf::as_stream(stream)));
+    });
+}
+
+rf_status rf_memset(void* dst, int value, size_t bytes, rf_stream stream)
+{
+    return guarded([&] {
+        if (bytes == 0) return;
+        RF_HIP_CHECK(hipMemsetAsync(dst, value, bytes, rf::as_stream(stream)));
+    });
+}
+
+rf_status rf_stream_create(rf_stream* stream)
+{
+    return guarded([&] {
+        hipStream_t s = nullptr;
+        RF_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        *stream = s;
+    });
+}
+
+rf_status rf_stream_destroy(rf_stream stream)
+{
+    return guarded([&] { RF_HIP_CHECK(hipStreamDestroy(rf::as_stream(stream))); });
+}
+
+rf_status rf_stream_synchronize(rf_stream stream)
+{
+    return guarded([&] { RF_HIP_CHECK(hipStreamSynchronize(rf::as_stream(stream))); });
+}
+
+rf_status rf_device_synchronize(void)
+{
+    return guarded([&] { RF_HIP_CHECK(hipDeviceSynchronize()); });
+}
+
+rf_status rf_event_create(void** event)
+{
+    return guarded([&] {
+        hipEvent_t e = nullptr;
+        RF_HIP_CHECK(hipEventCreate(&e));
+        *event = e;
+    });
+}
+
+rf_status rf_event_destroy(void* event)
+{
+    return guarded([&] { RF_HIP_CHECK(hipEventDestroy(static_cast<hipEvent_t>(event))); });
+}
+
+rf_status rf_event_record(void* event, rf_stream stream)
+{
+    return guarded([&] {
+        RF_HIP_CHECK(hipEventRecord(static_cast<hipEvent_t>(event), rf::as_stream(stream)));
+    });
+}
+
+rf_status rf_event_synchronize(void* event)
+{
+    return guarded([&] { RF_HIP_CHECK(hipEventSynchronize(static_cast<hipEvent_t>(event))); });
+}
+
+rf_status rf_event_elapsed_ms(void* start, void* stop, float* ms)
+{
+    return guarded([&] {
+        RF_HIP_CHECK(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start),
+                                         static_cast<hipEvent_t>(stop)));
+    });
+}
+
+}  // extern "C"

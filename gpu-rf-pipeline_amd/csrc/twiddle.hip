@@ -1,0 +1,52 @@
+// Twiddle tables (float64 on the host, stored as float32 on the device).
+
+#include "twiddle.hpp"
+#include "fft_core.hpp"
+
+#include <cmath>
+#include <mutex>
+#include <vector>
+
+namespace rf {
+
+void fill_twiddles(float2* host, long long count, long long n, long long step)
+{
+    // angle = -2 pi (m * step mod n) / n, reduced exactly in integers before the float64
+    // sin/cos so that large indices keep full accuracy.
+    const double two_pi = 6.283185307179586476925286766559;
+    for (long long m = 0; m < count; ++m) {
+        const long long e = (m * step) % n;
+        const double a = -two_pi * static_cast<double>(e) / static_cast<double>(n);
+        host[m] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+    }
+}
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::once_flag g_once[kMaxDevices];
+float2* g_table[kMaxDevices] = {};
+}  // namespace
+
+const float2* twiddle_table_4096()
+{
+    int device = 0;
+    RF_HIP_CHECK(hipGetDevice(&device));
+    if (device < 0 || device >= kMaxDevices) {
+        fail(RF_LIMIT, "twiddle table: device index out of range");
+    }
+    std::call_once(g_once[device], [device] {
+        std::vector<float2> host(kTwiddleSize);
+        fill_twiddles(host.data(), kTwiddleSize, kTwiddleSize, 1);
+        float2* dev = nullptr;
+        RF_HIP_CHECK(hipMalloc(&dev, kTwiddleSize * sizeof(float2)));
+        RF_HIP_CHECK(hipMemcpy(dev, host.data(), kTwiddleSize * sizeof(float2),
+                               hipMemcpyHostToDevice));
+        g_table[device] = dev;
+    });
+    if (g_table[device] == nullptr) {
+        fail(RF_HIP_ERROR, "twiddle table: initialisation failed on an earlier call");
+    }
+    return g_table[device];
+}
+
+}  // namespace rf

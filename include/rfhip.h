@@ -1,0 +1,204 @@
+/*
+ * rfhip.h — C ABI of the MI355X (gfx950) implementation of gpu-rf-pipeline's hot path.
+ *
+ * Every function is plain C: pointers, sizes and status codes, no C++ or HIP types. Device
+ * pointers are ordinary `void*`/`rf_cf32*` addresses returned by rf_malloc (or any hipMalloc).
+ * A stream is an `rf_stream` holding a hipStream_t; NULL means the default (null) stream.
+ *
+ * Conventions carried over from the reference's operator surface (src/device_ops.hpp:1-106,
+ * src/ops.hpp:1-52):
+ *   - compute entry points only enqueue work on the given stream; they never synchronise and
+ *     never copy to or from the host;
+ *   - the caller owns every input and output buffer; plan handles own their workspaces;
+ *   - taps are read from the caller's pointer on every launch (no program-wide tap bank, unlike
+ *     the reference's __constant__ c_taps, src/fir.cu:208-209);
+ *   - no function throws: failures return an rf_status, and rf_last_error() gives the message
+ *     (thread-local), in the reference's exception-text style (src/check.hpp:42-66).
+ */
+#ifndef RFHIP_H
+#define RFHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One complex float32 sample, I then Q: the .cf32 file layout (src/types.hpp:7-12). */
+typedef struct rf_cf32 {
+    float re;
+    float im;
+} rf_cf32;
+
+/* A hipStream_t. NULL is the default stream. */
+typedef void* rf_stream;
+
+typedef enum rf_status {
+    RF_OK = 0,
+    RF_INVALID_ARG = 1, /* bad argument: unknown impl, block size, channel count, NULL pointer */
+    RF_LIMIT = 2,       /* a documented size limit (shared memory, taps, FFT length, batch) */
+    RF_HIP_ERROR = 3,   /* the HIP runtime reported an error */
+    RF_ALLOC = 4        /* device or host allocation failed */
+} rf_status;
+
+/* Name of a status code ("RF_OK", ...). */
+const char* rf_status_string(rf_status status);
+/* Message of the last failing call on this thread ("" if none). Replaces the text of the
+ * std::runtime_error the reference throws (src/check.hpp:42-66, src/fir.cu:160-166). */
+const char* rf_last_error(void);
+/* Version of this ABI; bumped whenever a signature changes. */
+int rf_abi_version(void);
+
+/* ---------------------------------------------------------------------------------------
+ * Runtime utilities. No reference counterpart: the reference calls the CUDA runtime directly
+ * (src/device_buffer.hpp:11-76, src/bench.cpp:10-40). They let a binding without a GPU
+ * framework own device memory, streams and events through this library alone.
+ * ------------------------------------------------------------------------------------- */
+rf_status rf_device_count(int* count);
+rf_status rf_device_name(char* buffer, int length);
+/* Total global memory of the current device, in bytes. */
+rf_status rf_device_memory(size_t* bytes);
+rf_status rf_malloc(void** ptr, size_t bytes);
+rf_status rf_free(void* ptr);
+rf_status rf_host_alloc(void** ptr, size_t bytes); /* page-locked host memory */
+rf_status rf_host_free(void* ptr);
+/* Asynchronous on `stream`; pageable host memory makes them effectively synchronous. */
+rf_status rf_memcpy_h2d(void* dst, const void* src, size_t bytes, rf_stream stream);
+rf_status rf_memcpy_d2h(void* dst, const void* src, size_t bytes, rf_stream stream);
+rf_status rf_memcpy_d2d(void* dst, const void* src, size_t bytes, rf_stream stream);
+rf_status rf_memset(void* dst, int value, size_t bytes, rf_stream stream);
+rf_status rf_stream_create(rf_stream* stream);
+rf_status rf_stream_destroy(rf_stream stream);
+rf_status rf_stream_synchronize(rf_stream stream);
+rf_status rf_device_synchronize(void);
+/* Timing events (hipEvent_t behind a void*). */
+rf_status rf_event_create(void** event);
+rf_status rf_event_destroy(void* event);
+rf_status rf_event_record(void* event, rf_stream stream);
+rf_status rf_event_synchronize(void* event);
+rf_status rf_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* ---------------------------------------------------------------------------------------
+ * FIR:  y[n] = sum_{k=0}^{T-1} taps[k] * x[n-k],  x[m] = 0 for m < 0,  n < num_samples.
+ * Definition: src/fir.cu:1-5, tools/reference.py:15-17,103-111.
+ * ------------------------------------------------------------------------------------- */
+enum {
+    RF_FIR_NAIVE = 0,       /* one lane per output, global memory (src/fir.cu:20-43) */
+    RF_FIR_TILED = 1,       /* LDS tile with a T-1 halo (src/fir.cu:100-144) */
+    RF_FIR_TILED_CONST = 2, /* tiled, taps through the scalar (constant) cache (src/fir.cu:213-254) */
+    RF_FIR_REGISTER = 3,    /* R outputs per lane from a register window (src/fir.cu:337-409) */
+    RF_FIR_FFT_OLA = 4,     /* overlap-save with the library's LDS FFT; no reference counterpart */
+    RF_FIR_IMPL_COUNT = 5
+};
+#define RF_DEFAULT_FIR_BLOCK 256 /* kDefaultFirBlock, src/ops.hpp:35 */
+#define RF_MAX_FIR_BATCH 8       /* kMaxFirBatch, src/device_ops.hpp:46 */
+#define RF_MAX_CONST_TAPS 8192   /* kMaxConstTaps, src/fir.cu:208 */
+
+/* The variant table, in the reference's order (src/fir.cu:725-734); entry 0 is the default. */
+int rf_fir_impl_count(void);
+const char* rf_fir_impl_name(int impl);
+const char* rf_fir_impl_summary(int impl);
+/* parse_fir_impl (src/fused.cu:76-90): RF_INVALID_ARG for an unknown name. */
+rf_status rf_fir_parse_impl(const char* name, int* impl);
+
+/* fir_launch / fir_device (src/fir.cu:513-556) with the block size of the host wrappers
+ * (src/fir.cu:62-85,170-200,449-485): `block` threads per workgroup, a multiple of 32 up to
+ * 1024, or 0 for RF_DEFAULT_FIR_BLOCK. Tiled variants return RF_LIMIT when the halo does not
+ * fit in one workgroup's LDS; tiled_const and register return RF_LIMIT above
+ * RF_MAX_CONST_TAPS taps, as the reference does. num_samples == 0 enqueues nothing. */
+rf_status rf_fir(int impl, const rf_cf32* x, const float* taps, rf_cf32* y, int num_samples,
+                 int num_taps, int block, rf_stream stream);
+
+/* fir_batched_device (src/fused.cu:92-109): `count` (1..RF_MAX_FIR_BATCH) channels sharing one
+ * set of taps in a single launch. x, y and num_samples are HOST arrays of `count` entries
+ * holding device pointers and lengths. Results equal rf_fir with the same impl per channel. */
+rf_status rf_fir_batched(int impl, const rf_cf32* const* x, rf_cf32* const* y,
+                         const int* num_samples, int count, const float* taps, int num_taps,
+                         rf_stream stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Magnitude: mag[n] = |x[n]| (src/magnitude.cu:17-39, tools/reference.py:98-100).
+ * ------------------------------------------------------------------------------------- */
+rf_status rf_magnitude(const rf_cf32* x, float* mag, int num_samples, rf_stream stream);
+
+/* ---------------------------------------------------------------------------------------
+ * FFT plans: `batch` transforms of `nfft` points stored back to back; replaces FftPlan over
+ * cufftPlanMany (src/fft_plan.hpp:11-37). Unnormalised in both directions, like cuFFT.
+ * Powers of two run the library's LDS Stockham FFT (one workgroup up to 4096 points, a
+ * two-pass four-step transform up to 2^24); other lengths run an exact DFT.
+ * ------------------------------------------------------------------------------------- */
+typedef struct rf_fft_plan rf_fft_plan;
+#define RF_FFT_FORWARD (-1) /* CUFFT_FORWARD: exp(-2 pi i n k / N) */
+#define RF_FFT_INVERSE (1)  /* CUFFT_INVERSE: exp(+2 pi i n k / N) */
+rf_status rf_fft_plan_create(int nfft, int batch, rf_fft_plan** plan);
+rf_status rf_fft_plan_destroy(rf_fft_plan* plan);
+/* GPU memory the plan holds (tables and scratch), like cufftGetSize (src/xcorr.cu:331). */
+rf_status rf_fft_plan_bytes(const rf_fft_plan* plan, size_t* bytes);
+/* cufftExecC2C: in == out is allowed (in place). */
+rf_status rf_fft_exec(const rf_fft_plan* plan, const rf_cf32* in, rf_cf32* out, int direction,
+                      rf_stream stream);
+
+/* spectrum_device (src/spectrum.cu:11-17): the plan's batched forward FFT from `in` (already
+ * zero-padded to batch * nfft samples) to `out`. */
+rf_status rf_spectrum(const rf_fft_plan* plan, const rf_cf32* in, rf_cf32* out,
+                      rf_stream stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Polyphase channelizer (src/channelize.cu:1-72, tools/reference.py:24-30,128-146):
+ *   z[t][k] = sum_l h[l] x[tM - l] exp(+2 pi i k l / M),  t < ceil(N/M), k < M.
+ * ------------------------------------------------------------------------------------- */
+/* channelize_frames (src/channelize.cu:51-54): ceil(num_samples / channels). */
+int rf_channelize_frames(int num_samples, int channels);
+/* channelize_device (src/channelize.cu:56-72). `plan` is a plan of `channels` points times
+ * rf_channelize_frames() frames, or NULL: the library then uses its fused branch-sum + FFT
+ * kernel where one exists for `channels` and an internal plan otherwise. */
+rf_status rf_channelize(const rf_fft_plan* plan, const rf_cf32* x, const float* taps,
+                        rf_cf32* out, int num_samples, int channels, int num_taps,
+                        rf_stream stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Cross-correlation (src/xcorr.cu:1-7, tools/reference.py:19-22,114-125):
+ *   r[i] = sum_n b[n + k] conj(a[n]),  k = i - (num_a - 1),  i < num_a + num_b - 1.
+ * ------------------------------------------------------------------------------------- */
+enum { RF_XCORR_NAIVE = 0, RF_XCORR_TILED = 1, RF_XCORR_FFT = 2, RF_XCORR_IMPL_COUNT = 3 };
+int rf_xcorr_impl_count(void);
+const char* rf_xcorr_impl_name(int impl);
+const char* rf_xcorr_impl_summary(int impl);
+rf_status rf_xcorr_parse_impl(const char* name, int* impl);
+
+/* xcorr_naive_device / xcorr_tiled_device (src/xcorr.cu:50-64,174-188). */
+rf_status rf_xcorr_direct(int impl, const rf_cf32* a, const rf_cf32* b, rf_cf32* r, int num_a,
+                          int num_b, rf_stream stream);
+
+/* class XcorrFft (src/device_ops.hpp:70-88, src/xcorr.cu:315-375): setup-owning FFT
+ * cross-correlation for fixed lengths. With `phat`, GCC-PHAT: every frequency of B conj(A) is
+ * scaled to unit magnitude (0 where it is 0) before the inverse FFT (src/xcorr.cu:219-233,
+ * tools/reference.py:32-36,149-162). FFT length L = next power of two >= num_a + num_b - 1;
+ * RF_LIMIT above 2^24. */
+typedef struct rf_xcorr_fft rf_xcorr_fft;
+rf_status rf_xcorr_fft_create(int num_a, int num_b, int phat, rf_xcorr_fft** handle);
+rf_status rf_xcorr_fft_destroy(rf_xcorr_fft* handle);
+rf_status rf_xcorr_fft_bytes(const rf_xcorr_fft* handle, size_t* bytes);
+rf_status rf_xcorr_fft_length(const rf_xcorr_fft* handle, int* fft_length);
+rf_status rf_xcorr_fft_run(const rf_xcorr_fft* handle, const rf_cf32* a, const rf_cf32* b,
+                           rf_cf32* r, rf_stream stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Peak search: the host loop of rfgpu doa (src/main.cpp:390-401) as a device reduction.
+ * index = first k with the largest hypotf(r[k].re, r[k].im) (strict '>', like np.argmax).
+ * `result` is DEVICE memory. `frac` is the sub-sample offset in [-0.5, 0.5] of the parabola
+ * through |r| at index-1, index, index+1, evaluated in float64 (0 at either end).
+ * ------------------------------------------------------------------------------------- */
+typedef struct rf_peak_result {
+    int32_t index;
+    float magnitude;
+    double frac;
+} rf_peak_result;
+rf_status rf_peak(const rf_cf32* r, int n, rf_peak_result* result, rf_stream stream);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* RFHIP_H */

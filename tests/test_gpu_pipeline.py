@@ -1,0 +1,365 @@
+"""GPU results against the float64 oracle, through the C ABI (librfhip.so).
+
+Mirrors the reference's tests/test_pipeline.py: the same fixtures, the same tolerances
+(stated next to each operation), every --impl of the variant tables, exact results where the
+reference demands them (identity taps, zero inputs). Full-size workloads are checked through
+properties that do not need the oracle at full size (sampled outputs, round trips).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from _common import cf32, load_rfhip, meta, oracle, read_ci16_sigmf, taps
+
+pytestmark = pytest.mark.gpu
+rf = load_rfhip()
+
+FIR_IMPLS = [i.name for i in rf.fir_impls()]
+XCORR_IMPLS = [i.name for i in rf.xcorr_impls()]
+DIRECT_FIR = ["naive", "tiled", "tiled_const", "register"]
+
+# Tolerances of the reference's tests/test_pipeline.py.
+SPECTRUM_RTOL, SPECTRUM_ATOL = 1e-5, 2e-3      # :62-66
+MAGNITUDE_RTOL, MAGNITUDE_ATOL = 1e-6, 1e-6    # :90-92
+FIR_RTOL, FIR_ATOL = 1e-5, 1e-5                # :113-117
+XCORR_RTOL, XCORR_ATOL = 1e-5, 5e-3            # :170-177
+CHANNELIZE_RTOL, CHANNELIZE_ATOL = 1e-5, 1e-4  # :360-364
+GCC_ATOL = 1e-4                                # :408-411
+
+
+def kaiser_lowpass(num_taps: int, cutoff: float, beta: float) -> np.ndarray:
+    """Kaiser-windowed sinc, cutoff in cycles per sample, unit DC gain, float32."""
+    n = np.arange(num_taps) - (num_taps - 1) / 2
+    h = 2 * cutoff * np.sinc(2 * cutoff * n) * np.kaiser(num_taps, beta)
+    return (h / h.sum()).astype(np.float32)
+
+
+def noise(seed: int, n: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+
+
+# --- spectrum --------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("nfft", [1024, 256])
+@pytest.mark.parametrize("name", ["tone", "tone_odd", "short"])
+def test_spectrum_matches_oracle(name: str, nfft: int) -> None:
+    x = cf32(name)
+    got = rf.spectrum(x, nfft)
+    want = oracle.spectrum(x, nfft).ravel()
+    assert got.shape == want.shape
+    np.testing.assert_allclose(got, want, rtol=SPECTRUM_RTOL, atol=SPECTRUM_ATOL)
+
+
+@pytest.mark.parametrize("nfft", [1, 2, 4, 8, 16, 64, 100, 1000, 4096, 8192, 16384, 65536])
+def test_spectrum_any_length_matches_oracle(nfft: int) -> None:
+    # Powers of two up to 4096 in one workgroup, four-step above, exact DFT otherwise.
+    x = cf32("tone")
+    got = rf.spectrum(x, nfft)
+    want = oracle.spectrum(x, nfft).ravel()
+    scale = max(1.0, np.sqrt(nfft / 1024))
+    np.testing.assert_allclose(got, want, rtol=SPECTRUM_RTOL, atol=SPECTRUM_ATOL * scale)
+
+
+def test_fft_inverse_round_trip() -> None:
+    for nfft, batch in [(4096, 3), (1 << 15, 2), (1 << 18, 1), (48, 5)]:
+        x = noise(nfft, nfft * batch)
+        plan = rf.FftPlan(nfft, batch)
+        d = rf.DeviceBuffer.from_array(x)
+        plan.exec(d.ptr, d.ptr, rf.RF_FFT_FORWARD)
+        plan.exec(d.ptr, d.ptr, rf.RF_FFT_INVERSE)
+        back = d.download() / nfft
+        rf.synchronize()
+        plan.destroy()
+        np.testing.assert_allclose(back, x, atol=2e-5 * np.log2(nfft + 1))
+
+
+def test_spectrum_of_zeros_is_exactly_zero() -> None:
+    got = rf.spectrum(cf32("zeros"), 1024)
+    assert got.shape == (4096,) and not np.any(got)
+
+
+# --- magnitude -----------------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", ["tone", "tone_odd", "short", "ch0"])
+def test_magnitude_matches_oracle(name: str) -> None:
+    x = cf32(name)
+    np.testing.assert_allclose(rf.magnitude(x), oracle.magnitude(x), rtol=MAGNITUDE_RTOL,
+                               atol=MAGNITUDE_ATOL)
+
+
+def test_magnitude_of_zeros_is_exactly_zero() -> None:
+    got = rf.magnitude(cf32("zeros"))
+    assert got.shape == (4096,) and not np.any(got)
+
+
+def test_magnitude_unaligned_and_odd_lengths() -> None:
+    x = noise(7, 1001)
+    d = rf.DeviceBuffer.from_array(x)
+    m = rf.DeviceBuffer(1001, np.float32)
+    rf.magnitude_device(d.offset(1), m.ptr, 999)
+    got = m.download(999)
+    np.testing.assert_allclose(got, np.abs(x[1:1000].astype(np.complex128)), rtol=1e-6, atol=1e-6)
+
+
+# --- FIR -----------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("impl", FIR_IMPLS)
+@pytest.mark.parametrize("name,taps_name", [
+    ("tone", "lowpass"), ("tone", "lowpass_long"), ("tone_odd", "lowpass"),
+    ("tone_odd", "lowpass_long"), ("short", "lowpass"), ("short", "lowpass_long"),
+])
+def test_fir_matches_oracle(impl: str, name: str, taps_name: str) -> None:
+    x, h = cf32(name), taps(taps_name)
+    got = rf.fir(x, h, impl)
+    want = oracle.fir(x, h)
+    assert got.shape == want.shape
+    np.testing.assert_allclose(got, want, rtol=FIR_RTOL, atol=FIR_ATOL)
+
+
+@pytest.mark.parametrize("impl", FIR_IMPLS)
+@pytest.mark.parametrize("block", [32, 96, 1024])
+def test_fir_block_size_does_not_change_the_result(impl: str, block: int) -> None:
+    x, h = cf32("tone_odd"), taps("lowpass_long")
+    np.testing.assert_allclose(rf.fir(x, h, impl, block), oracle.fir(x, h), rtol=FIR_RTOL,
+                               atol=FIR_ATOL)
+
+
+@pytest.mark.parametrize("impl", FIR_IMPLS)
+@pytest.mark.parametrize("name", ["tone", "tone_odd", "short"])
+def test_fir_identity_returns_the_input_exactly(impl: str, name: str) -> None:
+    x = cf32(name)
+    np.testing.assert_array_equal(rf.fir(x, taps("identity"), impl), x)
+
+
+@pytest.mark.parametrize("impl", FIR_IMPLS)
+@pytest.mark.parametrize("taps_name", ["lowpass", "lowpass_long", "identity"])
+def test_fir_of_zeros_is_exactly_zero(impl: str, taps_name: str) -> None:
+    got = rf.fir(cf32("zeros"), taps(taps_name), impl)
+    assert got.shape == (4096,) and not np.any(got)
+
+
+def test_direct_fir_variants_agree_bit_for_bit() -> None:
+    # Every direct variant sums fma(h[k], x[n-k]) in k order from 0.
+    x, h = cf32("tone_odd"), taps("lowpass_long")
+    results = [rf.fir(x, h, impl) for impl in DIRECT_FIR]
+    for r in results[1:]:
+        np.testing.assert_array_equal(r, results[0])
+
+
+@pytest.mark.parametrize("impl", FIR_IMPLS)
+def test_fir_batched_equals_single_channel(impl: str) -> None:
+    h = taps("lowpass_long")
+    lens = [65536, 10007, 7, 4096, 1, 3000, 70001, 12]
+    xs = [noise(100 + i, n) for i, n in enumerate(lens)]
+    d_x = [rf.DeviceBuffer.from_array(x) for x in xs]
+    d_y = [rf.DeviceBuffer(n) for n in lens]
+    d_h = rf.DeviceBuffer.from_array(h)
+    idx = rf.fir_impl_index(impl)
+    rf.fir_batched_device(idx, [b.ptr for b in d_x], [b.ptr for b in d_y], lens, d_h.ptr, h.size)
+    for x, y in zip(xs, d_y):
+        np.testing.assert_array_equal(y.download(), rf.fir(x, h, impl))
+
+
+@pytest.mark.parametrize("impl", ["tiled_const", "register"])
+def test_fir_const_tap_limit(impl: str) -> None:
+    h = np.zeros(8193, dtype=np.float32)
+    with pytest.raises(rf.RfError) as e:
+        rf.fir(noise(1, 100), h, impl)
+    assert e.value.status == rf.RF_LIMIT
+
+
+def test_fir_tiled_lds_limit_is_reported() -> None:
+    # (T - 1 + B) * 8 bytes beyond a workgroup's LDS
+    h = np.zeros(40000, dtype=np.float32)
+    h[0] = 1
+    with pytest.raises(rf.RfError) as e:
+        rf.fir(noise(1, 100), h, "tiled")
+    assert e.value.status == rf.RF_LIMIT
+
+
+@pytest.mark.parametrize("impl", FIR_IMPLS)
+def test_fir_sigmf_ci16_input(impl: str) -> None:
+    x = read_ci16_sigmf("tone_ci16")
+    np.testing.assert_array_equal(rf.fir(x, taps("identity"), impl), x)
+    np.testing.assert_allclose(rf.fir(x, taps("lowpass"), impl), oracle.fir(x, taps("lowpass")),
+                               rtol=FIR_RTOL, atol=FIR_ATOL)
+
+
+@pytest.mark.parametrize("impl", FIR_IMPLS)
+def test_fir_long_random_filter(impl: str) -> None:
+    # 2049 taps: the longest overlap-save filter; random signs stress cancellation.
+    h = (np.random.default_rng(5).standard_normal(2049) / 40).astype(np.float32)
+    x = noise(6, 20000)
+    np.testing.assert_allclose(rf.fir(x, h, impl), oracle.fir(x, h), rtol=FIR_RTOL, atol=FIR_ATOL)
+
+
+def test_fir_config_plain_2e20_x_63() -> None:
+    # The CPU-runnable workload: 2^20 samples, 63-tap Hamming sinc (cutoff 0.2 fs).
+    n = 1 << 20
+    t = np.arange(n)
+    x = noise(0xC1, n) / np.sqrt(2) + sum(np.exp(2j * np.pi * f * t) for f in (0.05, 0.17, 0.31))
+    x = x.astype(np.complex64)
+    m = np.arange(63) - 31
+    h = 0.4 * np.sinc(0.4 * m) * np.hamming(63)
+    h = (h / h.sum()).astype(np.float32)
+    want = oracle.fir(x, h)
+    for impl in FIR_IMPLS:
+        np.testing.assert_allclose(rf.fir(x, h, impl), want, rtol=FIR_RTOL, atol=FIR_ATOL)
+
+
+@pytest.mark.parametrize("impl", ["register", "fft_ola"])
+def test_fir_full_size_long_filter_sampled_outputs(impl: str) -> None:
+    # 2^24 samples x 1025 Kaiser taps: 3000 outputs recomputed in float64 on the host.
+    n = 1 << 24
+    x = noise(0xC2, n) / np.float32(np.sqrt(2))
+    h = kaiser_lowpass(1025, 0.1, 8.0)
+    y = rf.fir(x, h, impl)
+    rng = np.random.default_rng(3)
+    idx = np.concatenate([np.arange(0, 1100), rng.integers(0, n, 1900), [n - 1]])
+    k = np.arange(1025)
+    src = idx[:, None] - k[None, :]
+    xv = np.where(src >= 0, x[np.clip(src, 0, None)].astype(np.complex128), 0)
+    want = xv @ h.astype(np.float64)
+    np.testing.assert_allclose(y[idx], want, rtol=FIR_RTOL, atol=FIR_ATOL)
+
+
+# --- cross-correlation ---------------------------------------------------------------------
+
+@pytest.mark.parametrize("impl", XCORR_IMPLS)
+@pytest.mark.parametrize("a,b", [("ch0", "ch1"), ("ch1", "ch0"), ("tone_odd", "short"),
+                                 ("short", "tone_odd"), ("short", "short")])
+def test_xcorr_matches_oracle(impl: str, a: str, b: str) -> None:
+    xa, xb = cf32(a), cf32(b)
+    got = rf.xcorr(xa, xb, impl)
+    want = oracle.xcorr(xa, xb)
+    assert got.shape == want.shape
+    np.testing.assert_allclose(got, want, rtol=XCORR_RTOL, atol=XCORR_ATOL)
+
+
+@pytest.mark.parametrize("impl", XCORR_IMPLS)
+def test_xcorr_recovers_the_known_delay(impl: str) -> None:
+    r = rf.xcorr(cf32("ch0"), cf32("ch1"), impl)
+    lags = oracle.xcorr_lags(16384, 16384)
+    assert lags[np.argmax(np.abs(r))] == meta("ch1")["true_delay_samples"]
+
+
+@pytest.mark.parametrize("impl", XCORR_IMPLS)
+def test_xcorr_of_zeros_is_exactly_zero(impl: str) -> None:
+    got = rf.xcorr(cf32("zeros"), cf32("zeros"), impl)
+    assert got.shape == (2 * 4096 - 1,) and not np.any(got)
+
+
+@pytest.mark.parametrize("na,nb", [(1, 1), (1, 2), (2, 3), (3, 1), (5, 4), (100, 3)])
+def test_xcorr_fft_tiny_lengths(na: int, nb: int) -> None:
+    a, b = noise(na, na), noise(nb + 50, nb)
+    np.testing.assert_allclose(rf.xcorr(a, b, "fft"), oracle.xcorr(a, b), rtol=XCORR_RTOL,
+                               atol=1e-5)
+    np.testing.assert_allclose(rf.gcc_phat(a, b), oracle.gcc_phat(a, b), atol=GCC_ATOL)
+
+
+# --- GCC-PHAT, peak, DOA -------------------------------------------------------------------
+
+@pytest.mark.parametrize("a,b", [("ch0", "ch1"), ("array0", "array3"), ("short", "tone_odd"),
+                                 ("tone", "tone_odd"), ("tone", "tone")])
+def test_gcc_phat_matches_oracle(a: str, b: str) -> None:
+    xa, xb = cf32(a), cf32(b)
+    got = rf.gcc_phat(xa, xb)
+    want = oracle.gcc_phat(xa, xb)
+    assert got.shape == want.shape
+    np.testing.assert_allclose(got, want, rtol=0, atol=GCC_ATOL)
+
+
+def test_gcc_phat_long_pair_matches_oracle() -> None:
+    # L = 2^20: the four-step path with 256 rows of 4096.
+    a = noise(41, 300000)
+    b = np.concatenate([np.zeros(123, np.complex64), a[:-123]]) + 0.3 * noise(42, 300000)
+    got = rf.gcc_phat(a, b.astype(np.complex64))
+    want = oracle.gcc_phat(a, b)
+    np.testing.assert_allclose(got, want, rtol=0, atol=GCC_ATOL)
+    assert np.argmax(np.abs(got)) - (a.size - 1) == 123
+
+
+def test_gcc_phat_peaks_at_the_known_delay() -> None:
+    r = rf.gcc_phat(cf32("ch0"), cf32("ch1"))
+    lags = oracle.xcorr_lags(16384, 16384)
+    assert lags[np.argmax(np.abs(r))] == meta("ch1")["true_delay_samples"]
+    peak = np.max(np.abs(r))
+    assert peak > 20 * np.sort(np.abs(r))[-2]
+
+
+def test_gcc_phat_of_zeros_is_exactly_zero() -> None:
+    got = rf.gcc_phat(cf32("zeros"), cf32("zeros"))
+    assert got.shape == (2 * 4096 - 1,) and not np.any(got)
+
+
+def test_device_peak_search_takes_the_first_maximum() -> None:
+    r = noise(8, 100000)
+    r[777] = 50 + 0j
+    r[90000] = 0 + 50j  # same magnitude, later index
+    d = rf.DeviceBuffer.from_array(r)
+    res = rf.peak(d.ptr, r.size)
+    assert res.index == 777 and res.magnitude == pytest.approx(50.0)
+    z = rf.DeviceBuffer.from_array(np.zeros(5000, np.complex64))
+    res = rf.peak(z.ptr, 5000)
+    assert res.index == 0 and res.magnitude == 0.0
+
+
+def test_device_peak_interpolates_a_sampled_parabola() -> None:
+    k = np.arange(2000, dtype=np.float64)
+    offset = 0.3
+    r = (100.0 - (k - 1000 - offset) ** 2).clip(0).astype(np.complex64)
+    d = rf.DeviceBuffer.from_array(r)
+    res = rf.peak(d.ptr, r.size)
+    assert res.index == 1000
+    assert res.frac == pytest.approx(offset, abs=1e-5)
+
+
+def test_doa_recovers_the_array_angle() -> None:
+    m = meta("array0")
+    chans = [cf32(f"array{i}") for i in range(m["channels"])]
+    report = rf.doa(chans, m["spacing_m"], m["speed_m_s"], m["sample_rate"])
+    assert report["tdoa_samples"] == [2, 4, 6]
+    assert report["angle_deg"] == pytest.approx(m["angle_deg"], abs=1e-3)
+    ref = oracle.doa(chans, m["spacing_m"], m["speed_m_s"], m["sample_rate"])
+    assert report["angle_deg"] == pytest.approx(ref, abs=1e-6)
+
+
+# --- channelizer ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("use_plan", [False, True])
+@pytest.mark.parametrize("name,taps_name,channels", [
+    ("tone", "channelizer_proto", 16), ("tone_odd", "channelizer_proto", 16),
+    ("tone", "lowpass_long", 64), ("short", "lowpass", 4), ("tone_odd", "identity", 1),
+    ("tone", "lowpass_long", 8), ("tone_odd", "lowpass", 6), ("tone", "lowpass_long", 1024),
+    ("tone", "channelizer_proto", 2), ("tone_odd", "lowpass_long", 12),
+])
+def test_channelize_matches_oracle(name: str, taps_name: str, channels: int,
+                                   use_plan: bool) -> None:
+    x, h = cf32(name), taps(taps_name)
+    got = rf.channelize(x, h, channels, use_plan=use_plan)
+    want = oracle.channelize(x, h, channels).ravel()
+    assert got.shape == want.shape
+    np.testing.assert_allclose(got, want, rtol=CHANNELIZE_RTOL, atol=CHANNELIZE_ATOL)
+
+
+def test_channelize_puts_a_tone_in_its_channel() -> None:
+    n = np.arange(16 * 512)
+    x = np.exp(2j * np.pi * 3 * n / 16).astype(np.complex64)
+    z = rf.channelize(x, taps("channelizer_proto"), 16).reshape(-1, 16)[16:]
+    power = np.mean(np.abs(z) ** 2, axis=0)
+    assert np.argmax(power) == 3
+    assert power[3] > 1000 * np.max(np.delete(power, 3))
+    np.testing.assert_allclose(np.abs(z[:, 3]), 1.0, atol=0.02)
+
+
+def test_channelize_config_shape_64_channels_1024_taps() -> None:
+    # The channelizer workload's filterbank (M = 64, 16 taps per branch) on 2^20 samples.
+    x = noise(0xC3, 1 << 20)
+    h = kaiser_lowpass(1024, 1 / 128, 8.0)
+    got = rf.channelize(x, h, 64)
+    want = oracle.channelize(x, h, 64).ravel()
+    np.testing.assert_allclose(got, want, rtol=CHANNELIZE_RTOL, atol=CHANNELIZE_ATOL)
