@@ -220,6 +220,132 @@ def bench_channelize(rf, args) -> dict:
     }
 
 
+def gcc_batch_input(count: int = 256, seed: int = 0xC4):
+    """Capture pairs: N log-uniform in [2^12, 2^20]; a complex white Gaussian; b = a delayed by
+    d ~ U[-200, 200] (integer part by indexing, fraction by a 65-tap Kaiser(beta=8)
+    windowed-sinc fractional-delay filter) plus independent AWGN at SNR ~ U[0, 20] dB."""
+    from scipy.signal import oaconvolve
+
+    rng = np.random.Generator(np.random.Philox(seed))
+    lengths = np.exp(rng.uniform(np.log(2.0 ** 12), np.log(2.0 ** 20), count)).astype(np.int64)
+    delays = rng.uniform(-200.0, 200.0, count)
+    snrs = rng.uniform(0.0, 20.0, count)
+    taps_k = np.arange(65) - 32
+    window = np.kaiser(65, 8.0)
+    pairs = []
+    for n, d, snr in zip(lengths, delays, snrs):
+        n = int(n)
+        di = int(np.floor(d))
+        frac = d - di
+        h = (np.sinc(taps_k - frac) * window).astype(np.float64)
+        h /= h.sum()
+        total = n + 400 + 64
+        src = (rng.standard_normal(total, dtype=np.float32)
+               + 1j * rng.standard_normal(total, dtype=np.float32)) * np.float32(np.sqrt(0.5))
+        shifted = oaconvolve(src, h, mode="full")[32: 32 + total]   # src delayed by frac
+        a = src[232: 232 + n]
+        b = shifted[232 - di: 232 - di + n]                          # delayed by di + frac
+        sigma = np.sqrt(0.5 * 10.0 ** (-snr / 10.0))
+        a = a + sigma * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+        b = b + sigma * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+        pairs.append((a.astype(np.complex64), b.astype(np.complex64)))
+    return pairs, delays
+
+
+def bench_gcc_batch(rf, args) -> dict:
+    pairs, delays = gcc_batch_input()
+    count = len(pairs)
+    stream = rf.Stream()
+    d_a = [rf.DeviceBuffer.from_array(a) for a, _ in pairs]
+    d_b = [rf.DeviceBuffer.from_array(b) for _, b in pairs]
+    na = [a.size for a, _ in pairs]
+    nb = [b.size for _, b in pairs]
+    h = rf.GccBatch(na, nb)
+    h.bind([x.ptr for x in d_a], [x.ptr for x in d_b])
+    res = rf.DeviceBuffer(16 * count, np.uint8)
+    ms_eager = time_steps(rf, lambda i: h.run(res.ptr, stream.handle), args.steps, args.warmup,
+                          stream.handle) / args.steps
+    ms_graph = time_steps(rf, lambda i: h.run(res.ptr, stream.handle, graph=True), args.steps,
+                          args.warmup, stream.handle) / args.steps
+    peaks = rf.read_peaks(res, count)
+
+    # The plain FP32 path: one XcorrFft (GCC-PHAT) and one device peak search per pair.
+    singles = [rf.XcorrFft(x, y, True) for x, y in zip(na, nb)]
+    d_r = [rf.DeviceBuffer(x + y - 1) for x, y in zip(na, nb)]
+    pres = rf.DeviceBuffer(16 * count, np.uint8)
+
+    def plain(i):
+        for k in range(count):
+            singles[k].run(d_a[k].ptr, d_b[k].ptr, d_r[k].ptr, stream.handle)
+            rf.peak_device(d_r[k].ptr, na[k] + nb[k] - 1, pres.offset(16 * k), stream.handle)
+
+    ms_plain = time_steps(rf, plain, max(2, args.steps // 4), 1, stream.handle) / max(2, args.steps // 4)
+    plain_peaks = rf.read_peaks(pres, count)
+    same = sum(p.index == q.index for p, q in zip(peaks, plain_peaks))
+    lag_err = [abs(p.index - (n - 1) + p.frac - d) for p, n, d in zip(peaks, na, delays)]
+    within = sum(e <= 0.1 for e in lag_err)
+    checked = None
+    if not args.no_cpu:
+        oracle = load_oracle()
+        order = np.argsort(na)[:12]
+        t0 = time.perf_counter()
+        agree = 0
+        for k in order:
+            idx, _ = oracle.peak(oracle.gcc_phat(pairs[k][0], pairs[k][1]))
+            agree += int(idx == peaks[k].index)
+        dt = time.perf_counter() - t0
+        cpu_pairs_per_s = len(order) / dt
+        checked = {"pairs": int(len(order)), "lag_equal_to_oracle": agree}
+    bytes_step = sum(16 * (x + y) for x, y in zip(na, nb))
+    best = min(ms_eager, ms_graph)
+    return {
+        "metric": METRIC, "value": round(count / (best * 1e-3), 1), "unit": "pairs/s",
+        "ms_per_step": round(best, 5), "dtype": "f32",
+        "config": {"workload": f"batched GCC-PHAT, {count} pairs of uneven length "
+                               f"(log-uniform 2^12..2^20 complex64), peak lag per pair",
+                   "pairs": count, "samples": int(sum(na) + sum(nb)),
+                   "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": round(bytes_step / (best * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(bytes_step / (best * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None},
+        "batched": {"eager_ms": round(ms_eager, 4), "graph_ms": round(ms_graph, 4)},
+        "plain_per_pair": {"ms": round(ms_plain, 4), "pairs_per_s": round(count / (ms_plain * 1e-3), 1),
+                           "speedup_of_batched": round(ms_plain / best, 2),
+                           "same_peak_index": same},
+        "accuracy": {"pairs_within_0.1_sample": within, "max_abs_lag_error": round(max(lag_err), 4),
+                     "oracle_check": checked},
+        "cpu_baseline": None if args.no_cpu else {
+            "value": round(cpu_pairs_per_s, 3), "unit": "pairs/s", "cores": 1, "kind": "port",
+            "sample": "float64 oracle GCC-PHAT + peak, one thread, the 12 shortest pairs"},
+    }
+
+
+def bench_copy(rf, args) -> dict:
+    """Achievable HBM bandwidth: the library's streaming copy over 1 GiB (read + write)."""
+    nbytes = 1 << 30
+    stream = rf.Stream()
+    src = rf.DeviceBuffer(nbytes, np.uint8)
+    dst = rf.DeviceBuffer(nbytes, np.uint8)
+    src.zero()
+
+    def step(i):
+        rf.check(rf.lib().rf_memcpy_d2d(dst.ptr, src.ptr, nbytes, stream.handle))
+
+    ms = time_steps(rf, step, args.steps, args.warmup, stream.handle)
+    ms_step = ms / args.steps
+    gbs = 2 * nbytes / (ms_step * 1e-3) / 1e9
+    return {
+        "metric": "copy GB/s (read + write)", "value": round(gbs, 1), "unit": "GB/s",
+        "ms_per_step": round(ms_step, 5), "dtype": "u8",
+        "config": {"workload": "device-to-device copy of 1 GiB", "bytes": nbytes,
+                   "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": None,
+    }
+
+
 def traffic_from_profiles(name: str):
     """HBM bytes per launch from a committed rocprofv3 counter summary, if one exists."""
     path = ROOT / "profiles" / f"{name}_traffic.json"
@@ -237,7 +363,8 @@ def main() -> int:
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--impl", default="fft_ola")
-    p.add_argument("--workload", default="fir_long", choices=["fir_long", "fir_plain", "channelize"])
+    p.add_argument("--workload", default="fir_long",
+                   choices=["fir_long", "fir_plain", "channelize", "gcc_batch", "copy"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = p.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -247,6 +374,10 @@ def main() -> int:
     rf = load_rfhip()
     if args.workload == "channelize":
         out = bench_channelize(rf, args)
+    elif args.workload == "copy":
+        out = bench_copy(rf, args)
+    elif args.workload == "gcc_batch":
+        out = bench_gcc_batch(rf, args)
     else:
         out = bench_fir(rf, args, args.workload == "fir_long")
     line = {"metric": out.pop("metric"), "value": out.pop("value"), "unit": out.pop("unit"),

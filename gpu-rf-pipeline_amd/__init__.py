@@ -77,6 +77,7 @@ _proto("rf_malloc", _st, ctypes.POINTER(_vp), _size)
 _proto("rf_free", _st, _vp)
 _proto("rf_host_alloc", _st, ctypes.POINTER(_vp), _size)
 _proto("rf_host_free", _st, _vp)
+_proto("rf_host_alloc_mapped", _st, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _size)
 for _n in ("rf_memcpy_h2d", "rf_memcpy_d2h", "rf_memcpy_d2d"):
     _proto(_n, _st, _vp, _vp, _size, _vp)
 _proto("rf_memset", _st, _vp, _int, _size, _vp)
@@ -115,6 +116,14 @@ _proto("rf_xcorr_fft_bytes", _st, _vp, ctypes.POINTER(_size))
 _proto("rf_xcorr_fft_length", _st, _vp, ctypes.POINTER(_int))
 _proto("rf_xcorr_fft_run", _st, _vp, _vp, _vp, _vp, _vp)
 _proto("rf_peak", _st, _vp, _int, _vp, _vp)
+_proto("rf_gcc_batch_create", _st, _int, ctypes.POINTER(_int), ctypes.POINTER(_int),
+       ctypes.POINTER(_vp))
+_proto("rf_gcc_batch_destroy", _st, _vp)
+_proto("rf_gcc_batch_bytes", _st, _vp, ctypes.POINTER(_size))
+_proto("rf_gcc_batch_bind", _st, _vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+       ctypes.POINTER(_vp))
+_proto("rf_gcc_batch_run", _st, _vp, _vp, _vp)
+_proto("rf_gcc_batch_launch_graph", _st, _vp, _vp, _vp)
 
 
 def lib() -> ctypes.CDLL:
@@ -605,6 +614,55 @@ def channelize(x, taps, channels: int, bench: Bench | None = None,
         synchronize()
         plan.destroy()
     return out[-1]
+
+
+class GccBatch:
+    """Batched GCC-PHAT + peak search over uneven pairs (rf_gcc_batch_*)."""
+
+    def __init__(self, num_a: Sequence[int], num_b: Sequence[int]):
+        count = len(num_a)
+        p = _vp()
+        na = (_int * count)(*num_a)
+        nb = (_int * count)(*num_b)
+        check(_lib.rf_gcc_batch_create(count, na, nb, ctypes.byref(p)))
+        self.handle = p.value
+        self.count = count
+        self.num_a = list(num_a)
+        self.num_b = list(num_b)
+
+    def bind(self, a: Sequence[int], b: Sequence[int], r: Sequence[int] | None = None) -> None:
+        aa = (_vp * self.count)(*a)
+        bb = (_vp * self.count)(*b)
+        rr = (_vp * self.count)(*r) if r is not None else None
+        check(_lib.rf_gcc_batch_bind(self.handle, aa, bb, rr))
+
+    def run(self, results: int, stream=None, graph: bool = False) -> None:
+        fn = _lib.rf_gcc_batch_launch_graph if graph else _lib.rf_gcc_batch_run
+        check(fn(self.handle, results, _sp(stream)))
+
+    def bytes(self) -> int:
+        n = _size(0)
+        check(_lib.rf_gcc_batch_bytes(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def destroy(self) -> None:
+        if self.handle:
+            check(_lib.rf_gcc_batch_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.rf_gcc_batch_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def read_peaks(d_results: "DeviceBuffer", count: int) -> list[PeakResult]:
+    raw = d_results.download().tobytes()
+    size = ctypes.sizeof(PeakResult)
+    return [PeakResult.from_buffer_copy(raw[i * size:(i + 1) * size]) for i in range(count)]
 
 
 def peak(r_device: int, n: int) -> PeakResult:

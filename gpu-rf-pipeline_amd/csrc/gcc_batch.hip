@@ -1,0 +1,542 @@
+// Batched GCC-PHAT over many capture pairs of uneven length, with the peak search fused in.
+//
+// The reference runs gcc_phat_cuda (src/xcorr.cu:379-399) once per pair and finds the peak on
+// the host (src/main.cpp:390-401). Here every pair's transform is cut into work items of the
+// same size, 4096 complex values, so that one persistent launch per pass covers the whole
+// batch whatever the pair lengths:
+//   pass 1  column FFTs of a and b (four-step column pass, C columns x n2 rows per item);
+//   pass 2  row FFTs of a and b, PHAT weighting, inverse row FFT, conjugate twiddle
+//           (F rows x n1 per item);
+//   pass 3  inverse column FFTs; every lag's |r| goes into a per-pair atomic max of
+//           (magnitude, first index) instead of being stored (unless the caller asks for r);
+//   finish  per pair: |r| at the peak's two neighbours recomputed from the pass-2 columns, and
+//           the sub-sample offset in float64 from the two-sample sinc ratio: a whitened pure
+//           delay d has |r[k]| = |sinc(k - d)|, so with m0 = |r[peak]| and m1 the larger
+//           neighbour, |sinc(1 - f)| / |sinc(f)| = f / (1 - f) gives f = m1 / (m0 + m1),
+//           towards that neighbour.
+// Pairs need 4096 <= L <= 2^24 (L = next power of two >= Na + Nb - 1, so Na + Nb - 1 > 2048);
+// GCC-PHAT's weights depend on L, so shorter pairs are not padded up but refused (the
+// single-pair rf_xcorr_fft handles them in one workgroup). Items are spread over the grid
+// with a fixed stride: all items cost the same.
+
+#include "fft_plan.hpp"
+#include "twiddle.hpp"
+#include "xcorr_common.hpp"
+
+#include <algorithm>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace {
+
+struct PairDesc {
+    const float2* a;
+    const float2* b;
+    float2* r;        // optional full output (num_a + num_b - 1 values)
+    float2* work;     // 2 L values
+    const float2* lo;
+    const float2* hi;
+    int num_a;
+    int num_b;
+    int log2L;
+    int log2n1;
+    int log2n2;
+    int col_items;    // n1 / C
+    int row_items;    // n2 / F
+    int pad;
+};
+
+}  // namespace
+
+struct rf_gcc_batch {
+    int count = 0;
+    std::vector<PairDesc> host;
+    PairDesc* dev = nullptr;
+    int* col_prefix = nullptr;  // count + 1
+    int* row_prefix = nullptr;  // count + 1
+    int col_total = 0;
+    int row_total = 0;
+    float2* work = nullptr;
+    std::map<int, float2*> tables;
+    std::size_t bytes = 0;
+    int grid_mult = 2;          // persistent workgroups per CU
+    hipStream_t capture = nullptr;
+    hipGraphExec_t graph = nullptr;
+    rf_peak_result* graph_results = nullptr;
+};
+
+namespace rf {
+namespace {
+
+constexpr int kItemThreads = 512;
+
+__device__ __forceinline__ int find_pair(const int* __restrict__ prefix, int count, int item)
+{
+    int lo = 0;
+    int hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= item) {
+            lo = mid;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    return lo;
+}
+
+__device__ __forceinline__ FourStep shape_of(const PairDesc& d)
+{
+    FourStep fs;
+    fs.log2L = d.log2L;
+    fs.log2n1 = d.log2n1;
+    fs.log2n2 = d.log2n2;
+    fs.lo = d.lo;
+    fs.hi = d.hi;
+    return fs;
+}
+
+__global__ void batch_init_kernel(rf_peak_result* results, int count)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) {
+        *reinterpret_cast<unsigned long long*>(&results[i]) = 0ULL;
+        results[i].frac = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(kItemThreads) void batch_column_fwd_kernel(
+    const PairDesc* __restrict__ pairs, const int* __restrict__ prefix, int count, int total,
+    const float2* __restrict__ table)
+{
+    extern __shared__ float2 lds[];
+    for (int item = blockIdx.x; item < 2 * total; item += gridDim.x) {
+        const int which = item >= total ? 1 : 0;
+        const int local = item - which * total;
+        const int p = find_pair(prefix, count, local);
+        const PairDesc d = pairs[p];
+        const FourStep fs = shape_of(d);
+        const long long L = 1LL << d.log2L;
+        const int log2n1 = d.log2n1;
+        const long long c0 = static_cast<long long>(local - prefix[p]) * column_tile(fs);
+        float2* dst = d.work + (which ? L : 0);
+        auto store = [=](long long j1, int k2, float2 v) {
+            dst[j1 + (static_cast<long long>(k2) << log2n1)] = v;
+        };
+        if (which == 0) {
+            const float2* a = d.a;
+            const int na = d.num_a;
+            column_pass_block<false>(
+                fs, c0,
+                [=](long long j1, int j2) {
+                    return rotated_a(a, na, L, j1 + (static_cast<long long>(j2) << log2n1));
+                },
+                store, lds, table);
+        } else {
+            const float2* b = d.b;
+            const int nb = d.num_b;
+            column_pass_block<false>(
+                fs, c0,
+                [=](long long j1, int j2) {
+                    return padded_b(b, nb, j1 + (static_cast<long long>(j2) << log2n1));
+                },
+                store, lds, table);
+        }
+        __syncthreads();  // the next item reuses the LDS
+    }
+}
+
+__global__ __launch_bounds__(kItemThreads) void batch_row_kernel(
+    const PairDesc* __restrict__ pairs, const int* __restrict__ prefix, int count, int total,
+    const float2* __restrict__ table)
+{
+    extern __shared__ float2 lds[];
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const int p = find_pair(prefix, count, item);
+        const PairDesc d = pairs[p];
+        const FourStep fs = shape_of(d);
+        const long long L = 1LL << d.log2L;
+        const int n1 = 1 << d.log2n1;
+        const int F = lds_batch_per_block(d.log2n1);
+        const int slots = lds_slots(n1);
+        const int row0 = (item - prefix[p]) * F;
+        float2* wa = d.work + static_cast<long long>(row0) * n1;
+        float2* wb = d.work + L + static_cast<long long>(row0) * n1;
+        float2* la = lds;
+        float2* lb = lds + F * slots;
+        const int total_e = F * n1;
+        for (int e = threadIdx.x; e < total_e; e += blockDim.x) {
+            const int s = (e >> d.log2n1) * slots + lds_pad(e & (n1 - 1));
+            la[s] = wa[e];
+            lb[s] = wb[e];
+        }
+        __syncthreads();
+        const int per = n1 >> 3;
+        const int f = threadIdx.x / per;
+        const int t = threadIdx.x - f * per;
+        lds_fft<false>(la + f * slots, d.log2n1, t, true, table);
+        lds_fft<false>(lb + f * slots, d.log2n1, t, true, table);
+        const float scale = 1.0f / static_cast<float>(L);
+        for (int e = threadIdx.x; e < total_e; e += blockDim.x) {
+            const int s = (e >> d.log2n1) * slots + lds_pad(e & (n1 - 1));
+            lb[s] = weigh(la[s], lb[s], scale, 1);
+        }
+        __syncthreads();
+        lds_fft<true>(lb + f * slots, d.log2n1, t, true, table);
+        for (int e = threadIdx.x; e < total_e; e += blockDim.x) {
+            const int rr = e >> d.log2n1;
+            const int j1 = e & (n1 - 1);
+            const float2 v = lb[rr * slots + lds_pad(j1)];
+            wb[e] = cmul(v, four_step_twiddle<true>(fs, static_cast<long long>(j1) * (row0 + rr)));
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kItemThreads) void batch_column_inv_peak_kernel(
+    const PairDesc* __restrict__ pairs, const int* __restrict__ prefix, int count, int total,
+    rf_peak_result* results, const float2* __restrict__ table)
+{
+    extern __shared__ float2 lds[];
+    __shared__ unsigned long long wave_best[kItemThreads / 64];
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const int p = find_pair(prefix, count, item);
+        const PairDesc d = pairs[p];
+        const FourStep fs = shape_of(d);
+        const long long L = 1LL << d.log2L;
+        const int log2n1 = d.log2n1;
+        const int n2 = 1 << d.log2n2;
+        const int C = column_tile(fs);
+        const int slots = lds_slots(n2);
+        const long long c0 = static_cast<long long>(item - prefix[p]) * C;
+        const float2* src = d.work + L;
+        const int total_e = C * n2;
+        for (int e = threadIdx.x; e < total_e; e += blockDim.x) {
+            const int row = e / C;
+            const int col = e - row * C;
+            lds[col * slots + lds_pad(row)] = src[(c0 + col) + (static_cast<long long>(row) << log2n1)];
+        }
+        __syncthreads();
+        const int per = n2 >> 3;
+        const int f = threadIdx.x / per;
+        const int t = threadIdx.x - f * per;
+        lds_fft<true>(lds + f * slots, d.log2n2, t, true, table);
+        const long long lags = static_cast<long long>(d.num_a) + d.num_b - 1;
+        unsigned long long best = 0ULL;
+        for (int e = threadIdx.x; e < total_e; e += blockDim.x) {
+            const int j2 = e / C;
+            const int col = e - j2 * C;
+            const long long n = (c0 + col) + (static_cast<long long>(j2) << log2n1);
+            if (n < lags) {
+                const float2 v = lds[col * slots + lds_pad(j2)];
+                const unsigned long long key = peak_key(hypotf(v.x, v.y), n);
+                best = key > best ? key : best;
+                if (d.r != nullptr) {
+                    d.r[n] = v;
+                }
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long other = __shfl_xor(best, off, 64);
+            best = other > best ? other : best;
+        }
+        if ((threadIdx.x & 63) == 0) {
+            wave_best[threadIdx.x >> 6] = best;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < kItemThreads / 64; ++w) {
+                best = wave_best[w] > best ? wave_best[w] : best;
+            }
+            atomicMax(reinterpret_cast<unsigned long long*>(&results[p]), best);
+        }
+        __syncthreads();
+    }
+}
+
+// r[n] from column j1 of the pass-2 output: sum_k2 Z[j1 + n1 k2] exp(+2 pi i j2 k2 / n2),
+// one wave, float32 terms summed in float64.
+__device__ double lag_magnitude(const PairDesc& d, const FourStep& fs, long long n)
+{
+    const long long L = 1LL << d.log2L;
+    const int n1 = 1 << d.log2n1;
+    const int n2 = 1 << d.log2n2;
+    const long long j1 = n & (n1 - 1);
+    const long long j2 = n >> d.log2n1;
+    const float2* col = d.work + L + j1;
+    double re = 0.0;
+    double im = 0.0;
+    for (int k2 = threadIdx.x; k2 < n2; k2 += 64) {
+        const long long m = ((j2 * k2) & (n2 - 1)) * n1;
+        const float2 w = four_step_twiddle<true>(fs, m);
+        const float2 z = col[static_cast<long long>(k2) * n1];
+        re += static_cast<double>(z.x) * w.x - static_cast<double>(z.y) * w.y;
+        im += static_cast<double>(z.x) * w.y + static_cast<double>(z.y) * w.x;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        re += __shfl_xor(re, off, 64);
+        im += __shfl_xor(im, off, 64);
+    }
+    return sqrt(re * re + im * im);
+}
+
+__global__ __launch_bounds__(64) void batch_finish_kernel(const PairDesc* __restrict__ pairs,
+                                                          rf_peak_result* results)
+{
+    const PairDesc d = pairs[blockIdx.x];
+    const FourStep fs = shape_of(d);
+    rf_peak_result* res = &results[blockIdx.x];
+    const unsigned long long key = *reinterpret_cast<const unsigned long long*>(res);
+    const long long lags = static_cast<long long>(d.num_a) + d.num_b - 1;
+    const int idx = static_cast<int>(0xFFFFFFFFu - static_cast<unsigned>(key & 0xFFFFFFFFULL));
+    const float mag = __uint_as_float(static_cast<unsigned>(key >> 32));
+    double frac = 0.0;
+    if (idx > 0 && idx + 1 < lags) {
+        const double ml = lag_magnitude(d, fs, idx - 1);
+        const double mh = lag_magnitude(d, fs, idx + 1);
+        const double m0 = static_cast<double>(mag);
+        if (m0 > 0.0) {
+            frac = mh >= ml ? mh / (m0 + mh) : -ml / (m0 + ml);
+            frac = frac > 0.5 ? 0.5 : (frac < -0.5 ? -0.5 : frac);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        res->index = idx;
+        res->magnitude = mag;
+        res->frac = frac;
+    }
+}
+
+std::size_t column_lds_bytes()
+{
+    // C * lds_slots(n2) with C * n2 = 4096: at most 4096 + 512 + C values, C <= 512.
+    return static_cast<std::size_t>(4096 + 512 + 512) * sizeof(float2);
+}
+
+std::size_t row_lds_bytes()
+{
+    // 2 F lds_slots(n1) with F n1 = 4096: at most 2 (4096 + 512 + F), F <= 4.
+    return static_cast<std::size_t>(2 * (4096 + 512 + 4)) * sizeof(float2);
+}
+
+void batch_run(rf_gcc_batch& h, rf_peak_result* results, hipStream_t stream)
+{
+    const float2* table = twiddle_table_4096();
+    const int cus = cu_count();
+    batch_init_kernel<<<static_cast<unsigned>(ceil_div(h.count, 256)), 256, 0, stream>>>(results,
+                                                                                         h.count);
+    check_launch("batch_init_kernel");
+    const int cap = h.grid_mult * cus;
+    const unsigned g1 = static_cast<unsigned>(std::min(2 * h.col_total, cap));
+    batch_column_fwd_kernel<<<g1, kItemThreads, column_lds_bytes(), stream>>>(
+        h.dev, h.col_prefix, h.count, h.col_total, table);
+    check_launch("batch_column_fwd_kernel");
+    const unsigned g2 = static_cast<unsigned>(std::min(h.row_total, cap));
+    batch_row_kernel<<<g2, kItemThreads, row_lds_bytes(), stream>>>(h.dev, h.row_prefix, h.count,
+                                                                    h.row_total, table);
+    check_launch("batch_row_kernel");
+    const unsigned g3 = static_cast<unsigned>(std::min(h.col_total, cap));
+    batch_column_inv_peak_kernel<<<g3, kItemThreads, column_lds_bytes(), stream>>>(
+        h.dev, h.col_prefix, h.count, h.col_total, results, table);
+    check_launch("batch_column_inv_peak_kernel");
+    batch_finish_kernel<<<static_cast<unsigned>(h.count), 64, 0, stream>>>(h.dev, results);
+    check_launch("batch_finish_kernel");
+}
+
+void destroy_batch(rf_gcc_batch* h)
+{
+    if (h == nullptr) {
+        return;
+    }
+    if (h->graph != nullptr) {
+        (void)hipGraphExecDestroy(h->graph);
+    }
+    if (h->capture != nullptr) {
+        (void)hipStreamDestroy(h->capture);
+    }
+    (void)hipFree(h->dev);
+    (void)hipFree(h->col_prefix);
+    (void)hipFree(h->row_prefix);
+    (void)hipFree(h->work);
+    for (auto& kv : h->tables) {
+        (void)hipFree(kv.second);
+    }
+    delete h;
+}
+
+}  // namespace
+}  // namespace rf
+
+using rf::guarded;
+
+extern "C" {
+
+rf_status rf_gcc_batch_create(int count, const int* num_a, const int* num_b,
+                              rf_gcc_batch** out)
+{
+    return guarded([&] {
+        if (out == nullptr) rf::fail(RF_INVALID_ARG, "gcc batch: NULL output handle");
+        *out = nullptr;
+        if (count < 1 || num_a == nullptr || num_b == nullptr) {
+            rf::fail(RF_INVALID_ARG, "gcc batch: need at least one pair and its lengths");
+        }
+        rf_gcc_batch* h = new rf_gcc_batch;
+        try {
+            h->count = count;
+            h->host.resize(static_cast<std::size_t>(count));
+            std::vector<int> colp(static_cast<std::size_t>(count) + 1, 0);
+            std::vector<int> rowp(static_cast<std::size_t>(count) + 1, 0);
+            long long work_total = 0;
+            for (int i = 0; i < count; ++i) {
+                if (num_a[i] < 1 || num_b[i] < 1) {
+                    rf::fail(RF_INVALID_ARG, "gcc batch: pair " + std::to_string(i)
+                                                 + " needs at least one sample in each input");
+                }
+                const long long lags = static_cast<long long>(num_a[i]) + num_b[i] - 1;
+                long long L = 1;
+                while (L < lags) {
+                    L *= 2;
+                }
+                if (L < 4096) {
+                    rf::fail(RF_LIMIT, "gcc batch: pair " + std::to_string(i) + " has "
+                                           + std::to_string(lags) + " lags; batched pairs need "
+                                           "more than 2048 (use rf_xcorr_fft)");
+                }
+                if (L > (1LL << 24)) {
+                    rf::fail(RF_LIMIT, "gcc batch: pair " + std::to_string(i) + " needs an FFT "
+                                           "longer than the supported 2^24 points");
+                }
+                const int log2L = rf::ilog2(L);
+                const rf::FourStep fs = rf::make_four_step_shape(log2L);
+                PairDesc& d = h->host[static_cast<std::size_t>(i)];
+                d = PairDesc{};
+                d.num_a = num_a[i];
+                d.num_b = num_b[i];
+                d.log2L = log2L;
+                d.log2n1 = fs.log2n1;
+                d.log2n2 = fs.log2n2;
+                d.col_items = (1 << fs.log2n1) / rf::column_tile(fs);
+                d.row_items = (1 << fs.log2n2) / rf::lds_batch_per_block(fs.log2n1);
+                colp[static_cast<std::size_t>(i) + 1] = colp[static_cast<std::size_t>(i)] + d.col_items;
+                rowp[static_cast<std::size_t>(i) + 1] = rowp[static_cast<std::size_t>(i)] + d.row_items;
+                work_total += 2 * L;
+            }
+            h->col_total = colp.back();
+            h->row_total = rowp.back();
+            (void)rf::twiddle_table_4096();
+            RF_HIP_CHECK(hipMalloc(&h->work, static_cast<std::size_t>(work_total) * sizeof(float2)));
+            long long offset = 0;
+            for (auto& d : h->host) {
+                d.work = h->work + offset;
+                offset += 2LL << d.log2L;
+                auto it = h->tables.find(d.log2L);
+                if (it == h->tables.end()) {
+                    float2* t = nullptr;
+                    const long long tsize = rf::four_step_table_size(d.log2L);
+                    RF_HIP_CHECK(hipMalloc(&t, static_cast<std::size_t>(tsize) * sizeof(float2)));
+                    rf::upload_four_step_tables(d.log2L, t);
+                    it = h->tables.emplace(d.log2L, t).first;
+                    h->bytes += static_cast<std::size_t>(tsize) * sizeof(float2);
+                }
+                d.lo = it->second;
+                d.hi = it->second + rf::kTwiddleSize;
+            }
+            RF_HIP_CHECK(hipMalloc(&h->dev, h->host.size() * sizeof(PairDesc)));
+            RF_HIP_CHECK(hipMalloc(&h->col_prefix, colp.size() * sizeof(int)));
+            RF_HIP_CHECK(hipMalloc(&h->row_prefix, rowp.size() * sizeof(int)));
+            RF_HIP_CHECK(hipMemcpy(h->col_prefix, colp.data(), colp.size() * sizeof(int),
+                                   hipMemcpyHostToDevice));
+            RF_HIP_CHECK(hipMemcpy(h->row_prefix, rowp.data(), rowp.size() * sizeof(int),
+                                   hipMemcpyHostToDevice));
+            RF_HIP_CHECK(hipMemcpy(h->dev, h->host.data(), h->host.size() * sizeof(PairDesc),
+                                   hipMemcpyHostToDevice));
+            h->bytes += static_cast<std::size_t>(work_total) * sizeof(float2)
+                        + h->host.size() * sizeof(PairDesc) + (colp.size() + rowp.size()) * sizeof(int);
+        } catch (...) {
+            rf::destroy_batch(h);
+            throw;
+        }
+        *out = h;
+    });
+}
+
+rf_status rf_gcc_batch_bind(rf_gcc_batch* h, const rf_cf32* const* a, const rf_cf32* const* b,
+                            rf_cf32* const* r)
+{
+    return guarded([&] {
+        if (h == nullptr || a == nullptr || b == nullptr) {
+            rf::fail(RF_INVALID_ARG, "gcc batch: NULL handle or channel arrays");
+        }
+        for (int i = 0; i < h->count; ++i) {
+            PairDesc& d = h->host[static_cast<std::size_t>(i)];
+            if (a[i] == nullptr || b[i] == nullptr) {
+                rf::fail(RF_INVALID_ARG, "gcc batch: NULL input of pair " + std::to_string(i));
+            }
+            d.a = reinterpret_cast<const float2*>(a[i]);
+            d.b = reinterpret_cast<const float2*>(b[i]);
+            d.r = r != nullptr ? reinterpret_cast<float2*>(r[i]) : nullptr;
+        }
+        RF_HIP_CHECK(hipDeviceSynchronize());  // no run may still read the old descriptors
+        RF_HIP_CHECK(hipMemcpy(h->dev, h->host.data(), h->host.size() * sizeof(PairDesc),
+                               hipMemcpyHostToDevice));
+    });
+}
+
+rf_status rf_gcc_batch_run(rf_gcc_batch* h, rf_peak_result* results, rf_stream stream)
+{
+    return guarded([&] {
+        if (h == nullptr || results == nullptr) rf::fail(RF_INVALID_ARG, "gcc batch: NULL");
+        if (h->host[0].a == nullptr) rf::fail(RF_INVALID_ARG, "gcc batch: run before bind");
+        rf::batch_run(*h, results, rf::as_stream(stream));
+    });
+}
+
+rf_status rf_gcc_batch_launch_graph(rf_gcc_batch* h, rf_peak_result* results, rf_stream stream)
+{
+    return guarded([&] {
+        if (h == nullptr || results == nullptr) rf::fail(RF_INVALID_ARG, "gcc batch: NULL");
+        if (h->host[0].a == nullptr) rf::fail(RF_INVALID_ARG, "gcc batch: run before bind");
+        if (h->graph == nullptr || h->graph_results != results) {
+            if (h->graph != nullptr) {
+                RF_HIP_CHECK(hipGraphExecDestroy(h->graph));
+                h->graph = nullptr;
+            }
+            if (h->capture == nullptr) {
+                RF_HIP_CHECK(hipStreamCreateWithFlags(&h->capture, hipStreamNonBlocking));
+            }
+            (void)rf::twiddle_table_4096();
+            hipGraph_t graph = nullptr;
+            RF_HIP_CHECK(hipStreamBeginCapture(h->capture, hipStreamCaptureModeThreadLocal));
+            try {
+                rf::batch_run(*h, results, h->capture);
+            } catch (...) {
+                (void)hipStreamEndCapture(h->capture, &graph);
+                if (graph != nullptr) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            RF_HIP_CHECK(hipStreamEndCapture(h->capture, &graph));
+            const hipError_t err = hipGraphInstantiate(&h->graph, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            RF_HIP_CHECK(err);
+            h->graph_results = results;
+        }
+        RF_HIP_CHECK(hipGraphLaunch(h->graph, rf::as_stream(stream)));
+    });
+}
+
+rf_status rf_gcc_batch_bytes(const rf_gcc_batch* h, size_t* bytes)
+{
+    return guarded([&] {
+        if (h == nullptr || bytes == nullptr) rf::fail(RF_INVALID_ARG, "gcc batch: NULL");
+        *bytes = h->bytes;
+    });
+}
+
+rf_status rf_gcc_batch_destroy(rf_gcc_batch* h)
+{
+    return guarded([&] { rf::destroy_batch(h); });
+}
+
+}  // extern "C"

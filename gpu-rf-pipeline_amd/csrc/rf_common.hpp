@@ -79,6 +79,8 @@ inline void check_launch(const char* what)
 int max_lds_per_block();
 // Compute units of the current device.
 int cu_count();
+// Enqueues the library's streaming device-to-device copy.
+void copy_device(const void* src, void* dst, size_t bytes, hipStream_t stream);
 
 inline long long ceil_div(long long a, long long b)
 {

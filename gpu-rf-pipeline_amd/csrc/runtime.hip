@@ -43,6 +43,47 @@ int cu_count()
     return cus;
 }
 
+namespace {
+
+// Device-to-device copy as a streaming kernel: 16-byte loads and stores when both pointers
+// allow it, a grid of a few workgroups per CU striding over the buffer. It is also the
+// reference point for the HBM-bandwidth and traffic-counter calibration.
+__global__ __launch_bounds__(256) void copy_kernel(const unsigned char* __restrict__ src,
+                                                   unsigned char* __restrict__ dst, size_t bytes)
+{
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    const size_t first = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool vec = ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15) == 0;
+    size_t done = 0;
+    if (vec) {
+        const size_t n16 = bytes / 16;
+        const float4* s4 = reinterpret_cast<const float4*>(src);
+        float4* d4 = reinterpret_cast<float4*>(dst);
+        for (size_t i = first; i < n16; i += stride) {
+            d4[i] = s4[i];
+        }
+        done = n16 * 16;
+    }
+    for (size_t i = done + first; i < bytes; i += stride) {
+        dst[i] = src[i];
+    }
+}
+
+}  // namespace
+
+void copy_device(const void* src, void* dst, size_t bytes, hipStream_t stream)
+{
+    if (bytes == 0) {
+        return;
+    }
+    const long long cap = 16LL * cu_count();
+    long long grid = ceil_div(static_cast<long long>(bytes / 16 + 1), 256);
+    grid = grid < cap ? grid : cap;
+    copy_kernel<<<static_cast<unsigned>(grid), 256, 0, stream>>>(
+        static_cast<const unsigned char*>(src), static_cast<unsigned char*>(dst), bytes);
+    check_launch("copy_kernel");
+}
+
 }  // namespace rf
 
 using rf::guarded;
@@ -133,6 +174,20 @@ rf_status rf_host_alloc(void** ptr, size_t bytes)
     });
 }
 
+rf_status rf_host_alloc_mapped(void** host, void** device, size_t bytes)
+{
+    return guarded([&] {
+        if (host == nullptr || device == nullptr) {
+            rf::fail(RF_INVALID_ARG, "rf_host_alloc_mapped: NULL output");
+        }
+        *host = nullptr;
+        *device = nullptr;
+        if (bytes == 0) return;
+        RF_HIP_CHECK(hipHostMalloc(host, bytes, hipHostMallocMapped));
+        RF_HIP_CHECK(hipHostGetDevicePointer(device, *host, 0));
+    });
+}
+
 rf_status rf_host_free(void* ptr)
 {
     return guarded([&] { RF_HIP_CHECK(hipHostFree(ptr)); });
@@ -158,8 +213,8 @@ rf_status rf_memcpy_d2d(void* dst, const void* src, size_t bytes, rf_stream stre
 {
     return guarded([&] {
         if (bytes == 0) return;
-        RF_HIP_CHECK(
-            hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, rf::as_stream(stream)));
+        if (dst == nullptr || src == nullptr) rf::fail(RF_INVALID_ARG, "rf_memcpy_d2d: NULL");
+        rf::copy_device(src, dst, bytes, rf::as_stream(stream));
     });
 }
 

@@ -1,0 +1,145 @@
+#pragma once
+
+// Device-pointer entry points with the reference's shape (src/device_ops.hpp:1-106), as thin
+// inline wrappers over the C ABI. Each one only enqueues work: no allocation, no host
+// transfers, no synchronisation. What changed, and why:
+//   - cufftHandle becomes rf_fft_plan* and cudaStream_t becomes rf_stream (a hipStream_t);
+//   - taps are read from the caller's pointer on every launch: there is no constant bank, so
+//     fir_load_const_taps only checks the reference's 8192-tap limit and fir_launch uses the
+//     `taps` it is given for every variant;
+//   - FirImpl gains FftOla (overlap-save with the library's FFT).
+
+#include "check.hpp"
+#include "device_buffer.hpp"
+#include "fft_plan.hpp"
+#include "types.hpp"
+
+#include <cstddef>
+#include <stdexcept>
+#include <string>
+
+// --- FIR ------------------------------------------------------------------------------------
+
+enum class FirImpl { Naive = RF_FIR_NAIVE, Tiled = RF_FIR_TILED, TiledConst = RF_FIR_TILED_CONST,
+                     Register = RF_FIR_REGISTER, FftOla = RF_FIR_FFT_OLA };
+
+inline bool parse_fir_impl(const char* name, FirImpl& impl)
+{
+    int idx = 0;
+    if (rf_fir_parse_impl(name, &idx) != RF_OK) {
+        return false;
+    }
+    impl = static_cast<FirImpl>(idx);
+    return true;
+}
+
+inline void fir_launch(FirImpl impl, const cf32* x, const float* taps, cf32* y, int num_samples,
+                       int num_taps, rf_stream stream)
+{
+    RF_CHECK(rf_fir(static_cast<int>(impl), as_rf(x), taps, as_rf(y), num_samples, num_taps, 0,
+                    stream));
+}
+
+inline void fir_load_const_taps(const float* /*taps*/, int num_taps, rf_stream /*stream*/)
+{
+    if (num_taps > RF_MAX_CONST_TAPS) {
+        throw std::runtime_error("fir: " + std::to_string(num_taps) + " taps, more than the "
+                                 + std::to_string(RF_MAX_CONST_TAPS)
+                                 + " the constant-cache variants accept; use naive");
+    }
+}
+
+inline void fir_device(FirImpl impl, const cf32* x, const float* taps, cf32* y, int num_samples,
+                       int num_taps)
+{
+    fir_launch(impl, x, taps, y, num_samples, num_taps, nullptr);
+}
+
+inline void fir_naive_device(const cf32* x, const float* taps, cf32* y, int num_samples,
+                             int num_taps)
+{
+    fir_launch(FirImpl::Naive, x, taps, y, num_samples, num_taps, nullptr);
+}
+
+constexpr int kMaxFirBatch = RF_MAX_FIR_BATCH;
+
+inline void fir_batched_device(FirImpl impl, const cf32* const* x, cf32* const* y,
+                               const int* num_samples, int count, const float* taps,
+                               int num_taps)
+{
+    RF_CHECK(rf_fir_batched(static_cast<int>(impl), reinterpret_cast<const rf_cf32* const*>(x),
+                            reinterpret_cast<rf_cf32* const*>(y), num_samples, count, taps,
+                            num_taps, nullptr));
+}
+
+inline void fir_naive_batched_device(const cf32* const* x, cf32* const* y,
+                                     const int* num_samples, int count, const float* taps,
+                                     int num_taps)
+{
+    fir_batched_device(FirImpl::Naive, x, y, num_samples, count, taps, num_taps);
+}
+
+// --- magnitude --------------------------------------------------------------------------------
+
+inline void magnitude_device(const cf32* x, float* mag, int num_samples)
+{
+    RF_CHECK(rf_magnitude(as_rf(x), mag, num_samples, nullptr));
+}
+
+// --- cross-correlation ------------------------------------------------------------------------
+
+inline void xcorr_naive_device(const cf32* a, const cf32* b, cf32* r, int num_a, int num_b)
+{
+    RF_CHECK(rf_xcorr_direct(RF_XCORR_NAIVE, as_rf(a), as_rf(b), as_rf(r), num_a, num_b, nullptr));
+}
+
+inline void xcorr_tiled_device(const cf32* a, const cf32* b, cf32* r, int num_a, int num_b)
+{
+    RF_CHECK(rf_xcorr_direct(RF_XCORR_TILED, as_rf(a), as_rf(b), as_rf(r), num_a, num_b, nullptr));
+}
+
+// FFT cross-correlation for fixed lengths; construction allocates, run() enqueues.
+class XcorrFft {
+public:
+    XcorrFft(int num_a, int num_b, bool phat = false)
+    {
+        RF_CHECK(rf_xcorr_fft_create(num_a, num_b, phat ? 1 : 0, &handle_));
+    }
+    ~XcorrFft() { rf_xcorr_fft_destroy(handle_); }
+    XcorrFft(const XcorrFft&) = delete;
+    XcorrFft& operator=(const XcorrFft&) = delete;
+
+    void run(const cf32* a, const cf32* b, cf32* r, rf_stream stream = nullptr) const
+    {
+        RF_CHECK(rf_xcorr_fft_run(handle_, as_rf(a), as_rf(b), as_rf(r), stream));
+    }
+    std::size_t bytes() const
+    {
+        std::size_t n = 0;
+        RF_CHECK(rf_xcorr_fft_bytes(handle_, &n));
+        return n;
+    }
+
+private:
+    rf_xcorr_fft* handle_ = nullptr;
+};
+
+// --- channelizer, spectrum ----------------------------------------------------------------------
+
+inline int channelize_frames(int num_samples, int channels)
+{
+    return rf_channelize_frames(num_samples, channels);
+}
+
+// `plan`: channels points x channelize_frames() frames, or nullptr for the fused kernel.
+inline void channelize_device(const rf_fft_plan* plan, const cf32* x, const float* taps, cf32* out,
+                              int num_samples, int channels, int num_taps)
+{
+    RF_CHECK(rf_channelize(plan, as_rf(x), taps, as_rf(out), num_samples, channels, num_taps,
+                           nullptr));
+}
+
+inline void spectrum_device(const rf_fft_plan* plan, const cf32* in, cf32* out)
+{
+    RF_CHECK(rf_spectrum(plan, as_rf(in), as_rf(out), nullptr));
+}

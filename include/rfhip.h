@@ -63,7 +63,11 @@ rf_status rf_malloc(void** ptr, size_t bytes);
 rf_status rf_free(void* ptr);
 rf_status rf_host_alloc(void** ptr, size_t bytes); /* page-locked host memory */
 rf_status rf_host_free(void* ptr);
-/* Asynchronous on `stream`; pageable host memory makes them effectively synchronous. */
+/* Page-locked host memory mapped into the device's address space (zero-copy): *host is the
+ * CPU address, *device the address kernels use. Free with rf_host_free(*host). */
+rf_status rf_host_alloc_mapped(void** host, void** device, size_t bytes);
+/* Asynchronous on `stream`; pageable host memory makes them effectively synchronous.
+ * rf_memcpy_d2d runs the library's streaming copy kernel (16-byte accesses). */
 rf_status rf_memcpy_h2d(void* dst, const void* src, size_t bytes, rf_stream stream);
 rf_status rf_memcpy_d2h(void* dst, const void* src, size_t bytes, rf_stream stream);
 rf_status rf_memcpy_d2d(void* dst, const void* src, size_t bytes, rf_stream stream);
@@ -196,6 +200,33 @@ typedef struct rf_peak_result {
     double frac;
 } rf_peak_result;
 rf_status rf_peak(const rf_cf32* r, int n, rf_peak_result* result, rf_stream stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Batched GCC-PHAT with the peak search fused in. No single reference function: it does the
+ * work of gcc_phat_cuda (src/xcorr.cu:379-399) plus the host peak loop of rfgpu doa
+ * (src/main.cpp:390-401) for `count` pairs of uneven length in one pass sequence of
+ * persistent kernels (column FFTs, row FFTs + PHAT weighting + inverse row FFTs, inverse
+ * column FFTs + peak), optionally replayed as one hipGraph.
+ * Each pair needs num_a + num_b - 1 > 2048 (FFT length 4096 .. 2^24); RF_LIMIT otherwise.
+ * ------------------------------------------------------------------------------------- */
+typedef struct rf_gcc_batch rf_gcc_batch;
+/* Setup: per-pair FFT shapes, work buffers (2 L values per pair), twiddle tables. */
+rf_status rf_gcc_batch_create(int count, const int* num_a, const int* num_b,
+                              rf_gcc_batch** handle);
+rf_status rf_gcc_batch_destroy(rf_gcc_batch* handle);
+rf_status rf_gcc_batch_bytes(const rf_gcc_batch* handle, size_t* bytes);
+/* Binds the inputs: a, b are HOST arrays of `count` device pointers; r is NULL (peaks only)
+ * or a HOST array of `count` device pointers (each NULL or num_a + num_b - 1 values) that also
+ * receive the full GCC-PHAT output. Synchronises the device (setup, not per run). */
+rf_status rf_gcc_batch_bind(rf_gcc_batch* handle, const rf_cf32* const* a,
+                            const rf_cf32* const* b, rf_cf32* const* r);
+/* Enqueues the batch on `stream`. results: DEVICE array of `count` rf_peak_result; index is
+ * the output index of pair i's peak (lag = index - (num_a - 1)). */
+rf_status rf_gcc_batch_run(rf_gcc_batch* handle, rf_peak_result* results, rf_stream stream);
+/* The same as one hipGraph: captured on first use (and again when `results` changes), then
+ * replayed with a single launch. */
+rf_status rf_gcc_batch_launch_graph(rf_gcc_batch* handle, rf_peak_result* results,
+                                    rf_stream stream);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -1,0 +1,133 @@
+"""Batched GCC-PHAT (rf_gcc_batch_*) against the float64 oracle and the single-pair path.
+
+Every pair's full output is compared with the oracle (tolerance of the reference's GCC-PHAT
+tests, atol 1e-4, tests/test_pipeline.py:408-411 of the reference); the fused peak must be the
+first maximum of the GPU's own output and the oracle's peak lag (bit-exact, with a stated tie
+rule); graph replays must reproduce the eager run exactly.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from _common import load_rfhip, oracle
+
+pytestmark = pytest.mark.gpu
+rf = load_rfhip()
+
+GCC_ATOL = 1e-4
+
+
+def delayed_pair(seed: int, na: int, nb: int, delay: int, noise: float):
+    rng = np.random.default_rng(seed)
+    src = rng.standard_normal(max(na, nb) + abs(delay) + 8) + 1j * rng.standard_normal(max(na, nb) + abs(delay) + 8)
+    a = src[abs(delay): abs(delay) + na]
+    b = src[abs(delay) - delay: abs(delay) - delay + nb]
+    a = a + noise * (rng.standard_normal(na) + 1j * rng.standard_normal(na))
+    b = b + noise * (rng.standard_normal(nb) + 1j * rng.standard_normal(nb))
+    return a.astype(np.complex64), b.astype(np.complex64)
+
+
+LENGTHS = [(2100, 2000, 13), (4096, 4096, -77), (5000, 3000, 150), (9000, 9001, 0),
+           (20000, 16000, -200), (33000, 33000, 199), (70000, 65536, 5), (131072, 131072, -1),
+           (300000, 200000, 123), (1 << 20, 1 << 20, -150), (3000, 70000, 40), (50000, 2500, -33)]
+
+
+@pytest.fixture(scope="module")
+def batch():
+    pairs = [delayed_pair(i, na, nb, d, 0.3) for i, (na, nb, d) in enumerate(LENGTHS)]
+    d_a = [rf.DeviceBuffer.from_array(a) for a, _ in pairs]
+    d_b = [rf.DeviceBuffer.from_array(b) for _, b in pairs]
+    d_r = [rf.DeviceBuffer(a.size + b.size - 1) for a, b in pairs]
+    h = rf.GccBatch([a.size for a, _ in pairs], [b.size for _, b in pairs])
+    h.bind([x.ptr for x in d_a], [x.ptr for x in d_b], [x.ptr for x in d_r])
+    res = rf.DeviceBuffer(16 * len(pairs), np.uint8)
+    h.run(res.ptr)
+    peaks = rf.read_peaks(res, len(pairs))
+    outs = [r.download() for r in d_r]
+    return pairs, h, d_a, d_b, d_r, res, peaks, outs
+
+
+def test_batched_outputs_match_the_oracle(batch) -> None:
+    pairs, _, _, _, _, _, _, outs = batch
+    for (a, b), got in zip(pairs, outs):
+        np.testing.assert_allclose(got, oracle.gcc_phat(a, b), rtol=0, atol=GCC_ATOL)
+
+
+def test_fused_peak_is_the_first_maximum_and_the_oracle_lag(batch) -> None:
+    pairs, _, _, _, _, _, peaks, outs = batch
+    for (a, b), (na, nb, delay), pk, got in zip(pairs, LENGTHS, peaks, outs):
+        mags = np.abs(got.astype(np.complex128))
+        assert pk.index == int(np.argmax(np.abs(got)))
+        ref_idx, ref_mag = oracle.peak(oracle.gcc_phat(a, b))
+        # Tie rule: a different index is accepted only if the oracle's float64 magnitudes
+        # of the two candidates differ by less than the float32 error bound.
+        if pk.index != ref_idx:
+            ref = np.abs(oracle.gcc_phat(a, b))
+            assert abs(ref[pk.index] - ref[ref_idx]) < 2 * GCC_ATOL
+        assert pk.index - (na - 1) == delay
+        assert pk.magnitude == pytest.approx(mags[pk.index], rel=1e-6)
+
+
+def test_single_pair_path_agrees(batch) -> None:
+    pairs, _, _, _, _, _, peaks, outs = batch
+    for (a, b), pk, got in list(zip(pairs, peaks, outs))[:6]:
+        single = rf.gcc_phat(a, b)
+        np.testing.assert_allclose(got, single, rtol=0, atol=GCC_ATOL)
+        assert int(np.argmax(np.abs(single))) == pk.index
+
+
+def test_graph_replays_match_the_eager_run(batch) -> None:
+    pairs, h, d_a, d_b, d_r, res, peaks, outs = batch
+    want = res.download().tobytes()
+    g_res = rf.DeviceBuffer(16 * len(pairs), np.uint8)
+    for replay in range(4):
+        g_res.zero()
+        rf.synchronize()
+        h.run(g_res.ptr, graph=True)
+        rf.synchronize()
+        assert g_res.download().tobytes() == want, f"replay {replay}"
+        # eager work between replays
+        h.run(res.ptr)
+        rf.synchronize()
+        assert res.download().tobytes() == want
+
+
+def test_peaks_only_mode_gives_the_same_peaks(batch) -> None:
+    pairs, h, d_a, d_b, d_r, res, peaks, outs = batch
+    h2 = rf.GccBatch([a.size for a, _ in pairs], [b.size for _, b in pairs])
+    h2.bind([x.ptr for x in d_a], [x.ptr for x in d_b])
+    r2 = rf.DeviceBuffer(16 * len(pairs), np.uint8)
+    h2.run(r2.ptr)
+    assert [(p.index, p.magnitude) for p in rf.read_peaks(r2, len(pairs))] == \
+        [(p.index, p.magnitude) for p in peaks]
+    h2.destroy()
+
+
+def test_fractional_delay_estimate() -> None:
+    # A pure fractional delay of a white full-band signal: b = a shifted by d in frequency.
+    n = 1 << 14
+    rng = np.random.default_rng(77)
+    a = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    truths = [10.3, -20.45, 7.0, 3.71, -0.25]
+    pairs = []
+    for d in truths:
+        f = np.fft.fftfreq(n)
+        b = np.fft.ifft(np.fft.fft(a) * np.exp(-2j * np.pi * f * d))
+        pairs.append((a.astype(np.complex64), b.astype(np.complex64)))
+    d_a = [rf.DeviceBuffer.from_array(x) for x, _ in pairs]
+    d_b = [rf.DeviceBuffer.from_array(y) for _, y in pairs]
+    h = rf.GccBatch([n] * len(pairs), [n] * len(pairs))
+    h.bind([x.ptr for x in d_a], [x.ptr for x in d_b])
+    res = rf.DeviceBuffer(16 * len(pairs), np.uint8)
+    h.run(res.ptr)
+    for d, pk in zip(truths, rf.read_peaks(res, len(pairs))):
+        est = pk.index - (n - 1) + pk.frac
+        assert est == pytest.approx(d, abs=0.1), (d, est)
+    h.destroy()
+
+
+def test_short_pairs_are_refused() -> None:
+    with pytest.raises(rf.RfError) as e:
+        rf.GccBatch([1000, 5000], [1000, 5000])
+    assert e.value.status == rf.RF_LIMIT
