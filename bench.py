@@ -139,7 +139,7 @@ def bench_fir(rf, args, long_filter: bool) -> dict:
 
     variants = {}
     for name in [v.name for v in rf.fir_impls()]:
-        if name == args.impl or name == "naive" and long_filter:
+        if args.no_variants or name == args.impl or name == "naive" and long_filter:
             continue
         vms = time_steps(rf, step_for(rf.fir_impl_index(name)), max(2, args.steps // 5), 1,
                          stream.handle) / max(2, args.steps // 5)
@@ -258,8 +258,9 @@ def bench_gcc_batch(rf, args) -> dict:
     stream = rf.Stream()
     d_a = [rf.DeviceBuffer.from_array(a) for a, _ in pairs]
     d_b = [rf.DeviceBuffer.from_array(b) for _, b in pairs]
-    na = [a.size for a, _ in pairs]
-    nb = [b.size for _, b in pairs]
+    na = [int(a.size) for a, _ in pairs]
+    nb = [int(b.size) for _, b in pairs]
+    delays = [float(d) for d in delays]
     h = rf.GccBatch(na, nb)
     h.bind([x.ptr for x in d_a], [x.ptr for x in d_b])
     res = rf.DeviceBuffer(16 * count, np.uint8)
@@ -281,13 +282,13 @@ def bench_gcc_batch(rf, args) -> dict:
 
     ms_plain = time_steps(rf, plain, max(2, args.steps // 4), 1, stream.handle) / max(2, args.steps // 4)
     plain_peaks = rf.read_peaks(pres, count)
-    same = sum(p.index == q.index for p, q in zip(peaks, plain_peaks))
-    lag_err = [abs(p.index - (n - 1) + p.frac - d) for p, n, d in zip(peaks, na, delays)]
-    within = sum(e <= 0.1 for e in lag_err)
+    same = int(sum(p.index == q.index for p, q in zip(peaks, plain_peaks)))
+    lag_err = [float(abs(p.index - (n - 1) + p.frac - d)) for p, n, d in zip(peaks, na, delays)]
+    within = int(sum(e <= 0.1 for e in lag_err))
     checked = None
     if not args.no_cpu:
         oracle = load_oracle()
-        order = np.argsort(na)[:12]
+        order = [int(k) for k in np.argsort(na)[:12]]
         t0 = time.perf_counter()
         agree = 0
         for k in order:
@@ -295,7 +296,7 @@ def bench_gcc_batch(rf, args) -> dict:
             agree += int(idx == peaks[k].index)
         dt = time.perf_counter() - t0
         cpu_pairs_per_s = len(order) / dt
-        checked = {"pairs": int(len(order)), "lag_equal_to_oracle": agree}
+        checked = {"pairs": len(order), "lag_equal_to_oracle": int(agree)}
     bytes_step = sum(16 * (x + y) for x, y in zip(na, nb))
     best = min(ms_eager, ms_graph)
     return {
@@ -366,6 +367,8 @@ def main() -> int:
     p.add_argument("--workload", default="fir_long",
                    choices=["fir_long", "fir_plain", "channelize", "gcc_batch", "copy"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--no-variants", action="store_true",
+                   help="FIR: time only --impl, not the other variants")
     args = p.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: the hot path runs on one GPU; multi-GPU runs are not part of this "

@@ -57,8 +57,11 @@ constexpr int kChanOutputs = 4096;  // frames * M per workgroup
 constexpr int kChanMaxWindowTaps = 16;
 
 // Branch sums of one lane for M <= 512: branch m, frames 8g .. 8g+7 of the workgroup.
-// PB (1..16) taps per branch as a compile-time window; lds holds
-// x[t0 M - pb M + 1 + i].
+// PB (1..16) taps per branch at compile time; lds holds x[t0 M - pb M + 1 + i]. The window of
+// 8 + PB - 1 samples along the branch is streamed from the newest sample to the oldest, each
+// sample going into every output that uses it, so only the eight accumulators and the taps
+// stay live, and each output still sums its taps in ascending order (p = 0, 1, ...), as
+// src/channelize.cu:40-48 does.
 template <int PB>
 __device__ __forceinline__ void branch_window(const float2* lds, const float* __restrict__ taps,
                                               int num_taps, int M, int m, int g, float2* acc)
@@ -69,23 +72,22 @@ __device__ __forceinline__ void branch_window(const float2* lds, const float* __
         const int l = p * M + m;
         h[p] = l < num_taps ? taps[l] : 0.0f;
     }
-    constexpr int W = 8 + PB - 1;
-    float2 w[W];
-#pragma unroll
-    for (int q = 0; q < W; ++q) {
-        w[q] = lds[(8 * g + 1 + q) * M - 1 - m];
-    }
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-        float re = 0.0f;
-        float im = 0.0f;
+        acc[j] = make_float2(0.0f, 0.0f);
+    }
+    constexpr int W = 8 + PB - 1;
 #pragma unroll
-        for (int p = 0; p < PB; ++p) {
-            const float2 v = w[j - p + PB - 1];
-            re = fmaf(h[p], v.x, re);
-            im = fmaf(h[p], v.y, im);
+    for (int q = W - 1; q >= 0; --q) {
+        const float2 v = lds[(8 * g + 1 + q) * M - 1 - m];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int p = j - q + PB - 1;  // output j uses window sample q with tap p
+            if (p >= 0 && p < PB) {
+                acc[j].x = fmaf(h[p], v.x, acc[j].x);
+                acc[j].y = fmaf(h[p], v.y, acc[j].y);
+            }
         }
-        acc[j] = make_float2(re, im);
     }
 }
 
@@ -105,16 +107,28 @@ __device__ __forceinline__ float2 branch_generic(const float2* lds, const float*
     return make_float2(re, im);
 }
 
-template <int PB>
+// LOG2M != 0 fixes the channel count at compile time (two radix-8 passes for M = 64, the
+// common filterbank size), which keeps the frame FFTs in about half the registers of the
+// general-length transform.
+template <int PB, int LOG2M>
 __global__ __launch_bounds__(kChanThreads) void channelize_fused_kernel(
     const float2* __restrict__ x, const float* __restrict__ taps, float2* __restrict__ out,
-    int n, int log2M, int num_taps, long long frames, const float2* __restrict__ table)
+    int n, int log2M_rt, int num_taps, long long frames, const float2* __restrict__ table)
 {
     extern __shared__ float2 lds[];
+    const int log2M = LOG2M ? LOG2M : log2M_rt;
     const int M = 1 << log2M;
     const int F = kChanOutputs >> log2M;
     const int pb = PB ? PB : (num_taps + M - 1) / M;
-    const long long t0 = static_cast<long long>(blockIdx.x) * F;
+    // XCD-major order: workgroup b runs on XCD b % 8, which gets consecutive frame groups, so
+    // the (pb-1) M history each group re-reads was just loaded into that XCD's L2.
+    const long long groups = (frames + F - 1) / F;
+    const long long per_xcd = (groups + 7) / 8;
+    const long long group = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (group >= groups) {
+        return;  // the whole workgroup: padding of the grid to a multiple of 8
+    }
+    const long long t0 = group * F;
     const long long base = t0 * M - static_cast<long long>(pb) * M + 1;
     const int len = (F + pb - 1) * M;
     for (int i = threadIdx.x; i < len; i += kChanThreads) {
@@ -122,8 +136,9 @@ __global__ __launch_bounds__(kChanThreads) void channelize_fused_kernel(
     }
     __syncthreads();
 
+    // PB != 0 is only launched for M <= 512 (try_fused), where each lane owns one branch.
     float2 acc[8];
-    const bool windowed = PB != 0 && M <= kChanThreads;
+    constexpr bool windowed = PB != 0;
     if (windowed) {
         const int m = threadIdx.x & (M - 1);
         const int g = threadIdx.x >> log2M;
@@ -158,7 +173,12 @@ __global__ __launch_bounds__(kChanThreads) void channelize_fused_kernel(
     const int per = M >> 3;
     const int f = threadIdx.x / per;
     const int t = threadIdx.x - f * per;
-    lds_fft<true>(lds + f * slots, log2M, t, f < count, table);
+    if (LOG2M == 6) {
+        lds_pass<true, 3>(lds + f * slots, 64, t, 1, f < count, table);
+        lds_pass<true, 3>(lds + f * slots, 64, t, 8, f < count, table);
+    } else {
+        lds_fft<true>(lds + f * slots, log2M, t, f < count, table);
+    }
     const int total = count * M;
     float2* dst = out + t0 * M;
     for (int e = threadIdx.x; e < total; e += kChanThreads) {
@@ -180,8 +200,8 @@ void launch_fused(const float2* x, const float* taps, float2* out, int n, int lo
                   int num_taps, long long frames, std::size_t bytes, hipStream_t stream)
 {
     const int F = kChanOutputs >> log2M;
-    const unsigned grid = static_cast<unsigned>(ceil_div(frames, F));
-    auto kernel = channelize_fused_kernel<PB>;
+    const unsigned grid = static_cast<unsigned>(8 * ceil_div(ceil_div(frames, F), 8));
+    auto kernel = log2M == 6 ? channelize_fused_kernel<PB, 6> : channelize_fused_kernel<PB, 0>;
     if (bytes > 64 * 1024
         && hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -111,6 +111,77 @@ __device__ __forceinline__ void dft8(float2* v)
     v[7] = csub(e3, t3);
 }
 
+// u *= exp(-+2 pi i e / 16) for a compile-time e (after full unrolling); the quarter turns
+// are exact swaps.
+template <bool INV>
+__device__ __forceinline__ float2 rot16(float2 u, int e)
+{
+    e &= 15;
+    if (e == 0) {
+        return u;
+    }
+    if (e == 4) {
+        return INV ? make_float2(-u.y, u.x) : make_float2(u.y, -u.x);
+    }
+    if (e == 8) {
+        return make_float2(-u.x, -u.y);
+    }
+    if (e == 12) {
+        return INV ? make_float2(u.y, -u.x) : make_float2(-u.y, u.x);
+    }
+    // cos and sin of 2 pi e / 16 for e = 0..3 (the rest follow by quarter turns)
+    const float c1 = 0.92387953251128675613f;  // cos(pi / 8)
+    const float s1 = 0.38268343236508977173f;  // sin(pi / 8)
+    const float h = 0.70710678118654752440f;   // cos(pi / 4)
+    const float cs[4] = {1.0f, c1, h, s1};
+    const float sn[4] = {0.0f, s1, h, c1};
+    const int q = e & 3;
+    const int quarter = e >> 2;
+    float c = cs[q];
+    float sv = sn[q];
+    // rotate (c, s) by quarter * 90 degrees: (c, s) -> (-s, c)
+    for (int i = 0; i < quarter; ++i) {
+        const float tmp = c;
+        c = -sv;
+        sv = tmp;
+    }
+    // forward: multiply by (c - i s); inverse: by (c + i s)
+    const float si = INV ? sv : -sv;
+    return make_float2(u.x * c - u.y * si, u.x * si + u.y * c);
+}
+
+// 16-point DFT as 4 x 4: DFT4 over n1 of x[4 n1 + n2], twiddle w16^(n2 k1), DFT4 over n2;
+// output X[k1 + 4 k2] back into v in natural order.
+template <bool INV>
+__device__ __forceinline__ void dft16(float2* v)
+{
+    float2 y[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        float2 a = v[n2];
+        float2 b = v[4 + n2];
+        float2 c = v[8 + n2];
+        float2 d = v[12 + n2];
+        dft4<INV>(a, b, c, d);
+        y[4 * n2 + 0] = a;
+        y[4 * n2 + 1] = rot16<INV>(b, n2);
+        y[4 * n2 + 2] = rot16<INV>(c, 2 * n2);
+        y[4 * n2 + 3] = rot16<INV>(d, 3 * n2);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        float2 a = y[k1];
+        float2 b = y[4 + k1];
+        float2 c = y[8 + k1];
+        float2 d = y[12 + k1];
+        dft4<INV>(a, b, c, d);
+        v[k1] = a;
+        v[k1 + 4] = b;
+        v[k1 + 8] = c;
+        v[k1 + 12] = d;
+    }
+}
+
 // One radix-2^lr butterfly on u[0 .. R-1] with its Stockham twiddles. `k` = i mod p and
 // `tw_step` = 4096 / (p R): u[r] *= W^(r k tw_step).
 template <bool INV>
@@ -136,59 +207,64 @@ __device__ __forceinline__ void twiddle_and_dft(float2* u, int lr, int k,
     }
 }
 
+// One pass of radix R = 2^LR (compile-time, so the eight values stay in registers): each
+// thread reads eight values (one radix-8, two radix-4 or four radix-2 butterflies), waits at a
+// barrier, and writes its results.
+template <bool INV, int LR>
+__device__ __forceinline__ void lds_pass(float2* buf, int P, int t, int p, bool active,
+                                         const float2* __restrict__ table)
+{
+    constexpr int R = 1 << LR;
+    constexpr int NB = 8 >> LR;  // butterflies per thread
+    const int eighth = P >> 3;
+    const int stride = P >> LR;  // P / R
+    float2 u[8];
+    if (active) {
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const int bi = b >> LR;
+            const int r = b & (R - 1);
+            u[b] = buf[lds_pad(t + bi * eighth + r * stride)];
+        }
+    }
+    __syncthreads();
+    if (active) {
+        const int tw_step = kTwiddleSize / (p * R);
+#pragma unroll
+        for (int bi = 0; bi < NB; ++bi) {
+            const int i = t + bi * eighth;
+            const int k = i & (p - 1);
+            twiddle_and_dft<INV>(u + bi * R, LR, k, table, tw_step);
+            const int j = (i - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                buf[lds_pad(j + r * p)] = u[bi * R + r];
+            }
+        }
+    }
+    __syncthreads();
+}
+
 // A transform of P = 2^log2P (8 <= P <= 4096) points done by P/8 threads of a workgroup on
-// data already in LDS (padded layout), in place, natural order in and out. Each thread reads
-// eight values per pass (one radix-8 butterfly, two radix-4 or four radix-2 butterflies),
-// so the pass needs one barrier between the reads and the writes. `t` is the thread's index
-// within the transform; every thread of the workgroup must call this (it has barriers).
-// `active` is false for threads that only take part in the barriers.
+// data already in LDS (padded layout), in place, natural order in and out. Every thread of the
+// workgroup must call this (it has barriers); `t` is the thread's index within the transform
+// and `active` is false for threads that only take part in the barriers.
 template <bool INV>
 __device__ void lds_fft(float2* buf, int log2P, int t, bool active,
                         const float2* __restrict__ table)
 {
     const int P = 1 << log2P;
-    const int eighth = P >> 3;
-    const int passes = pass_count(log2P);
+    const int full8 = log2P / 3;
     int p = 1;
-    for (int pass = 0; pass < passes; ++pass) {
-        const int lr = pass_radix_log2(log2P, pass);
-        const int R = 1 << lr;
-        const int nb = 8 >> lr;       // butterflies per thread
-        const int stride = P >> lr;   // P / R
-        float2 u[8];
-        if (active) {
-#pragma unroll
-            for (int b = 0; b < 8; ++b) {
-                const int bi = b >> lr;          // which butterfly
-                const int r = b & (R - 1);       // which input of it
-                const int i = t + bi * eighth;   // butterfly index
-                if (bi < nb) {
-                    u[b] = buf[lds_pad(i + r * stride)];
-                }
-            }
-        }
-        __syncthreads();
-        if (active) {
-            const int tw_step = kTwiddleSize / (p * R);
-#pragma unroll
-            for (int bi = 0; bi < 8; ++bi) {
-                if (bi < nb) {
-                    const int i = t + bi * eighth;
-                    const int k = i & (p - 1);
-                    float2* ub = u + bi * R;
-                    twiddle_and_dft<INV>(ub, lr, k, table, tw_step);
-                    const int j = (i - k) * R + k;
-#pragma unroll
-                    for (int r = 0; r < 8; ++r) {
-                        if (r < R) {
-                            buf[lds_pad(j + r * p)] = ub[r];
-                        }
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        p *= R;
+    for (int pass = 0; pass < full8; ++pass) {
+        lds_pass<INV, 3>(buf, P, t, p, active, table);
+        p <<= 3;
+    }
+    const int rem = log2P - 3 * full8;
+    if (rem == 2) {
+        lds_pass<INV, 2>(buf, P, t, p, active, table);
+    } else if (rem == 1) {
+        lds_pass<INV, 1>(buf, P, t, p, active, table);
     }
 }
 

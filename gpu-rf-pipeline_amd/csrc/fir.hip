@@ -17,6 +17,7 @@
 // fft_ola is new: overlap-save with a 4096-point LDS FFT for long filters.
 
 #include "fft_core.hpp"
+#include "fir_internal.hpp"
 #include "rf_common.hpp"
 #include "twiddle.hpp"
 
@@ -310,54 +311,83 @@ constexpr int kOlaThreads = kOlaP / 8;
 constexpr int kOlaMinTaps = 65;   // at or below this, the direct register kernel is cheaper
 constexpr int kOlaMaxTaps = 2049; // keeps Q >= P / 2
 
-// A radix-8 Stockham pass of the 4096-point transform through LDS (butterfly t, span p).
-template <bool INV>
-__device__ __forceinline__ void ola_lds_pass(float2* buf, int t, int p,
-                                             const float2* __restrict__ table)
+// The 4096-point transforms run as three radix-16 Stockham passes (spans 1, 16, 256) with 256
+// lanes holding 16 values each. The first pass reads x straight from HBM, the last pass leaves
+// outputs t + 256 r in the lane's registers, where the spectrum multiply and the first inverse
+// pass happen without touching LDS; each transform makes two LDS round trips. LDS slot of
+// element j: j + j/16, which puts the 16 lanes of each stride-16 write and of each
+// (t/16)*256 + t%16 write on 16 different banks.
+// Twiddles: per-device table [span][k][r] = exp(-2 pi i r k / (16 span)), r < 16, k < span,
+// span 16 at offset 0 and span 256 at offset 256; a lane reads its 16 factors as 8 16-byte
+// loads.
+constexpr int kOlaLanes = kOlaP / 16;  // 256
+
+__device__ __forceinline__ int pad16(int j)
 {
-    float2 u[8];
+    return j + (j >> 4);
+}
+
+constexpr int kOlaSlots = kOlaP + kOlaP / 16;
+
+template <bool INV>
+__device__ __forceinline__ void ola_twiddle_dft16(float2* u, int k, const float2* stage)
+{
+    const float4* tw = reinterpret_cast<const float4*>(stage + 16 * k);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        u[r] = buf[lds_pad(t + r * kOlaThreads)];
+    for (int q = 0; q < 8; ++q) {
+        const float4 w = tw[q];
+        if (q > 0) {
+            u[2 * q] = INV ? cmulconj(u[2 * q], make_float2(w.x, w.y))
+                           : cmul(u[2 * q], make_float2(w.x, w.y));
+        }
+        u[2 * q + 1] = INV ? cmulconj(u[2 * q + 1], make_float2(w.z, w.w))
+                           : cmul(u[2 * q + 1], make_float2(w.z, w.w));
     }
-    __syncthreads();
-    const int k = t & (p - 1);
-    twiddle_and_dft<INV>(u, 3, k, table, kTwiddleSize / (p * 8));
-    const int j = (t - k) * 8 + k;
+    dft16<INV>(u);
+}
+
+// Span-1 pass from registers (inputs t + 256 r) into LDS at 16 t + r.
+template <bool INV>
+__device__ __forceinline__ void ola16_first(float2* buf, int t, float2* u)
+{
+    dft16<INV>(u);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        buf[lds_pad(j + r * p)] = u[r];
+    for (int r = 0; r < 16; ++r) {
+        buf[pad16(16 * t + r)] = u[r];
     }
     __syncthreads();
 }
 
-// The two ends of the 4096-point transform that never touch LDS: the first pass (span 1, no
-// twiddles) takes inputs t + 512 r and writes 8t + r; the last pass (span 512) reads t + 512 r
-// and leaves outputs t + 512 r in the lane's registers.
+// Span-16 pass through LDS.
 template <bool INV>
-__device__ __forceinline__ void ola_first_pass(float2* buf, int t, float2* u)
+__device__ __forceinline__ void ola16_middle(float2* buf, int t, const float2* stages)
 {
-    if (INV) {
-        dft8<true>(u);
-    } else {
-        dft8<false>(u);
-    }
+    float2 u[16];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        buf[lds_pad(8 * t + r)] = u[r];
+    for (int r = 0; r < 16; ++r) {
+        u[r] = buf[pad16(t + r * kOlaLanes)];
+    }
+    __syncthreads();
+    const int k = t & 15;
+    ola_twiddle_dft16<INV>(u, k, stages);
+    const int j = (t - k) * 16 + k;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        buf[pad16(j + 16 * r)] = u[r];
     }
     __syncthreads();
 }
 
+// Span-256 pass from LDS; outputs t + 256 r stay in registers.
 template <bool INV>
-__device__ __forceinline__ void ola_last_pass(const float2* buf, int t, float2* u,
-                                              const float2* __restrict__ table)
+__device__ __forceinline__ void ola16_last(const float2* buf, int t, float2* u,
+                                           const float2* stages)
 {
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        u[r] = buf[lds_pad(t + r * kOlaThreads)];
+    for (int r = 0; r < 16; ++r) {
+        u[r] = buf[pad16(t + r * kOlaLanes)];
     }
-    twiddle_and_dft<INV>(u, 3, t, table, 1);
+    ola_twiddle_dft16<INV>(u, t, stages + 256);
 }
 
 __global__ __launch_bounds__(kOlaThreads) void fir_ola_taps_kernel(
@@ -377,66 +407,96 @@ __global__ __launch_bounds__(kOlaThreads) void fir_ola_taps_kernel(
     }
 }
 
-__device__ __forceinline__ void fir_ola_block(const float2* __restrict__ x,
-                                              const float2* __restrict__ spec,
-                                              float2* __restrict__ y, int n, int num_taps,
-                                              long long block, float2* buf,
-                                              const float2* __restrict__ table)
+// One segment: seg[i] = x[in0 + i], i < 4096 (zero outside the input), filtered by the
+// spectrum, outputs y[out0 + i - (T-1)] for T-1 <= i < 4096.
+__device__ __forceinline__ void ola_segment(const float2* __restrict__ x,
+                                            const float2* __restrict__ spec,
+                                            float2* __restrict__ y, int n, int halo,
+                                            long long out0, float2* buf,
+                                            const float2* __restrict__ stages)
 {
-    const int halo = num_taps - 1;
-    const int q = kOlaP - halo;
-    const long long out0 = block * q;
-    const long long in0 = out0 - halo;
     const int t = threadIdx.x;
-    float2 u[8];
+    const long long in0 = out0 - halo;
+    float2 u[16];
+    if (in0 >= 0 && in0 + kOlaP <= n) {
+        const float2* src = x + in0 + t;
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        u[r] = load_sample(x, in0 + t + r * kOlaThreads, n);
+        for (int r = 0; r < 16; ++r) {
+            u[r] = src[r * kOlaLanes];
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            u[r] = load_sample(x, in0 + t + r * kOlaLanes, n);
+        }
     }
-    ola_first_pass<false>(buf, t, u);
-    ola_lds_pass<false>(buf, t, 8, table);
-    ola_lds_pass<false>(buf, t, 64, table);
-    ola_last_pass<false>(buf, t, u, table);
+    ola16_first<false>(buf, t, u);
+    ola16_middle<false>(buf, t, stages);
+    ola16_last<false>(buf, t, u, stages);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        u[r] = cmul(u[r], spec[t + r * kOlaThreads]);
+    for (int r = 0; r < 16; ++r) {
+        u[r] = cmul(u[r], spec[t + r * kOlaLanes]);
     }
     __syncthreads();  // every lane has read the forward result before it is overwritten
-    ola_first_pass<true>(buf, t, u);
-    ola_lds_pass<true>(buf, t, 8, table);
-    ola_lds_pass<true>(buf, t, 64, table);
-    ola_last_pass<true>(buf, t, u, table);
+    ola16_first<true>(buf, t, u);
+    ola16_middle<true>(buf, t, stages);
+    ola16_last<true>(buf, t, u, stages);
+    if (out0 + kOlaP - halo <= n) {
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        const int pos = t + r * kOlaThreads;
-        const long long o = out0 + pos - halo;
-        if (pos >= halo && o < n) {
-            y[o] = u[r];
+        for (int r = 0; r < 16; ++r) {
+            const int pos = t + r * kOlaLanes;
+            if (pos >= halo) {
+                y[out0 + pos - halo] = u[r];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int pos = t + r * kOlaLanes;
+            const long long o = out0 + pos - halo;
+            if (pos >= halo && o < n) {
+                y[o] = u[r];
+            }
         }
     }
 }
 
-__global__ __launch_bounds__(kOlaThreads) void fir_ola_kernel(const float2* __restrict__ x,
-                                                              const float2* __restrict__ spec,
-                                                              float2* __restrict__ y, int n,
-                                                              int num_taps,
-                                                              const float2* __restrict__ table)
+// Segment of a workgroup, XCD-major: workgroups are dispatched round-robin over the 8 XCDs, so
+// workgroup b runs on XCD b % 8; giving XCD x the consecutive segments x*per .. x*per + per - 1
+// in order makes each segment's (T-1)-sample halo, read by the previous segment, an L2 hit.
+__device__ __forceinline__ long long xcd_major(long long b, long long per)
 {
-    __shared__ float2 buf[lds_slots(kOlaP)];
-    fir_ola_block(x, spec, y, n, num_taps, blockIdx.x, buf, table);
+    return (b & 7) * per + (b >> 3);
 }
 
-__global__ __launch_bounds__(kOlaThreads) void fir_ola_batched_kernel(
-    FirBatch batch, const float2* __restrict__ spec, int num_taps,
-    const float2* __restrict__ table)
+__global__ __launch_bounds__(kOlaLanes) void fir_ola_kernel(const float2* __restrict__ x,
+                                                           const float2* __restrict__ spec,
+                                                           float2* __restrict__ y, int n,
+                                                           int num_taps, long long segments,
+                                                           long long per_xcd,
+                                                           const float2* __restrict__ stages)
 {
-    __shared__ float2 buf[lds_slots(kOlaP)];
+    __shared__ float2 buf[kOlaSlots];
+    const long long seg = xcd_major(blockIdx.x, per_xcd);
+    if (seg >= segments) {
+        return;  // the whole workgroup: padding of the grid to a multiple of 8
+    }
+    const int halo = num_taps - 1;
+    ola_segment(x, spec, y, n, halo, seg * (kOlaP - halo), buf, stages);
+}
+
+__global__ __launch_bounds__(kOlaLanes) void fir_ola_batched_kernel(
+    FirBatch batch, const float2* __restrict__ spec, int num_taps,
+    const float2* __restrict__ stages)
+{
+    __shared__ float2 buf[kOlaSlots];
     const int c = blockIdx.y;
-    const long long q = kOlaP - (num_taps - 1);
-    if (static_cast<long long>(blockIdx.x) * q >= batch.n[c]) {
+    const int halo = num_taps - 1;
+    const long long out0 = static_cast<long long>(blockIdx.x) * (kOlaP - halo);
+    if (out0 >= batch.n[c]) {
         return;
     }
-    fir_ola_block(batch.x[c], spec, batch.y[c], batch.n[c], num_taps, blockIdx.x, buf, table);
+    ola_segment(batch.x[c], spec, batch.y[c], batch.n[c], halo, out0, buf, stages);
 }
 
 // --- host side ------------------------------------------------------------------------------
@@ -452,6 +512,8 @@ const ImplInfo kFirImpls[RF_FIR_IMPL_COUNT] = {
     {"tiled_const", "tiled, with the taps read through the scalar (constant) cache"},
     {"register", "4, 8 or 16 outputs per lane from a register window over a padded LDS tile"},
     {"fft_ola", "overlap-save with a 4096-point LDS FFT (register kernel for <= 65 taps)"},
+    {"mfma_f16x3", "Toeplitz GEMM on FP16 matrix cores, hi/lo split, FP32 accumulate"},
+    {"mfma_bf16x3", "Toeplitz GEMM on BF16 matrix cores, hi/lo split, FP32 accumulate"},
 };
 
 void check_fir_args(int impl, const void* x, const float* taps, const void* y, int n,
@@ -531,18 +593,10 @@ int register_outputs(int num_taps, int block, long long longest, std::size_t* by
     return chosen;
 }
 
-// Raises a kernel's dynamic-LDS limit above 64 KiB where the runtime wants that; a runtime
-// that does not need it may refuse the call, and the launch itself then reports any problem.
 template <class Kernel>
 void allow_lds(Kernel kernel, std::size_t bytes)
 {
-    if (bytes > 64 * 1024) {
-        if (hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                static_cast<int>(bytes)) != hipSuccess) {
-            (void)hipGetLastError();
-        }
-    }
+    allow_large_lds(reinterpret_cast<const void*>(kernel), bytes);
 }
 
 void launch_register(const float2* x, const float* taps, float2* y, int n, int num_taps,
@@ -569,24 +623,12 @@ bool ola_applies(int num_taps)
     return num_taps > kOlaMinTaps && num_taps <= kOlaMaxTaps;
 }
 
-// Stream-ordered workspace for the tap spectrum of fft_ola: allocated and released on the
-// caller's stream, so the entry point stays enqueue-only.
-struct StreamScratch {
-    void* ptr = nullptr;
-    hipStream_t stream = nullptr;
-    StreamScratch(std::size_t bytes, hipStream_t s) : stream(s)
-    {
-        RF_HIP_CHECK(hipMallocAsync(&ptr, bytes, stream));
-    }
-    ~StreamScratch()
-    {
-        if (ptr != nullptr) {
-            (void)hipFreeAsync(ptr, stream);
-        }
-    }
-    StreamScratch(const StreamScratch&) = delete;
-    StreamScratch& operator=(const StreamScratch&) = delete;
-};
+// The matrix-core variants cover the same tap range as overlap-save; shorter filters run the
+// direct register kernel (exact for a single tap), longer ones too (their LDS would not fit).
+bool mfma_applies(int num_taps)
+{
+    return num_taps > kOlaMinTaps && num_taps <= kOlaMaxTaps;
+}
 
 void launch_ola_taps(const float* taps, int num_taps, float2* spec, const float2* table,
                      hipStream_t stream)
@@ -635,12 +677,22 @@ void fir_launch(int impl, const float2* x, const float* taps, float2* y, int n, 
         StreamScratch spec(kOlaP * sizeof(float2), stream);
         launch_ola_taps(taps, num_taps, static_cast<float2*>(spec.ptr), table, stream);
         const long long q = kOlaP - (num_taps - 1);
-        const unsigned blocks = static_cast<unsigned>(ceil_div(n, q));
-        fir_ola_kernel<<<blocks, kOlaThreads, 0, stream>>>(
-            x, static_cast<const float2*>(spec.ptr), y, n, num_taps, table);
+        const long long segments = ceil_div(n, q);
+        const long long per_xcd = ceil_div(segments, 8);
+        fir_ola_kernel<<<static_cast<unsigned>(8 * per_xcd), kOlaLanes, 0, stream>>>(
+            x, static_cast<const float2*>(spec.ptr), y, n, num_taps, segments, per_xcd,
+            ola_stage_table());
         check_launch("fir_ola_kernel");
         return;
     }
+    case RF_FIR_MFMA_F16X3:
+    case RF_FIR_MFMA_BF16X3:
+        if (!mfma_applies(num_taps)) {
+            launch_register(x, taps, y, n, num_taps, block, stream);
+            return;
+        }
+        fir_mfma_launch(impl == RF_FIR_MFMA_BF16X3, x, taps, y, n, num_taps, stream);
+        return;
     }
 }
 
@@ -648,6 +700,13 @@ void fir_batched_launch(int impl, const FirBatch& batch, int count, int longest,
                         const float* taps, int num_taps, hipStream_t stream)
 {
     const int block = RF_DEFAULT_FIR_BLOCK;
+    if (impl == RF_FIR_MFMA_F16X3 || impl == RF_FIR_MFMA_BF16X3) {
+        // One launch per channel: the matrix-core kernel's tile shape is per capture.
+        for (int c = 0; c < count; ++c) {
+            fir_launch(impl, batch.x[c], taps, batch.y[c], batch.n[c], num_taps, block, stream);
+        }
+        return;
+    }
     switch (impl) {
     case RF_FIR_NAIVE: {
         const dim3 grid(static_cast<unsigned>(ceil_div(longest, block)), count);
@@ -682,8 +741,8 @@ void fir_batched_launch(int impl, const FirBatch& batch, int count, int longest,
             launch_ola_taps(taps, num_taps, static_cast<float2*>(spec.ptr), table, stream);
             const long long q = kOlaP - (num_taps - 1);
             const dim3 grid(static_cast<unsigned>(ceil_div(longest, q)), count);
-            fir_ola_batched_kernel<<<grid, kOlaThreads, 0, stream>>>(
-                batch, static_cast<const float2*>(spec.ptr), num_taps, table);
+            fir_ola_batched_kernel<<<grid, kOlaLanes, 0, stream>>>(
+                batch, static_cast<const float2*>(spec.ptr), num_taps, ola_stage_table());
             check_launch("fir_ola_batched_kernel");
             return;
         }

@@ -1,0 +1,42 @@
+// Pieces shared by the FIR sources (fir.hip, fir_mfma.hip). Internal to the library.
+#pragma once
+
+#include "rf_common.hpp"
+
+namespace rf {
+
+// Stream-ordered workspace: allocated and released on the caller's stream, so an entry point
+// that needs a small table (a tap spectrum, split taps) stays enqueue-only.
+struct StreamScratch {
+    void* ptr = nullptr;
+    hipStream_t stream = nullptr;
+    StreamScratch(std::size_t bytes, hipStream_t s) : stream(s)
+    {
+        RF_HIP_CHECK(hipMallocAsync(&ptr, bytes, stream));
+    }
+    ~StreamScratch()
+    {
+        if (ptr != nullptr) {
+            (void)hipFreeAsync(ptr, stream);
+        }
+    }
+    StreamScratch(const StreamScratch&) = delete;
+    StreamScratch& operator=(const StreamScratch&) = delete;
+};
+
+// Raises a kernel's dynamic-LDS limit above 64 KiB where the runtime wants that; a runtime
+// that does not need it may refuse the call, and the launch itself then reports any problem.
+inline void allow_large_lds(const void* kernel, std::size_t bytes)
+{
+    if (bytes > 64 * 1024
+        && hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               static_cast<int>(bytes)) != hipSuccess) {
+        (void)hipGetLastError();
+    }
+}
+
+// Split-precision Toeplitz-GEMM FIR on the matrix cores (fir_mfma.hip).
+void fir_mfma_launch(bool bf16, const float2* x, const float* taps, float2* y, int n,
+                     int num_taps, hipStream_t stream);
+
+}  // namespace rf

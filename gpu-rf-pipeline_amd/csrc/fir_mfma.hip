@@ -1,0 +1,270 @@
+// Split-precision FIR on the matrix cores: a Toeplitz GEMM on v_mfma_f32_32x32x16_{f16,bf16}
+// with a hi/lo split of both operands and FP32 accumulation.
+//
+// GEMM form of y[n] = sum_k h[k] x[n - k]: a wave owns 1024 consecutive outputs
+// y[n_w + 32 m + c] (m, c < 32). With kk = k + c' shifted so that every column shares one
+// A matrix,
+//   Y[m][c] = sum_kk A[m][kk] B[kk][c],  A[m][kk] = x[n_w + 32 m + 31 - kk],
+//                                        B[kk][c] = h[kk - 31 + c]  (0 outside 0..T-1),
+// kk < Kp = ceil((T + 31) / 16) * 16, i.e. (T + 31) / T ~ 3 % more products than the direct
+// sum. Each operand is split v = hi + lo in the 16-bit type and three products are
+// accumulated (hi*hi + hi*lo + lo*hi); the dropped lo*lo term is 2^-22 of a product for
+// FP16 and 2^-16 for BF16.
+//
+// FP16 range: the taps are scaled by a power of two so that max|h| is in [2^13, 2^14), and each
+// workgroup scales its samples the same way from its own max|x|; the FP32 result is scaled
+// back exactly. BF16 has FP32's exponent range and needs no scaling.
+//
+// LDS: the sample tile as four 16-bit arrays (re/im x hi/lo), so a lane's eight consecutive
+// samples are one 16-byte read; the taps reversed and stored as eight copies shifted by one
+// element each (hi and lo), so a lane's eight taps, whose start is not a multiple of eight, are
+// still one aligned 16-byte read from the copy with the matching shift.
+
+#include "fir_internal.hpp"
+
+#include <string>
+
+namespace rf {
+namespace {
+
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kMfmaThreads = 256;                // 4 waves, 1024 outputs each
+constexpr int kMfmaOutputs = 4 * 1024;           // per workgroup
+
+__host__ __device__ constexpr int mfma_kp(int num_taps)
+{
+    return ((num_taps + 31 + 15) / 16) * 16;
+}
+
+// Length of one shifted tap copy: indices up to T + 78 are read.
+__host__ __device__ constexpr int mfma_copy_len(int num_taps)
+{
+    return ((num_taps + 80 + 7) / 8) * 8;
+}
+
+template <bool BF16>
+struct Split;
+
+template <>
+struct Split<false> {
+    typedef _Float16 T;
+    typedef f16x8 V;
+    __device__ static void split(float v, T& hi, T& lo)
+    {
+        hi = static_cast<_Float16>(v);
+        lo = static_cast<_Float16>(v - static_cast<float>(hi));
+    }
+    __device__ static f32x16 mma(V a, V b, f32x16 c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+};
+
+template <>
+struct Split<true> {
+    typedef __bf16 T;
+    typedef bf16x8 V;
+    __device__ static void split(float v, T& hi, T& lo)
+    {
+        hi = static_cast<__bf16>(v);
+        lo = static_cast<__bf16>(v - static_cast<float>(hi));
+    }
+    __device__ static f32x16 mma(V a, V b, f32x16 c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+// Power-of-two exponent e with max * 2^e in [2^13, 2^14) (0 for max == 0).
+__device__ __forceinline__ int fp16_scale_exponent(float max_abs)
+{
+    if (!(max_abs > 0.0f)) {
+        return 0;
+    }
+    int e = 0;
+    (void)frexpf(max_abs, &e);  // max = f 2^e, f in [0.5, 1)
+    int s = 14 - e;
+    return s > 100 ? 100 : (s < -100 ? -100 : s);
+}
+
+__device__ float block_max(float v, float* scratch)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        v = fmaxf(v, __shfl_xor(v, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        scratch[threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    float m = scratch[0];
+    for (int w = 1; w < static_cast<int>(blockDim.x >> 6); ++w) {
+        m = fmaxf(m, scratch[w]);
+    }
+    __syncthreads();
+    return m;
+}
+
+// Workspace written by the prep kernel: hi copies, lo copies (8 x copy_len each), then one
+// float with 2^-tap_exponent.
+template <bool BF16>
+__global__ __launch_bounds__(kMfmaThreads) void fir_mfma_taps_kernel(
+    const float* __restrict__ taps, int num_taps, typename Split<BF16>::T* ws, float* descale)
+{
+    __shared__ float scratch[kMfmaThreads / 64];
+    float m = 0.0f;
+    for (int i = threadIdx.x; i < num_taps; i += blockDim.x) {
+        m = fmaxf(m, fabsf(taps[i]));
+    }
+    m = block_max(m, scratch);
+    const int e = BF16 ? 0 : fp16_scale_exponent(m);
+    const int len = mfma_copy_len(num_taps);
+    typename Split<BF16>::T* hi = ws;
+    typename Split<BF16>::T* lo = ws + 8 * len;
+    for (int idx = threadIdx.x; idx < 8 * len; idx += blockDim.x) {
+        const int r = idx / len;
+        const int v = idx - r * len;
+        const int k = num_taps + 47 - v - r;  // copy[r][v] = h[T + 47 - v - r]
+        const float h = (k >= 0 && k < num_taps) ? ldexpf(taps[k], e) : 0.0f;
+        Split<BF16>::split(h, hi[idx], lo[idx]);
+    }
+    if (threadIdx.x == 0) {
+        *descale = ldexpf(1.0f, -e);
+    }
+}
+
+template <bool BF16>
+__global__ __launch_bounds__(kMfmaThreads) void fir_mfma_kernel(
+    const float2* __restrict__ x, const typename Split<BF16>::T* __restrict__ ws,
+    const float* __restrict__ tap_descale, float2* __restrict__ y, int n, int num_taps)
+{
+    typedef typename Split<BF16>::T H;
+    typedef typename Split<BF16>::V V;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ float scratch[kMfmaThreads / 64];
+    const int kp = mfma_kp(num_taps);
+    const int U = kMfmaOutputs + kp - 32;
+    const int len = mfma_copy_len(num_taps);
+    H* rh = reinterpret_cast<H*>(smem);
+    H* rl = rh + U;
+    H* ih = rl + U;
+    H* il = ih + U;
+    H* th = il + U;           // 8 copies
+    H* tl = th + 8 * len;     // 8 copies
+    const long long n0 = static_cast<long long>(blockIdx.x) * kMfmaOutputs;
+    const long long x0 = n0 - kp + 32;
+
+    // Samples: block max for the FP16 scale, then the split tile.
+    float m = 0.0f;
+    for (int u = threadIdx.x; u < U; u += blockDim.x) {
+        const long long s = x0 + u;
+        if (s >= 0 && s < n) {
+            const float2 v = x[s];
+            m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+        }
+    }
+    const int ex = BF16 ? 0 : fp16_scale_exponent(block_max(m, scratch));
+    for (int u = threadIdx.x; u < U; u += blockDim.x) {
+        const long long s = x0 + u;
+        const float2 v = (s >= 0 && s < n) ? x[s] : make_float2(0.0f, 0.0f);
+        Split<BF16>::split(ldexpf(v.x, ex), rh[u], rl[u]);
+        Split<BF16>::split(ldexpf(v.y, ex), ih[u], il[u]);
+    }
+    for (int idx = threadIdx.x; idx < 16 * len; idx += blockDim.x) {
+        th[idx] = ws[idx];  // th and tl are contiguous, as in the workspace
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int i = lane & 31;
+    const int g = lane >> 5;
+    f32x16 acc_re0 = {};
+    f32x16 acc_im0 = {};
+    f32x16 acc_re1 = {};
+    f32x16 acc_im1 = {};
+    const int steps = kp / 16;
+    for (int s = 0; s < steps; ++s) {
+        const int u0 = 1024 * w + 32 * i - 16 * s - 8 * g + kp - 8;
+        const V arh = *reinterpret_cast<const V*>(rh + u0);
+        const V arl = *reinterpret_cast<const V*>(rl + u0);
+        const V aih = *reinterpret_cast<const V*>(ih + u0);
+        const V ail = *reinterpret_cast<const V*>(il + u0);
+        const int b = num_taps + 71 - 16 * s - 8 * g - i;  // b0 + 48 > 0
+        const int r = b & 7;
+        const int base = r * len + (b - r);
+        const V bh = *reinterpret_cast<const V*>(th + base);
+        const V bl = *reinterpret_cast<const V*>(tl + base);
+        if (s & 1) {
+            acc_re1 = Split<BF16>::mma(arh, bh, acc_re1);
+            acc_im1 = Split<BF16>::mma(aih, bh, acc_im1);
+            acc_re1 = Split<BF16>::mma(arh, bl, acc_re1);
+            acc_im1 = Split<BF16>::mma(aih, bl, acc_im1);
+            acc_re1 = Split<BF16>::mma(arl, bh, acc_re1);
+            acc_im1 = Split<BF16>::mma(ail, bh, acc_im1);
+        } else {
+            acc_re0 = Split<BF16>::mma(arh, bh, acc_re0);
+            acc_im0 = Split<BF16>::mma(aih, bh, acc_im0);
+            acc_re0 = Split<BF16>::mma(arh, bl, acc_re0);
+            acc_im0 = Split<BF16>::mma(aih, bl, acc_im0);
+            acc_re0 = Split<BF16>::mma(arl, bh, acc_re0);
+            acc_im0 = Split<BF16>::mma(ail, bh, acc_im0);
+        }
+    }
+    const float scale = ldexpf(*tap_descale, -ex);
+    const long long nw = n0 + 1024 * w;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int row = (v & 3) + 8 * (v >> 2) + 4 * g;
+        const long long o = nw + 32 * row + i;
+        if (o < n) {
+            y[o] = make_float2((acc_re0[v] + acc_re1[v]) * scale, (acc_im0[v] + acc_im1[v]) * scale);
+        }
+    }
+}
+
+std::size_t mfma_lds_bytes(int num_taps)
+{
+    const std::size_t U = kMfmaOutputs + mfma_kp(num_taps) - 32;
+    return (4 * U + 16 * static_cast<std::size_t>(mfma_copy_len(num_taps))) * 2;
+}
+
+template <bool BF16>
+void launch_mfma(const float2* x, const float* taps, float2* y, int n, int num_taps,
+                 hipStream_t stream)
+{
+    const std::size_t bytes = mfma_lds_bytes(num_taps);
+    if (bytes > static_cast<std::size_t>(max_lds_per_block())) {
+        fail(RF_LIMIT, "fir mfma: " + std::to_string(num_taps) + " taps need "
+                           + std::to_string(bytes) + " bytes of LDS per workgroup");
+    }
+    const std::size_t ws_bytes =
+        16 * static_cast<std::size_t>(mfma_copy_len(num_taps)) * 2 + 16;
+    StreamScratch ws(ws_bytes, stream);
+    typedef typename Split<BF16>::T H;
+    H* tabs = static_cast<H*>(ws.ptr);
+    float* descale = reinterpret_cast<float*>(static_cast<char*>(ws.ptr) + ws_bytes - 16);
+    fir_mfma_taps_kernel<BF16><<<1, kMfmaThreads, 0, stream>>>(taps, num_taps, tabs, descale);
+    check_launch("fir_mfma_taps_kernel");
+    allow_large_lds(reinterpret_cast<const void*>(fir_mfma_kernel<BF16>), bytes);
+    const unsigned grid = static_cast<unsigned>(ceil_div(n, kMfmaOutputs));
+    fir_mfma_kernel<BF16><<<grid, kMfmaThreads, bytes, stream>>>(x, tabs, descale, y, n,
+                                                                  num_taps);
+    check_launch("fir_mfma_kernel");
+}
+
+}  // namespace
+
+void fir_mfma_launch(bool bf16, const float2* x, const float* taps, float2* y, int n,
+                     int num_taps, hipStream_t stream)
+{
+    if (bf16) {
+        launch_mfma<true>(x, taps, y, n, num_taps, stream);
+    } else {
+        launch_mfma<false>(x, taps, y, n, num_taps, stream);
+    }
+}
+
+}  // namespace rf

@@ -56,10 +56,22 @@ __global__ __launch_bounds__(256) void copy_kernel(const unsigned char* __restri
     const bool vec = ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15) == 0;
     size_t done = 0;
     if (vec) {
+        // Four 16-byte loads in flight per lane before their stores.
         const size_t n16 = bytes / 16;
         const float4* s4 = reinterpret_cast<const float4*>(src);
         float4* d4 = reinterpret_cast<float4*>(dst);
-        for (size_t i = first; i < n16; i += stride) {
+        size_t i = first;
+        for (; i + 3 * stride < n16; i += 4 * stride) {
+            const float4 a = s4[i];
+            const float4 b = s4[i + stride];
+            const float4 c = s4[i + 2 * stride];
+            const float4 d = s4[i + 3 * stride];
+            d4[i] = a;
+            d4[i + stride] = b;
+            d4[i + 2 * stride] = c;
+            d4[i + 3 * stride] = d;
+        }
+        for (; i < n16; i += stride) {
             d4[i] = s4[i];
         }
         done = n16 * 16;
@@ -76,7 +88,7 @@ void copy_device(const void* src, void* dst, size_t bytes, hipStream_t stream)
     if (bytes == 0) {
         return;
     }
-    const long long cap = 16LL * cu_count();
+    const long long cap = 8LL * cu_count();
     long long grid = ceil_div(static_cast<long long>(bytes / 16 + 1), 256);
     grid = grid < cap ? grid : cap;
     copy_kernel<<<static_cast<unsigned>(grid), 256, 0, stream>>>(

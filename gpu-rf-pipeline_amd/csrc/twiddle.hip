@@ -49,4 +49,38 @@ const float2* twiddle_table_4096()
     return g_table[device];
 }
 
+namespace {
+std::once_flag g_stage_once[kMaxDevices];
+float2* g_stage[kMaxDevices] = {};
+}  // namespace
+
+const float2* ola_stage_table()
+{
+    int device = 0;
+    RF_HIP_CHECK(hipGetDevice(&device));
+    if (device < 0 || device >= kMaxDevices) {
+        fail(RF_LIMIT, "twiddle table: device index out of range");
+    }
+    std::call_once(g_stage_once[device], [device] {
+        std::vector<float2> host;
+        for (int span : {16, 256}) {
+            for (int k = 0; k < span; ++k) {
+                float2 w[16];
+                // exp(-2 pi i r k / (16 span)): entries r*k of a transform of 16*span points
+                fill_twiddles(w, 16, 16LL * span, k);
+                host.insert(host.end(), w, w + 16);
+            }
+        }
+        float2* dev = nullptr;
+        RF_HIP_CHECK(hipMalloc(&dev, host.size() * sizeof(float2)));
+        RF_HIP_CHECK(hipMemcpy(dev, host.data(), host.size() * sizeof(float2),
+                               hipMemcpyHostToDevice));
+        g_stage[device] = dev;
+    });
+    if (g_stage[device] == nullptr) {
+        fail(RF_HIP_ERROR, "twiddle table: initialisation failed on an earlier call");
+    }
+    return g_stage[device];
+}
+
 }  // namespace rf

@@ -93,7 +93,9 @@ enum {
     RF_FIR_TILED_CONST = 2, /* tiled, taps through the scalar (constant) cache (src/fir.cu:213-254) */
     RF_FIR_REGISTER = 3,    /* R outputs per lane from a register window (src/fir.cu:337-409) */
     RF_FIR_FFT_OLA = 4,     /* overlap-save with the library's LDS FFT; no reference counterpart */
-    RF_FIR_IMPL_COUNT = 5
+    RF_FIR_MFMA_F16X3 = 5,  /* Toeplitz GEMM, FP16 matrix cores, hi/lo split; no counterpart */
+    RF_FIR_MFMA_BF16X3 = 6, /* the same in BF16 (reduced precision, error ~2^-16 relative) */
+    RF_FIR_IMPL_COUNT = 7
 };
 #define RF_DEFAULT_FIR_BLOCK 256 /* kDefaultFirBlock, src/ops.hpp:35 */
 #define RF_MAX_FIR_BATCH 8       /* kMaxFirBatch, src/device_ops.hpp:46 */

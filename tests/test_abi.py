@@ -41,7 +41,7 @@ def test_abi_version_and_status_strings() -> None:
 def test_fir_table_keeps_the_reference_order() -> None:
     # src/fir.cu:725-734, then this build's addition.
     assert [i.name for i in rf.fir_impls()] == ["naive", "tiled", "tiled_const", "register",
-                                               "fft_ola"]
+                                               "fft_ola", "mfma_f16x3", "mfma_bf16x3"]
     assert [i.name for i in rf.xcorr_impls()] == ["naive", "tiled", "fft"]
     assert rf.fir_impl_index("register") == 3
 

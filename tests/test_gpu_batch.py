@@ -19,8 +19,10 @@ GCC_ATOL = 1e-4
 
 
 def delayed_pair(seed: int, na: int, nb: int, delay: int, noise: float):
+    # b[n] = a's source delayed by `delay`: b[n] = src[n + D - delay], a[n] = src[n + D].
     rng = np.random.default_rng(seed)
-    src = rng.standard_normal(max(na, nb) + abs(delay) + 8) + 1j * rng.standard_normal(max(na, nb) + abs(delay) + 8)
+    total = max(na, nb) + 2 * abs(delay) + 8
+    src = rng.standard_normal(total) + 1j * rng.standard_normal(total)
     a = src[abs(delay): abs(delay) + na]
     b = src[abs(delay) - delay: abs(delay) - delay + nb]
     a = a + noise * (rng.standard_normal(na) + 1j * rng.standard_normal(na))

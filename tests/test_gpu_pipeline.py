@@ -20,6 +20,9 @@ rf = load_rfhip()
 FIR_IMPLS = [i.name for i in rf.fir_impls()]
 XCORR_IMPLS = [i.name for i in rf.xcorr_impls()]
 DIRECT_FIR = ["naive", "tiled", "tiled_const", "register"]
+# Reduced-precision forms and their stated per-output error bound, as a multiple of
+# sum_k |h[k]| |x[n-k]| (BF16 hi/lo keeps 16 significant bits of each operand and drops lo*lo).
+REDUCED_BOUND = {"mfma_bf16x3": 2.0 ** -15}
 
 # Tolerances of the reference's tests/test_pipeline.py.
 SPECTRUM_RTOL, SPECTRUM_ATOL = 1e-5, 2e-3      # :62-66
@@ -40,6 +43,20 @@ def kaiser_lowpass(num_taps: int, cutoff: float, beta: float) -> np.ndarray:
 def noise(seed: int, n: int) -> np.ndarray:
     rng = np.random.default_rng(seed)
     return (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+
+
+def assert_fir_close(impl: str, got: np.ndarray, x: np.ndarray, h: np.ndarray,
+                     want: np.ndarray | None = None) -> None:
+    """FP32-equivalent variants: the reference's rtol = atol = 1e-5. Reduced-precision forms:
+    |error[n]| <= bound * sum_k |h[k]| |x[n-k]| + 1e-7, checked output by output."""
+    want = oracle.fir(x, h) if want is None else want
+    assert got.shape == want.shape
+    if impl in REDUCED_BOUND:
+        scale = np.abs(oracle.fir(np.abs(x.astype(np.complex128)), np.abs(h.astype(np.float64))))
+        err = np.abs(got.astype(np.complex128) - want)
+        assert np.all(err <= REDUCED_BOUND[impl] * scale + 1e-7), float(np.max(err - REDUCED_BOUND[impl] * scale))
+    else:
+        np.testing.assert_allclose(got, want, rtol=FIR_RTOL, atol=FIR_ATOL)
 
 
 # --- spectrum --------------------------------------------------------------------------------
@@ -114,18 +131,14 @@ def test_magnitude_unaligned_and_odd_lengths() -> None:
 ])
 def test_fir_matches_oracle(impl: str, name: str, taps_name: str) -> None:
     x, h = cf32(name), taps(taps_name)
-    got = rf.fir(x, h, impl)
-    want = oracle.fir(x, h)
-    assert got.shape == want.shape
-    np.testing.assert_allclose(got, want, rtol=FIR_RTOL, atol=FIR_ATOL)
+    assert_fir_close(impl, rf.fir(x, h, impl), x, h)
 
 
 @pytest.mark.parametrize("impl", FIR_IMPLS)
 @pytest.mark.parametrize("block", [32, 96, 1024])
 def test_fir_block_size_does_not_change_the_result(impl: str, block: int) -> None:
     x, h = cf32("tone_odd"), taps("lowpass_long")
-    np.testing.assert_allclose(rf.fir(x, h, impl, block), oracle.fir(x, h), rtol=FIR_RTOL,
-                               atol=FIR_ATOL)
+    assert_fir_close(impl, rf.fir(x, h, impl, block), x, h)
 
 
 @pytest.mark.parametrize("impl", FIR_IMPLS)
@@ -185,8 +198,7 @@ def test_fir_tiled_lds_limit_is_reported() -> None:
 def test_fir_sigmf_ci16_input(impl: str) -> None:
     x = read_ci16_sigmf("tone_ci16")
     np.testing.assert_array_equal(rf.fir(x, taps("identity"), impl), x)
-    np.testing.assert_allclose(rf.fir(x, taps("lowpass"), impl), oracle.fir(x, taps("lowpass")),
-                               rtol=FIR_RTOL, atol=FIR_ATOL)
+    assert_fir_close(impl, rf.fir(x, taps("lowpass"), impl), x, taps("lowpass"))
 
 
 @pytest.mark.parametrize("impl", FIR_IMPLS)
@@ -194,7 +206,26 @@ def test_fir_long_random_filter(impl: str) -> None:
     # 2049 taps: the longest overlap-save filter; random signs stress cancellation.
     h = (np.random.default_rng(5).standard_normal(2049) / 40).astype(np.float32)
     x = noise(6, 20000)
-    np.testing.assert_allclose(rf.fir(x, h, impl), oracle.fir(x, h), rtol=FIR_RTOL, atol=FIR_ATOL)
+    assert_fir_close(impl, rf.fir(x, h, impl), x, h)
+
+
+@pytest.mark.parametrize("impl", ["mfma_f16x3", "mfma_bf16x3"])
+def test_fir_wide_dynamic_range(impl: str) -> None:
+    # Samples of 1e-6 in the first half and 1e4 in the second, taps from 1e-9 to 1e-1: the FP16
+    # form scales taps and each workgroup's samples by powers of two. Outputs whose workgroup
+    # saw only one of the two levels keep the per-output bound.
+    rng = np.random.default_rng(9)
+    n = 40000
+    x = noise(10, n) * np.float32(1e-6)
+    x[n // 2:] *= np.float32(1e10)
+    h = (rng.standard_normal(300) * np.logspace(-9, -1, 300)).astype(np.float32)
+    got = rf.fir(x, h, impl).astype(np.complex128)
+    want = oracle.fir(x, h)
+    scale = np.abs(oracle.fir(np.abs(x.astype(np.complex128)), np.abs(h.astype(np.float64))))
+    bound = REDUCED_BOUND.get(impl, 2.0 ** -18)
+    keep = np.r_[0:n // 2 - 8192, n // 2 + 8192:n]
+    err = np.abs(got - want)[keep]
+    assert np.all(err <= bound * scale[keep] + 1e-30), float(np.max(err / scale[keep]))
 
 
 def test_fir_config_plain_2e20_x_63() -> None:
@@ -208,10 +239,10 @@ def test_fir_config_plain_2e20_x_63() -> None:
     h = (h / h.sum()).astype(np.float32)
     want = oracle.fir(x, h)
     for impl in FIR_IMPLS:
-        np.testing.assert_allclose(rf.fir(x, h, impl), want, rtol=FIR_RTOL, atol=FIR_ATOL)
+        assert_fir_close(impl, rf.fir(x, h, impl), x, h, want)
 
 
-@pytest.mark.parametrize("impl", ["register", "fft_ola"])
+@pytest.mark.parametrize("impl", ["register", "fft_ola", "mfma_f16x3", "mfma_bf16x3"])
 def test_fir_full_size_long_filter_sampled_outputs(impl: str) -> None:
     # 2^24 samples x 1025 Kaiser taps: 3000 outputs recomputed in float64 on the host.
     n = 1 << 24
@@ -224,7 +255,11 @@ def test_fir_full_size_long_filter_sampled_outputs(impl: str) -> None:
     src = idx[:, None] - k[None, :]
     xv = np.where(src >= 0, x[np.clip(src, 0, None)].astype(np.complex128), 0)
     want = xv @ h.astype(np.float64)
-    np.testing.assert_allclose(y[idx], want, rtol=FIR_RTOL, atol=FIR_ATOL)
+    if impl in REDUCED_BOUND:
+        scale = np.abs(xv) @ np.abs(h.astype(np.float64))
+        assert np.all(np.abs(y[idx] - want) <= REDUCED_BOUND[impl] * scale + 1e-7)
+    else:
+        np.testing.assert_allclose(y[idx], want, rtol=FIR_RTOL, atol=FIR_ATOL)
 
 
 # --- cross-correlation ---------------------------------------------------------------------
