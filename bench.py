@@ -264,6 +264,7 @@ def bench_gcc_batch(rf, args) -> dict:
     h = rf.GccBatch(na, nb)
     h.bind([x.ptr for x in d_a], [x.ptr for x in d_b])
     res = rf.DeviceBuffer(16 * count, np.uint8)
+    tuned = h.tune(res.ptr, stream.handle)
     ms_eager = time_steps(rf, lambda i: h.run(res.ptr, stream.handle), args.steps, args.warmup,
                           stream.handle) / args.steps
     ms_graph = time_steps(rf, lambda i: h.run(res.ptr, stream.handle, graph=True), args.steps,
@@ -310,7 +311,8 @@ def bench_gcc_batch(rf, args) -> dict:
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(bytes_step / (best * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": None},
-        "batched": {"eager_ms": round(ms_eager, 4), "graph_ms": round(ms_graph, 4)},
+        "batched": {"eager_ms": round(ms_eager, 4), "graph_ms": round(ms_graph, 4),
+                    "tuned_workgroups_per_cu": tuned},
         "plain_per_pair": {"ms": round(ms_plain, 4), "pairs_per_s": round(count / (ms_plain * 1e-3), 1),
                            "speedup_of_batched": round(ms_plain / best, 2),
                            "same_peak_index": same},

@@ -124,6 +124,7 @@ _proto("rf_gcc_batch_bind", _st, _vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
        ctypes.POINTER(_vp))
 _proto("rf_gcc_batch_run", _st, _vp, _vp, _vp)
 _proto("rf_gcc_batch_launch_graph", _st, _vp, _vp, _vp)
+_proto("rf_gcc_batch_tune", _st, _vp, _vp, _vp, ctypes.POINTER(_int))
 
 
 def lib() -> ctypes.CDLL:
@@ -639,6 +640,12 @@ class GccBatch:
     def run(self, results: int, stream=None, graph: bool = False) -> None:
         fn = _lib.rf_gcc_batch_launch_graph if graph else _lib.rf_gcc_batch_run
         check(fn(self.handle, results, _sp(stream)))
+
+    def tune(self, results: int, stream=None) -> int:
+        """Picks the persistent grid size (workgroups per CU) by timing; returns it."""
+        best = _int(0)
+        check(_lib.rf_gcc_batch_tune(self.handle, results, _sp(stream), ctypes.byref(best)))
+        return best.value
 
     def bytes(self) -> int:
         n = _size(0)

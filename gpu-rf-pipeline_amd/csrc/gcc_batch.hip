@@ -526,6 +526,55 @@ rf_status rf_gcc_batch_launch_graph(rf_gcc_batch* h, rf_peak_result* results, rf
     });
 }
 
+rf_status rf_gcc_batch_tune(rf_gcc_batch* h, rf_peak_result* results, rf_stream stream,
+                            int* workgroups_per_cu)
+{
+    return guarded([&] {
+        if (h == nullptr || results == nullptr) rf::fail(RF_INVALID_ARG, "gcc batch: NULL");
+        if (h->host[0].a == nullptr) rf::fail(RF_INVALID_ARG, "gcc batch: tune before bind");
+        hipStream_t s = rf::as_stream(stream);
+        hipEvent_t start = nullptr;
+        hipEvent_t stop = nullptr;
+        RF_HIP_CHECK(hipEventCreate(&start));
+        RF_HIP_CHECK(hipEventCreate(&stop));
+        const int candidates[] = {1, 2, 3, 4, 6, 8};
+        float best_ms = 0.0f;
+        int best = h->grid_mult;
+        try {
+            for (int c : candidates) {
+                h->grid_mult = c;
+                rf::batch_run(*h, results, s);  // warm-up
+                RF_HIP_CHECK(hipEventRecord(start, s));
+                for (int rep = 0; rep < 3; ++rep) {
+                    rf::batch_run(*h, results, s);
+                }
+                RF_HIP_CHECK(hipEventRecord(stop, s));
+                RF_HIP_CHECK(hipEventSynchronize(stop));
+                float ms = 0.0f;
+                RF_HIP_CHECK(hipEventElapsedTime(&ms, start, stop));
+                if (best_ms == 0.0f || ms < best_ms) {
+                    best_ms = ms;
+                    best = c;
+                }
+            }
+        } catch (...) {
+            (void)hipEventDestroy(start);
+            (void)hipEventDestroy(stop);
+            throw;
+        }
+        (void)hipEventDestroy(start);
+        (void)hipEventDestroy(stop);
+        h->grid_mult = best;
+        if (h->graph != nullptr) {  // the captured graph used the old grid
+            RF_HIP_CHECK(hipGraphExecDestroy(h->graph));
+            h->graph = nullptr;
+        }
+        if (workgroups_per_cu != nullptr) {
+            *workgroups_per_cu = best;
+        }
+    });
+}
+
 rf_status rf_gcc_batch_bytes(const rf_gcc_batch* h, size_t* bytes)
 {
     return guarded([&] {

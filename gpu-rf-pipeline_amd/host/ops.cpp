@@ -75,18 +75,20 @@ std::vector<cf32> xcorr_fft(const std::vector<cf32>& a, const std::vector<cf32>&
 
 }  // namespace
 
+// One host wrapper per library variant; the table's names, order and summaries come from the
+// library (rf_fir_impl_name / rf_fir_impl_summary), so a new variant needs one line here.
 const std::vector<Impl<FirFn>>& fir_impls()
 {
-    static const std::vector<Impl<FirFn>> impls = {
-        {rf_fir_impl_name(RF_FIR_NAIVE), rf_fir_impl_summary(RF_FIR_NAIVE), fir_host<RF_FIR_NAIVE>},
-        {rf_fir_impl_name(RF_FIR_TILED), rf_fir_impl_summary(RF_FIR_TILED), fir_host<RF_FIR_TILED>},
-        {rf_fir_impl_name(RF_FIR_TILED_CONST), rf_fir_impl_summary(RF_FIR_TILED_CONST),
-         fir_host<RF_FIR_TILED_CONST>},
-        {rf_fir_impl_name(RF_FIR_REGISTER), rf_fir_impl_summary(RF_FIR_REGISTER),
-         fir_host<RF_FIR_REGISTER>},
-        {rf_fir_impl_name(RF_FIR_FFT_OLA), rf_fir_impl_summary(RF_FIR_FFT_OLA),
-         fir_host<RF_FIR_FFT_OLA>},
-    };
+    static const FirFn fns[] = {fir_host<0>, fir_host<1>, fir_host<2>, fir_host<3>,
+                                fir_host<4>, fir_host<5>, fir_host<6>};
+    static_assert(sizeof(fns) / sizeof(fns[0]) == RF_FIR_IMPL_COUNT, "one wrapper per variant");
+    static const std::vector<Impl<FirFn>> impls = [] {
+        std::vector<Impl<FirFn>> v;
+        for (int i = 0; i < rf_fir_impl_count(); ++i) {
+            v.push_back({rf_fir_impl_name(i), rf_fir_impl_summary(i), fns[i]});
+        }
+        return v;
+    }();
     return impls;
 }
 

@@ -229,6 +229,11 @@ rf_status rf_gcc_batch_run(rf_gcc_batch* handle, rf_peak_result* results, rf_str
  * replayed with a single launch. */
 rf_status rf_gcc_batch_launch_graph(rf_gcc_batch* handle, rf_peak_result* results,
                                     rf_stream stream);
+/* Autotuner: times the bound batch with 1, 2, 3, 4, 6 and 8 persistent workgroups per CU
+ * (3 runs each after a warm-up, device events on `stream`, which it synchronises), keeps the
+ * fastest for later runs and reports it. The results buffer is overwritten. */
+rf_status rf_gcc_batch_tune(rf_gcc_batch* handle, rf_peak_result* results, rf_stream stream,
+                            int* workgroups_per_cu);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -129,6 +129,16 @@ def test_fractional_delay_estimate() -> None:
     h.destroy()
 
 
+def test_autotuned_grid_gives_the_same_results(batch) -> None:
+    pairs, h, d_a, d_b, d_r, res, peaks, outs = batch
+    want = res.download().tobytes()
+    best = h.tune(res.ptr)
+    assert best in (1, 2, 3, 4, 6, 8)
+    h.run(res.ptr)
+    rf.synchronize()
+    assert res.download().tobytes() == want
+
+
 def test_short_pairs_are_refused() -> None:
     with pytest.raises(rf.RfError) as e:
         rf.GccBatch([1000, 5000], [1000, 5000])
