@@ -30,6 +30,7 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector, published
 METRIC = "Msamples/s + %HBM roofline (FIR, channelizer); GCC-PHAT pairs/s at fixed error"
 
 
@@ -298,7 +299,11 @@ def bench_gcc_batch(rf, args) -> dict:
         dt = time.perf_counter() - t0
         cpu_pairs_per_s = len(order) / dt
         checked = {"pairs": len(order), "lag_equal_to_oracle": int(agree)}
-    bytes_step = sum(16 * (x + y) for x, y in zip(na, nb))
+    # Algorithmic bytes: both inputs read once (8 B per sample); the per-pair result is 16 B.
+    bytes_step = sum(8 * (x + y) + 16 for x, y in zip(na, nb))
+    # FFT work: three L-point transforms per pair (5 L log2 L each) plus the weighting.
+    lengths = [1 << int(x + y - 2).bit_length() for x, y in zip(na, nb)]
+    flop_step = sum(15 * n * (n.bit_length() - 1) + 12 * n for n in lengths)
     best = min(ms_eager, ms_graph)
     return {
         "metric": METRIC, "value": round(count / (best * 1e-3), 1), "unit": "pairs/s",
@@ -311,6 +316,10 @@ def bench_gcc_batch(rf, args) -> dict:
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(bytes_step / (best * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": None},
+        "fft_valu": {"flop_per_step": int(flop_step),
+                     "achieved_tflops": round(flop_step / (best * 1e-3) / 1e12, 2),
+                     "peak_tflops": FP32_PEAK_TFLOPS,
+                     "frac": round(flop_step / (best * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)},
         "batched": {"eager_ms": round(ms_eager, 4), "graph_ms": round(ms_graph, 4),
                     "tuned_workgroups_per_cu": tuned},
         "plain_per_pair": {"ms": round(ms_plain, 4), "pairs_per_s": round(count / (ms_plain * 1e-3), 1),

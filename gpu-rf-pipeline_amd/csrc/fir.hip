@@ -16,6 +16,7 @@
 //     R samples (conflict-free 16-byte LDS reads).
 // fft_ola is new: overlap-save with a 4096-point LDS FFT for long filters.
 
+#include "fft4096.hpp"
 #include "fft_core.hpp"
 #include "fir_internal.hpp"
 #include "rf_common.hpp"
@@ -302,93 +303,15 @@ __global__ void fir_register_batched_kernel(FirBatch batch, const float* __restr
 // Overlap-save: with P = 4096 and Q = P - (T - 1) outputs per block,
 //   seg[i] = x[bQ - (T-1) + i],  i < P (zero outside the input)
 //   y[bQ + j] = IFFT(FFT(seg) * FFT(h)) [T - 1 + j] / P,  j < Q.
-// 512 lanes per block, one radix-8 butterfly per lane per pass (4096 = 8^4). The first forward
-// pass reads x straight from HBM and the last inverse pass writes y straight back; the
-// spectrum multiply sits between the last forward and the first inverse pass, in registers.
-constexpr int kOlaLog2 = 12;
-constexpr int kOlaP = 1 << kOlaLog2;
+// 256 lanes per segment with the register-resident radix-16 transform of fft4096.hpp: the
+// forward transform reads x straight from HBM, the spectrum multiply sits between the forward
+// and the inverse transform in registers, and the inverse transform writes y straight back.
+// The tap spectrum is made once per call by a 512-lane generic LDS transform.
+constexpr int kOlaLog2 = k4096Log2;
+constexpr int kOlaP = k4096;
 constexpr int kOlaThreads = kOlaP / 8;
 constexpr int kOlaMinTaps = 65;   // at or below this, the direct register kernel is cheaper
 constexpr int kOlaMaxTaps = 2049; // keeps Q >= P / 2
-
-// The 4096-point transforms run as three radix-16 Stockham passes (spans 1, 16, 256) with 256
-// lanes holding 16 values each. The first pass reads x straight from HBM, the last pass leaves
-// outputs t + 256 r in the lane's registers, where the spectrum multiply and the first inverse
-// pass happen without touching LDS; each transform makes two LDS round trips. LDS slot of
-// element j: j + j/16, which puts the 16 lanes of each stride-16 write and of each
-// (t/16)*256 + t%16 write on 16 different banks.
-// Twiddles: per-device table [span][k][r] = exp(-2 pi i r k / (16 span)), r < 16, k < span,
-// span 16 at offset 0 and span 256 at offset 256; a lane reads its 16 factors as 8 16-byte
-// loads.
-constexpr int kOlaLanes = kOlaP / 16;  // 256
-
-__device__ __forceinline__ int pad16(int j)
-{
-    return j + (j >> 4);
-}
-
-constexpr int kOlaSlots = kOlaP + kOlaP / 16;
-
-template <bool INV>
-__device__ __forceinline__ void ola_twiddle_dft16(float2* u, int k, const float2* stage)
-{
-    const float4* tw = reinterpret_cast<const float4*>(stage + 16 * k);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const float4 w = tw[q];
-        if (q > 0) {
-            u[2 * q] = INV ? cmulconj(u[2 * q], make_float2(w.x, w.y))
-                           : cmul(u[2 * q], make_float2(w.x, w.y));
-        }
-        u[2 * q + 1] = INV ? cmulconj(u[2 * q + 1], make_float2(w.z, w.w))
-                           : cmul(u[2 * q + 1], make_float2(w.z, w.w));
-    }
-    dft16<INV>(u);
-}
-
-// Span-1 pass from registers (inputs t + 256 r) into LDS at 16 t + r.
-template <bool INV>
-__device__ __forceinline__ void ola16_first(float2* buf, int t, float2* u)
-{
-    dft16<INV>(u);
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        buf[pad16(16 * t + r)] = u[r];
-    }
-    __syncthreads();
-}
-
-// Span-16 pass through LDS.
-template <bool INV>
-__device__ __forceinline__ void ola16_middle(float2* buf, int t, const float2* stages)
-{
-    float2 u[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        u[r] = buf[pad16(t + r * kOlaLanes)];
-    }
-    __syncthreads();
-    const int k = t & 15;
-    ola_twiddle_dft16<INV>(u, k, stages);
-    const int j = (t - k) * 16 + k;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        buf[pad16(j + 16 * r)] = u[r];
-    }
-    __syncthreads();
-}
-
-// Span-256 pass from LDS; outputs t + 256 r stay in registers.
-template <bool INV>
-__device__ __forceinline__ void ola16_last(const float2* buf, int t, float2* u,
-                                           const float2* stages)
-{
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        u[r] = buf[pad16(t + r * kOlaLanes)];
-    }
-    ola_twiddle_dft16<INV>(u, t, stages + 256);
-}
 
 __global__ __launch_bounds__(kOlaThreads) void fir_ola_taps_kernel(
     const float* __restrict__ taps, int num_taps, float2* __restrict__ spec,
@@ -422,29 +345,25 @@ __device__ __forceinline__ void ola_segment(const float2* __restrict__ x,
         const float2* src = x + in0 + t;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            u[r] = src[r * kOlaLanes];
+            u[r] = src[r * k4096Lanes];
         }
     } else {
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            u[r] = load_sample(x, in0 + t + r * kOlaLanes, n);
+            u[r] = load_sample(x, in0 + t + r * k4096Lanes, n);
         }
     }
-    ola16_first<false>(buf, t, u);
-    ola16_middle<false>(buf, t, stages);
-    ola16_last<false>(buf, t, u, stages);
+    fft4096<false>(buf, t, u, stages);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        u[r] = cmul(u[r], spec[t + r * kOlaLanes]);
+        u[r] = cmul(u[r], spec[t + r * k4096Lanes]);
     }
     __syncthreads();  // every lane has read the forward result before it is overwritten
-    ola16_first<true>(buf, t, u);
-    ola16_middle<true>(buf, t, stages);
-    ola16_last<true>(buf, t, u, stages);
+    fft4096<true>(buf, t, u, stages);
     if (out0 + kOlaP - halo <= n) {
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const int pos = t + r * kOlaLanes;
+            const int pos = t + r * k4096Lanes;
             if (pos >= halo) {
                 y[out0 + pos - halo] = u[r];
             }
@@ -452,7 +371,7 @@ __device__ __forceinline__ void ola_segment(const float2* __restrict__ x,
     } else {
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const int pos = t + r * kOlaLanes;
+            const int pos = t + r * k4096Lanes;
             const long long o = out0 + pos - halo;
             if (pos >= halo && o < n) {
                 y[o] = u[r];
@@ -469,14 +388,14 @@ __device__ __forceinline__ long long xcd_major(long long b, long long per)
     return (b & 7) * per + (b >> 3);
 }
 
-__global__ __launch_bounds__(kOlaLanes) void fir_ola_kernel(const float2* __restrict__ x,
+__global__ __launch_bounds__(k4096Lanes) void fir_ola_kernel(const float2* __restrict__ x,
                                                            const float2* __restrict__ spec,
                                                            float2* __restrict__ y, int n,
                                                            int num_taps, long long segments,
                                                            long long per_xcd,
                                                            const float2* __restrict__ stages)
 {
-    __shared__ float2 buf[kOlaSlots];
+    __shared__ float2 buf[k4096Slots];
     const long long seg = xcd_major(blockIdx.x, per_xcd);
     if (seg >= segments) {
         return;  // the whole workgroup: padding of the grid to a multiple of 8
@@ -485,11 +404,11 @@ __global__ __launch_bounds__(kOlaLanes) void fir_ola_kernel(const float2* __rest
     ola_segment(x, spec, y, n, halo, seg * (kOlaP - halo), buf, stages);
 }
 
-__global__ __launch_bounds__(kOlaLanes) void fir_ola_batched_kernel(
+__global__ __launch_bounds__(k4096Lanes) void fir_ola_batched_kernel(
     FirBatch batch, const float2* __restrict__ spec, int num_taps,
     const float2* __restrict__ stages)
 {
-    __shared__ float2 buf[kOlaSlots];
+    __shared__ float2 buf[k4096Slots];
     const int c = blockIdx.y;
     const int halo = num_taps - 1;
     const long long out0 = static_cast<long long>(blockIdx.x) * (kOlaP - halo);
@@ -679,9 +598,9 @@ void fir_launch(int impl, const float2* x, const float* taps, float2* y, int n, 
         const long long q = kOlaP - (num_taps - 1);
         const long long segments = ceil_div(n, q);
         const long long per_xcd = ceil_div(segments, 8);
-        fir_ola_kernel<<<static_cast<unsigned>(8 * per_xcd), kOlaLanes, 0, stream>>>(
+        fir_ola_kernel<<<static_cast<unsigned>(8 * per_xcd), k4096Lanes, 0, stream>>>(
             x, static_cast<const float2*>(spec.ptr), y, n, num_taps, segments, per_xcd,
-            ola_stage_table());
+            fft4096_stage_table());
         check_launch("fir_ola_kernel");
         return;
     }
@@ -741,8 +660,8 @@ void fir_batched_launch(int impl, const FirBatch& batch, int count, int longest,
             launch_ola_taps(taps, num_taps, static_cast<float2*>(spec.ptr), table, stream);
             const long long q = kOlaP - (num_taps - 1);
             const dim3 grid(static_cast<unsigned>(ceil_div(longest, q)), count);
-            fir_ola_batched_kernel<<<grid, kOlaLanes, 0, stream>>>(
-                batch, static_cast<const float2*>(spec.ptr), num_taps, ola_stage_table());
+            fir_ola_batched_kernel<<<grid, k4096Lanes, 0, stream>>>(
+                batch, static_cast<const float2*>(spec.ptr), num_taps, fft4096_stage_table());
             check_launch("fir_ola_batched_kernel");
             return;
         }

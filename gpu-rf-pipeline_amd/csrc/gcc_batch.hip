@@ -16,9 +16,12 @@
 //           towards that neighbour.
 // Pairs need 4096 <= L <= 2^24 (L = next power of two >= Na + Nb - 1, so Na + Nb - 1 > 2048);
 // GCC-PHAT's weights depend on L, so shorter pairs are not padded up but refused (the
-// single-pair rf_xcorr_fft handles them in one workgroup). Items are spread over the grid
-// with a fixed stride: all items cost the same.
+// single-pair rf_xcorr_fft handles them in one workgroup). All items cost the same; a
+// persistent grid walks them in XCD-major order. Pairs whose rows are 4096 points (L >= 2^15)
+// run their row pass as one 4096-point row per item on 256 lanes (register-resident radix-16
+// transforms); shorter rows use the generic LDS transform, F rows per item.
 
+#include "fft4096.hpp"
 #include "fft_plan.hpp"
 #include "twiddle.hpp"
 #include "xcorr_common.hpp"
@@ -54,9 +57,18 @@ struct rf_gcc_batch {
     std::vector<PairDesc> host;
     PairDesc* dev = nullptr;
     int* col_prefix = nullptr;  // count + 1
-    int* row_prefix = nullptr;  // count + 1
     int col_total = 0;
-    int row_total = 0;
+    // Row-pass items, split by row length: pairs with 4096-point rows run the register-resident
+    // radix-16 kernel, the others the generic LDS kernel. Each list is a set of pair indices
+    // with an item prefix over them.
+    struct RowList {
+        int count = 0;
+        int total = 0;
+        int* prefix = nullptr;  // count + 1
+        int* pairs = nullptr;   // count
+    };
+    RowList rows4096;
+    RowList rows_generic;
     float2* work = nullptr;
     std::map<int, float2*> tables;
     std::size_t bytes = 0;
@@ -111,7 +123,7 @@ __global__ __launch_bounds__(kItemThreads) void batch_column_fwd_kernel(
     const float2* __restrict__ table)
 {
     extern __shared__ float2 lds[];
-    for (int item = blockIdx.x; item < 2 * total; item += gridDim.x) {
+    xcd_major_items(2 * total, [&](int item) {
         const int which = item >= total ? 1 : 0;
         const int local = item - which * total;
         const int p = find_pair(prefix, count, local);
@@ -144,23 +156,24 @@ __global__ __launch_bounds__(kItemThreads) void batch_column_fwd_kernel(
                 store, lds, table);
         }
         __syncthreads();  // the next item reuses the LDS
-    }
+    });
 }
 
 __global__ __launch_bounds__(kItemThreads) void batch_row_kernel(
-    const PairDesc* __restrict__ pairs, const int* __restrict__ prefix, int count, int total,
-    const float2* __restrict__ table)
+    const PairDesc* __restrict__ pairs, const int* __restrict__ prefix,
+    const int* __restrict__ list, int count, int total, const float2* __restrict__ table)
 {
     extern __shared__ float2 lds[];
-    for (int item = blockIdx.x; item < total; item += gridDim.x) {
-        const int p = find_pair(prefix, count, item);
+    xcd_major_items(total, [&](int item) {
+        const int k = find_pair(prefix, count, item);
+        const int p = list[k];
         const PairDesc d = pairs[p];
         const FourStep fs = shape_of(d);
         const long long L = 1LL << d.log2L;
         const int n1 = 1 << d.log2n1;
         const int F = lds_batch_per_block(d.log2n1);
         const int slots = lds_slots(n1);
-        const int row0 = (item - prefix[p]) * F;
+        const int row0 = (item - prefix[k]) * F;
         float2* wa = d.work + static_cast<long long>(row0) * n1;
         float2* wb = d.work + L + static_cast<long long>(row0) * n1;
         float2* la = lds;
@@ -191,7 +204,27 @@ __global__ __launch_bounds__(kItemThreads) void batch_row_kernel(
             wb[e] = cmul(v, four_step_twiddle<true>(fs, static_cast<long long>(j1) * (row0 + rr)));
         }
         __syncthreads();
-    }
+    });
+}
+
+// Row pass of the pairs whose rows are 4096 points: one row per item, 256 lanes, the
+// register-resident radix-16 transforms.
+__global__ __launch_bounds__(k4096Lanes) void batch_row4096_kernel(
+    const PairDesc* __restrict__ pairs, const int* __restrict__ prefix,
+    const int* __restrict__ list, int count, int total, const float2* __restrict__ stages)
+{
+    __shared__ float2 buf[k4096Slots];
+    xcd_major_items(total, [&](int item) {
+        const int k = find_pair(prefix, count, item);
+        const PairDesc d = pairs[list[k]];
+        const FourStep fs = shape_of(d);
+        const long long L = 1LL << d.log2L;
+        const long long row = item - prefix[k];
+        const float2* wa = d.work + (row << k4096Log2);
+        float2* wb = d.work + L + (row << k4096Log2);
+        xcorr_row4096(wa, wb, fs, row, 1, 1.0f / static_cast<float>(L), buf, stages);
+        __syncthreads();  // the next item reuses buf
+    });
 }
 
 __global__ __launch_bounds__(kItemThreads) void batch_column_inv_peak_kernel(
@@ -200,7 +233,7 @@ __global__ __launch_bounds__(kItemThreads) void batch_column_inv_peak_kernel(
 {
     extern __shared__ float2 lds[];
     __shared__ unsigned long long wave_best[kItemThreads / 64];
-    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    xcd_major_items(total, [&](int item) {
         const int p = find_pair(prefix, count, item);
         const PairDesc d = pairs[p];
         const FourStep fs = shape_of(d);
@@ -252,7 +285,7 @@ __global__ __launch_bounds__(kItemThreads) void batch_column_inv_peak_kernel(
             atomicMax(reinterpret_cast<unsigned long long*>(&results[p]), best);
         }
         __syncthreads();
-    }
+    });
 }
 
 // r[n] from column j1 of the pass-2 output: sum_k2 Z[j1 + n1 k2] exp(+2 pi i j2 k2 / n2),
@@ -328,16 +361,28 @@ void batch_run(rf_gcc_batch& h, rf_peak_result* results, hipStream_t stream)
     batch_init_kernel<<<static_cast<unsigned>(ceil_div(h.count, 256)), 256, 0, stream>>>(results,
                                                                                          h.count);
     check_launch("batch_init_kernel");
+    // Persistent grids: grid_mult workgroups per CU, never more than the items, rounded up to a
+    // multiple of 8 for the XCD-major walk (surplus workgroups find no item).
     const int cap = h.grid_mult * cus;
-    const unsigned g1 = static_cast<unsigned>(std::min(2 * h.col_total, cap));
-    batch_column_fwd_kernel<<<g1, kItemThreads, column_lds_bytes(), stream>>>(
-        h.dev, h.col_prefix, h.count, h.col_total, table);
+    auto grid_for = [cap](int items) {
+        return static_cast<unsigned>(ceil_div(std::min(items, cap), 8) * 8);
+    };
+    batch_column_fwd_kernel<<<grid_for(2 * h.col_total), kItemThreads, column_lds_bytes(),
+                              stream>>>(h.dev, h.col_prefix, h.count, h.col_total, table);
     check_launch("batch_column_fwd_kernel");
-    const unsigned g2 = static_cast<unsigned>(std::min(h.row_total, cap));
-    batch_row_kernel<<<g2, kItemThreads, row_lds_bytes(), stream>>>(h.dev, h.row_prefix, h.count,
-                                                                    h.row_total, table);
-    check_launch("batch_row_kernel");
-    const unsigned g3 = static_cast<unsigned>(std::min(h.col_total, cap));
+    if (h.rows4096.count > 0) {
+        batch_row4096_kernel<<<grid_for(h.rows4096.total), k4096Lanes, 0, stream>>>(
+            h.dev, h.rows4096.prefix, h.rows4096.pairs, h.rows4096.count, h.rows4096.total,
+            fft4096_stage_table());
+        check_launch("batch_row4096_kernel");
+    }
+    if (h.rows_generic.count > 0) {
+        batch_row_kernel<<<grid_for(h.rows_generic.total), kItemThreads, row_lds_bytes(),
+                           stream>>>(h.dev, h.rows_generic.prefix, h.rows_generic.pairs,
+                                     h.rows_generic.count, h.rows_generic.total, table);
+        check_launch("batch_row_kernel");
+    }
+    const unsigned g3 = grid_for(h.col_total);
     batch_column_inv_peak_kernel<<<g3, kItemThreads, column_lds_bytes(), stream>>>(
         h.dev, h.col_prefix, h.count, h.col_total, results, table);
     check_launch("batch_column_inv_peak_kernel");
@@ -358,7 +403,10 @@ void destroy_batch(rf_gcc_batch* h)
     }
     (void)hipFree(h->dev);
     (void)hipFree(h->col_prefix);
-    (void)hipFree(h->row_prefix);
+    for (auto* list : {&h->rows4096, &h->rows_generic}) {
+        (void)hipFree(list->prefix);
+        (void)hipFree(list->pairs);
+    }
     (void)hipFree(h->work);
     for (auto& kv : h->tables) {
         (void)hipFree(kv.second);
@@ -387,7 +435,8 @@ rf_status rf_gcc_batch_create(int count, const int* num_a, const int* num_b,
             h->count = count;
             h->host.resize(static_cast<std::size_t>(count));
             std::vector<int> colp(static_cast<std::size_t>(count) + 1, 0);
-            std::vector<int> rowp(static_cast<std::size_t>(count) + 1, 0);
+            std::vector<int> row_pairs[2];            // 4096-point rows, other rows
+            std::vector<int> row_prefix[2] = {{0}, {0}};
             long long work_total = 0;
             for (int i = 0; i < count; ++i) {
                 if (num_a[i] < 1 || num_b[i] < 1) {
@@ -420,12 +469,14 @@ rf_status rf_gcc_batch_create(int count, const int* num_a, const int* num_b,
                 d.col_items = (1 << fs.log2n1) / rf::column_tile(fs);
                 d.row_items = (1 << fs.log2n2) / rf::lds_batch_per_block(fs.log2n1);
                 colp[static_cast<std::size_t>(i) + 1] = colp[static_cast<std::size_t>(i)] + d.col_items;
-                rowp[static_cast<std::size_t>(i) + 1] = rowp[static_cast<std::size_t>(i)] + d.row_items;
+                const int kind = fs.log2n1 == rf::k4096Log2 ? 0 : 1;
+                row_pairs[kind].push_back(i);
+                row_prefix[kind].push_back(row_prefix[kind].back() + d.row_items);
                 work_total += 2 * L;
             }
             h->col_total = colp.back();
-            h->row_total = rowp.back();
             (void)rf::twiddle_table_4096();
+            (void)rf::fft4096_stage_table();
             RF_HIP_CHECK(hipMalloc(&h->work, static_cast<std::size_t>(work_total) * sizeof(float2)));
             long long offset = 0;
             for (auto& d : h->host) {
@@ -445,15 +496,30 @@ rf_status rf_gcc_batch_create(int count, const int* num_a, const int* num_b,
             }
             RF_HIP_CHECK(hipMalloc(&h->dev, h->host.size() * sizeof(PairDesc)));
             RF_HIP_CHECK(hipMalloc(&h->col_prefix, colp.size() * sizeof(int)));
-            RF_HIP_CHECK(hipMalloc(&h->row_prefix, rowp.size() * sizeof(int)));
             RF_HIP_CHECK(hipMemcpy(h->col_prefix, colp.data(), colp.size() * sizeof(int),
                                    hipMemcpyHostToDevice));
-            RF_HIP_CHECK(hipMemcpy(h->row_prefix, rowp.data(), rowp.size() * sizeof(int),
-                                   hipMemcpyHostToDevice));
+            rf_gcc_batch::RowList* lists[2] = {&h->rows4096, &h->rows_generic};
+            for (int kind = 0; kind < 2; ++kind) {
+                rf_gcc_batch::RowList& list = *lists[kind];
+                list.count = static_cast<int>(row_pairs[kind].size());
+                list.total = row_prefix[kind].back();
+                if (list.count == 0) {
+                    continue;
+                }
+                RF_HIP_CHECK(hipMalloc(&list.prefix, row_prefix[kind].size() * sizeof(int)));
+                RF_HIP_CHECK(hipMalloc(&list.pairs, row_pairs[kind].size() * sizeof(int)));
+                RF_HIP_CHECK(hipMemcpy(list.prefix, row_prefix[kind].data(),
+                                       row_prefix[kind].size() * sizeof(int),
+                                       hipMemcpyHostToDevice));
+                RF_HIP_CHECK(hipMemcpy(list.pairs, row_pairs[kind].data(),
+                                       row_pairs[kind].size() * sizeof(int),
+                                       hipMemcpyHostToDevice));
+                h->bytes += (row_prefix[kind].size() + row_pairs[kind].size()) * sizeof(int);
+            }
             RF_HIP_CHECK(hipMemcpy(h->dev, h->host.data(), h->host.size() * sizeof(PairDesc),
                                    hipMemcpyHostToDevice));
             h->bytes += static_cast<std::size_t>(work_total) * sizeof(float2)
-                        + h->host.size() * sizeof(PairDesc) + (colp.size() + rowp.size()) * sizeof(int);
+                        + h->host.size() * sizeof(PairDesc) + colp.size() * sizeof(int);
         } catch (...) {
             rf::destroy_batch(h);
             throw;
@@ -507,6 +573,7 @@ rf_status rf_gcc_batch_launch_graph(rf_gcc_batch* h, rf_peak_result* results, rf
                 RF_HIP_CHECK(hipStreamCreateWithFlags(&h->capture, hipStreamNonBlocking));
             }
             (void)rf::twiddle_table_4096();
+            (void)rf::fft4096_stage_table();
             hipGraph_t graph = nullptr;
             RF_HIP_CHECK(hipStreamBeginCapture(h->capture, hipStreamCaptureModeThreadLocal));
             try {

@@ -54,7 +54,7 @@ std::once_flag g_stage_once[kMaxDevices];
 float2* g_stage[kMaxDevices] = {};
 }  // namespace
 
-const float2* ola_stage_table()
+const float2* fft4096_stage_table()
 {
     int device = 0;
     RF_HIP_CHECK(hipGetDevice(&device));

@@ -12,7 +12,7 @@ const float2* twiddle_table_4096();
 // Twiddles of the 4096-point radix-16 transform's passes with span 16 and 256, laid out
 // [pass][k][r] = exp(-2 pi i r k / (16 span)), r < 16, k < span (offsets 0 and 256; 4352
 // values).
-const float2* ola_stage_table();
+const float2* fft4096_stage_table();
 
 // Host-side float64 -> float32 table of exp(-2 pi i m / n) for m < count.
 void fill_twiddles(float2* host, long long count, long long n, long long step);

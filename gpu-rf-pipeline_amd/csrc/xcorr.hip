@@ -262,6 +262,19 @@ __global__ __launch_bounds__(512) void xcorr_row_kernel(float2* __restrict__ wor
     }
 }
 
+// Row pass for 4096-point rows: one row per workgroup, 256 lanes, register-resident radix-16
+// transforms (same arithmetic as xcorr_row_kernel up to the transform's rounding).
+__global__ __launch_bounds__(k4096Lanes) void xcorr_row4096_kernel(float2* __restrict__ work,
+                                                                   FourStep fs, int phat,
+                                                                   const float2* __restrict__ stages)
+{
+    __shared__ float2 buf[k4096Slots];
+    const long long L = 1LL << fs.log2L;
+    const long long row = blockIdx.x;
+    xcorr_row4096(work + (row << k4096Log2), work + L + (row << k4096Log2), fs, row, phat,
+                  1.0f / static_cast<float>(L), buf, stages);
+}
+
 // Inverse column pass of b's work buffer into r (natural order, first num_a + num_b - 1).
 __global__ __launch_bounds__(512) void xcorr_column_inv_kernel(const float2* __restrict__ work,
                                                                float2* __restrict__ r,
@@ -405,12 +418,18 @@ void xcorr_fft_run(const rf_xcorr_fft& h, const float2* a, const float2* b, floa
     xcorr_column_fwd_kernel<<<cgrid, C * n2 / 8, cbytes, stream>>>(a, b, h.work, h.num_a,
                                                                    h.num_b, fs, table);
     check_launch("xcorr_column_fwd_kernel");
-    const int F = lds_batch_per_block(fs.log2n1);
-    const std::size_t rbytes =
-        2 * static_cast<std::size_t>(F) * lds_slots(1 << fs.log2n1) * sizeof(float2);
-    xcorr_row_kernel<<<static_cast<unsigned>(ceil_div(n2, F)), F * ((1 << fs.log2n1) >> 3),
-                       rbytes, stream>>>(h.work, fs, h.phat, table);
-    check_launch("xcorr_row_kernel");
+    if (fs.log2n1 == k4096Log2) {
+        xcorr_row4096_kernel<<<static_cast<unsigned>(n2), k4096Lanes, 0, stream>>>(
+            h.work, fs, h.phat, fft4096_stage_table());
+        check_launch("xcorr_row4096_kernel");
+    } else {
+        const int F = lds_batch_per_block(fs.log2n1);
+        const std::size_t rbytes =
+            2 * static_cast<std::size_t>(F) * lds_slots(1 << fs.log2n1) * sizeof(float2);
+        xcorr_row_kernel<<<static_cast<unsigned>(ceil_div(n2, F)), F * ((1 << fs.log2n1) >> 3),
+                           rbytes, stream>>>(h.work, fs, h.phat, table);
+        check_launch("xcorr_row_kernel");
+    }
     const dim3 igrid(static_cast<unsigned>((1 << fs.log2n1) / C), 1);
     xcorr_column_inv_kernel<<<igrid, C * n2 / 8, cbytes, stream>>>(
         h.work, r, static_cast<long long>(h.num_a) + h.num_b - 1, fs, table);
@@ -529,6 +548,7 @@ rf_status rf_xcorr_fft_create(int num_a, int num_b, int phat, rf_xcorr_fft** out
         h->log2L = rf::ilog2(L);
         try {
             (void)rf::twiddle_table_4096();
+            (void)rf::fft4096_stage_table();
             if (h->log2L > rf::kTwiddleLog2) {
                 h->fs = rf::make_four_step_shape(h->log2L);
                 const long long tsize = rf::four_step_table_size(h->log2L);

@@ -1,6 +1,8 @@
 // Device helpers shared by the single-pair and the batched cross-correlation kernels.
 #pragma once
 
+#include "fft4096.hpp"
+#include "fft_passes.hpp"
 #include "rf_common.hpp"
 
 namespace rf {
@@ -41,6 +43,56 @@ __device__ __forceinline__ unsigned long long peak_key(float mag, long long idx)
 {
     return (static_cast<unsigned long long>(__float_as_uint(mag)) << 32)
            | static_cast<unsigned long long>(0xFFFFFFFFu - static_cast<unsigned>(idx));
+}
+
+// Row pass of the four-step cross-correlation for rows of n1 = 4096 points, 256 lanes: row
+// `row` of a's and b's column-pass outputs (wa, wb point at the row) are transformed, weighed,
+// transformed back and multiplied by the conjugate four-step twiddle w_L^-(j1 row), then
+// written over b's row. The caller puts a barrier between two calls that share `buf`.
+__device__ __forceinline__ void xcorr_row4096(const float2* __restrict__ wa,
+                                              float2* __restrict__ wb, const FourStep& fs,
+                                              long long row, int phat, float scale, float2* buf,
+                                              const float2* __restrict__ stages)
+{
+    const int t = threadIdx.x;
+    float2 ua[16];
+    float2 ub[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        ua[r] = wa[t + r * k4096Lanes];
+        ub[r] = wb[t + r * k4096Lanes];
+    }
+    fft4096<false>(buf, t, ua, stages);
+    __syncthreads();  // every lane has read a's last pass from buf
+    fft4096<false>(buf, t, ub, stages);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        ub[r] = weigh(ua[r], ub[r], scale, phat);
+    }
+    __syncthreads();
+    fft4096<true>(buf, t, ub, stages);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int j1 = t + r * k4096Lanes;
+        wb[j1] = cmul(ub[r], four_step_twiddle<true>(fs, j1 * row));
+    }
+}
+
+// Persistent walk over `total` items in XCD-major order: workgroups are dispatched round-robin
+// over the 8 XCDs (workgroup b on XCD b % 8), so the items are cut into 8 contiguous ranges and
+// range x is walked in order by XCD x's workgroups; neighbouring items, which share cache lines
+// (column tiles narrower than a line) or twiddle rows, then run together behind one L2. Needs a
+// grid that is a multiple of 8; every item is visited exactly once whatever the XCD count.
+template <class Body>
+__device__ __forceinline__ void xcd_major_items(int total, Body body)
+{
+    const int per = (total + 7) >> 3;
+    const int lo = (blockIdx.x & 7) * per;
+    const int hi = lo + per < total ? lo + per : total;
+    const int step = gridDim.x >> 3;
+    for (int item = lo + static_cast<int>(blockIdx.x >> 3); item < hi; item += step) {
+        body(item);
+    }
 }
 
 }  // namespace rf
