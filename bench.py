@@ -333,6 +333,107 @@ def bench_gcc_batch(rf, args) -> dict:
     }
 
 
+def bench_delay(rf, args) -> dict:
+    """Learned delay search: 4096 pairs of 2^14 samples (delay U[-512, 512), SNR U[-10, 20] dB,
+    AR(1) noise), network trained on 2^20 pairs synthesised on the GPU; compared with the exact
+    batched GCC-PHAT on the same pairs and with the coarse-proposal ablation."""
+    count, n = 4096, 1 << 14
+    train_pairs, train_batch, lr = 1 << 20, 1024, 1e-3
+    stream = rf.Stream()
+    mlp = rf.DelayMlp(train_batch, seed=0xC5A)
+    t0 = time.perf_counter()
+    losses = mlp.train(0xC5A, train_pairs, train_batch, n, lr, stream)
+    train_s = time.perf_counter() - t0
+    a = rf.DeviceBuffer(count * n)
+    b = rf.DeviceBuffer(count * n)
+    d = rf.DeviceBuffer(count, np.float32)
+    rf.synth_pairs(0xC5, 0, count, n, a.ptr, b.ptr, d.ptr, stream)
+    rf.synchronize()
+    delays = d.download()
+    search = rf.DelaySearch(count, n)
+    res = rf.DeviceBuffer(32 * count, np.uint8)
+
+    def timed(mlp_or_none, threshold):
+        ms = time_steps(rf, lambda i: search.run(a.ptr, b.ptr, mlp_or_none, threshold, res.ptr,
+                                                 stream), args.steps, args.warmup,
+                        stream.handle) / args.steps
+        return ms, rf.read_delay_results(res, count)
+
+    ms_mlp, r_mlp = timed(mlp, 0.9)
+    ms_coarse, r_coarse = timed(None, 0.0)
+    ms_full, r_full = timed(None, 2.0)
+    batch = rf.GccBatch([n] * count, [n] * count)
+    batch.bind([a.offset(k * n) for k in range(count)], [b.offset(k * n) for k in range(count)])
+    pres = rf.DeviceBuffer(16 * count, np.uint8)
+    batch.tune(pres.ptr, stream.handle)
+    ms_exact = time_steps(rf, lambda i: batch.run(pres.ptr, stream.handle, graph=True),
+                          args.steps, args.warmup, stream.handle) / args.steps
+    exact = rf.read_peaks(pres, count)
+
+    def quality(results):
+        agree = sum(r.index == e.index for r, e in zip(results, exact))
+        fallback = sum(r.path == 1 for r in results)
+        err = [abs(r.index - (n - 1) + r.frac - dd) for r, dd in zip(results, delays)]
+        return {"lag_equal_to_exact": int(agree), "agreement": round(agree / count, 5),
+                "fallback_rate": round(fallback / count, 5),
+                "within_0.1_sample": int(sum(e <= 0.1 for e in err)),
+                "max_abs_lag_error": round(float(max(err)), 4)}
+
+    labels = [int(min(max((int(np.rint(x)) + 512) >> 4, 0), 63)) for x in delays]
+    proposals_right = sum(r.proposal == lab for r, lab in zip(r_mlp, labels))
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import delay as D
+        oracle = load_oracle()
+        ha = a.download(12 * n).reshape(12, n)
+        hb = b.download(12 * n).reshape(12, n)
+        t0 = time.perf_counter()
+        for k in range(12):
+            D.features(ha[k], hb[k])
+            oracle.peak(oracle.gcc_phat(ha[k], hb[k]))
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(12 / dt, 3), "unit": "pairs/s", "cores": 1, "kind": "port",
+               "sample": "float64 oracle features + exact GCC-PHAT + peak, one thread, 12 pairs"}
+    L = 1 << 15
+    flop_features = 2 * 5 * L * 15 + 12 * L          # two forward FFTs + weighting
+    flop_inverse = 5 * L * 15
+    flop_mlp = 2 * (256 * 256 + 256 * 256 + 256 * 64)
+    fallback = sum(r.path == 1 for r in r_mlp)
+    flop_step = count * (flop_features + flop_mlp) + fallback * flop_inverse \
+        + (count - fallback) * (L * 8 * (6 + 1) + L * 6)
+    bytes_step = count * (16 * n + 32)
+    best = ms_mlp
+    return {
+        "metric": METRIC, "value": round(count / (best * 1e-3), 1), "unit": "pairs/s",
+        "ms_per_step": round(best, 5), "dtype": "f32 (network: bf16 operands, f32 accumulate)",
+        "config": {"workload": f"learned delay search, {count} pairs of {n} complex64 samples "
+                               "(delay U[-512,512), SNR U[-10,20] dB, AR(1) noise), "
+                               "threshold 0.9",
+                   "pairs": count, "samples_per_channel": n, "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": round(bytes_step / (best * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(bytes_step / (best * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None},
+        "fft_valu": {"flop_per_step": int(flop_step),
+                     "achieved_tflops": round(flop_step / (best * 1e-3) / 1e12, 2),
+                     "peak_tflops": FP32_PEAK_TFLOPS,
+                     "frac": round(flop_step / (best * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)},
+        "training": {"pairs": train_pairs, "batch": train_batch, "steps": int(losses.size),
+                     "learning_rate": lr, "seconds": round(train_s, 2),
+                     "loss_first": round(float(losses[:8].mean()), 4),
+                     "loss_last": round(float(losses[-8:].mean()), 4)},
+        "mlp": {"ms": round(ms_mlp, 4), **quality(r_mlp),
+                "proposal_equal_to_label": int(proposals_right)},
+        "ablation_coarse_proposal": {"ms": round(ms_coarse, 4), **quality(r_coarse)},
+        "full_inverse_path": {"ms": round(ms_full, 4), **quality(r_full)},
+        "exact_gcc_batch": {"ms": round(ms_exact, 4),
+                            "pairs_per_s": round(count / (ms_exact * 1e-3), 1),
+                            "speedup_of_mlp_search": round(ms_exact / ms_mlp, 3)},
+        "cpu_baseline": cpu,
+    }
+
+
 def bench_copy(rf, args) -> dict:
     """Achievable HBM bandwidth: the library's streaming copy over 1 GiB (read + write)."""
     nbytes = 1 << 30
@@ -376,7 +477,7 @@ def main() -> int:
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--impl", default="fft_ola")
     p.add_argument("--workload", default="fir_long",
-                   choices=["fir_long", "fir_plain", "channelize", "gcc_batch", "copy"])
+                   choices=["fir_long", "fir_plain", "channelize", "gcc_batch", "delay", "copy"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-variants", action="store_true",
                    help="FIR: time only --impl, not the other variants")
@@ -390,6 +491,8 @@ def main() -> int:
         out = bench_channelize(rf, args)
     elif args.workload == "copy":
         out = bench_copy(rf, args)
+    elif args.workload == "delay":
+        out = bench_delay(rf, args)
     elif args.workload == "gcc_batch":
         out = bench_gcc_batch(rf, args)
     else:

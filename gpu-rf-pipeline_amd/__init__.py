@@ -125,6 +125,21 @@ _proto("rf_gcc_batch_bind", _st, _vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
 _proto("rf_gcc_batch_run", _st, _vp, _vp, _vp)
 _proto("rf_gcc_batch_launch_graph", _st, _vp, _vp, _vp)
 _proto("rf_gcc_batch_tune", _st, _vp, _vp, _vp, ctypes.POINTER(_int))
+_ull = ctypes.c_ulonglong
+_fp = ctypes.c_float
+_proto("rf_synth_pairs", _st, _ull, _int, _int, _int, _vp, _vp, _vp, _vp)
+_proto("rf_delay_mlp_create", _st, _int, _ull, ctypes.POINTER(_vp))
+_proto("rf_delay_mlp_destroy", _st, _vp)
+_proto("rf_delay_mlp_params", _st, _vp, ctypes.POINTER(_vp), ctypes.POINTER(_size))
+_proto("rf_delay_mlp_forward", _st, _vp, _vp, _int, _vp, _vp)
+_proto("rf_delay_mlp_gradients", _st, _vp, _vp, _vp, _int, _vp, _vp, _vp)
+_proto("rf_delay_mlp_train_step", _st, _vp, _vp, _vp, _int, _fp, _vp, _vp)
+_proto("rf_delay_mlp_train", _st, _vp, _ull, _int, _int, _int, _fp, _vp, _vp)
+_proto("rf_delay_search_create", _st, _int, _int, ctypes.POINTER(_vp))
+_proto("rf_delay_search_destroy", _st, _vp)
+_proto("rf_delay_search_bytes", _st, _vp, ctypes.POINTER(_size))
+_proto("rf_delay_search_features", _st, _vp, _vp, _vp, _vp, _vp, _vp)
+_proto("rf_delay_search_run", _st, _vp, _vp, _vp, _vp, _fp, _vp, _vp)
 
 
 def lib() -> ctypes.CDLL:
@@ -664,6 +679,133 @@ class GccBatch:
                 self.handle = None
         except Exception:
             pass
+
+
+DELAY_FEATURES = 256
+DELAY_BINS = 64
+DELAY_SPAN = 512
+DELAY_MLP_PARAMS = 148032
+
+
+class DelayResult(ctypes.Structure):
+    """rf_delay_result: peak (index, magnitude, frac) plus proposal, confidence and path."""
+    _fields_ = [("index", ctypes.c_int32), ("magnitude", ctypes.c_float),
+                ("frac", ctypes.c_double), ("proposal", ctypes.c_int32),
+                ("confidence", ctypes.c_float), ("path", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+def synth_pairs(seed: int, first: int, count: int, n: int, a: int, b: int,
+                delays: int | None = None, stream=None) -> None:
+    """Synthetic capture pairs into device buffers a, b (count * n each) and delays."""
+    check(_lib.rf_synth_pairs(seed, first, count, n, a, b, delays, _sp(stream)))
+
+
+class DelayMlp:
+    """Delay-proposal network (rf_delay_mlp_*): 256-256-256-64, GELU, BF16 GEMM operands."""
+
+    def __init__(self, max_batch: int, seed: int = 0xC5A):
+        p = _vp()
+        check(_lib.rf_delay_mlp_create(max_batch, seed, ctypes.byref(p)))
+        self.handle = p.value
+        self.max_batch = max_batch
+
+    def params_ptr(self) -> int:
+        p = _vp()
+        n = _size(0)
+        check(_lib.rf_delay_mlp_params(self.handle, ctypes.byref(p), ctypes.byref(n)))
+        return p.value
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(DELAY_MLP_PARAMS, dtype=np.float32)
+        check(_lib.rf_memcpy_d2h(out.ctypes.data, self.params_ptr(), out.nbytes, None))
+        check(_lib.rf_stream_synchronize(None))
+        return out
+
+    def set_params(self, params: np.ndarray) -> None:
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        assert params.size == DELAY_MLP_PARAMS
+        check(_lib.rf_memcpy_h2d(self.params_ptr(), params.ctypes.data, params.nbytes, None))
+        check(_lib.rf_stream_synchronize(None))
+
+    def forward(self, features: int, batch: int, probs: int, stream=None) -> None:
+        check(_lib.rf_delay_mlp_forward(self.handle, features, batch, probs, _sp(stream)))
+
+    def gradients(self, features: int, labels: int, batch: int, grads: int | None, loss: int,
+                  stream=None) -> None:
+        check(_lib.rf_delay_mlp_gradients(self.handle, features, labels, batch, grads, loss,
+                                          _sp(stream)))
+
+    def train_step(self, features: int, labels: int, batch: int, lr: float, loss: int,
+                   stream=None) -> None:
+        check(_lib.rf_delay_mlp_train_step(self.handle, features, labels, batch, lr, loss,
+                                           _sp(stream)))
+
+    def train(self, seed: int, pairs: int, batch: int, n: int, lr: float,
+              stream=None) -> np.ndarray:
+        """Trains on `pairs` synthesised pairs; returns the per-step mean losses."""
+        steps = -(-pairs // batch)
+        losses = np.zeros(steps, dtype=np.float32)
+        check(_lib.rf_delay_mlp_train(self.handle, seed, pairs, batch, n, lr,
+                                      losses.ctypes.data, _sp(stream)))
+        return losses
+
+    def destroy(self) -> None:
+        if self.handle:
+            check(_lib.rf_delay_mlp_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.rf_delay_mlp_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class DelaySearch:
+    """Learned delay search over `count` pairs of n samples (rf_delay_search_*)."""
+
+    def __init__(self, count: int, n: int):
+        p = _vp()
+        check(_lib.rf_delay_search_create(count, n, ctypes.byref(p)))
+        self.handle = p.value
+        self.count = count
+        self.n = n
+
+    def features(self, a: int, b: int, features: int | None, coarse: int | None,
+                 stream=None) -> None:
+        check(_lib.rf_delay_search_features(self.handle, a, b, features, coarse, _sp(stream)))
+
+    def run(self, a: int, b: int, mlp: "DelayMlp | None", threshold: float, results: int,
+            stream=None) -> None:
+        check(_lib.rf_delay_search_run(self.handle, a, b, mlp.handle if mlp else None,
+                                       threshold, results, _sp(stream)))
+
+    def bytes(self) -> int:
+        n = _size(0)
+        check(_lib.rf_delay_search_bytes(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def destroy(self) -> None:
+        if self.handle:
+            check(_lib.rf_delay_search_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.rf_delay_search_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def read_delay_results(d_results: "DeviceBuffer", count: int) -> list[DelayResult]:
+    raw = d_results.download().tobytes()
+    size = ctypes.sizeof(DelayResult)
+    return [DelayResult.from_buffer_copy(raw[i * size:(i + 1) * size]) for i in range(count)]
 
 
 def read_peaks(d_results: "DeviceBuffer", count: int) -> list[PeakResult]:

@@ -1,0 +1,611 @@
+// Learned delay search: batched GCC-PHAT over equal-length capture pairs in which a small
+// network proposes the delay and the GCC-PHAT output is then evaluated exactly, but only over a
+// 64-lag window around the proposal; pairs the network is unsure of take the full inverse
+// transform. No single reference function: the exact path is gcc_phat_cuda
+// (src/xcorr.cu:379-399) plus the peak loop of rfgpu doa (src/main.cpp:390-401).
+//
+// Per pair (n samples per channel, L = nextpow2(2n - 1), 2^15 <= L <= 2^20, rows of 4096):
+//   1 forward column pass of a and b (shared with rf_gcc_batch);
+//   2 row pass: 4096-point FFTs of a's and b's rows, PHAT weighting W[k] (the GCC-PHAT
+//     cross-spectrum, 1/L included), W stored in place of b's row, and band sums of the
+//     de-rotated spectrum W'[k] = W[k] exp(+2 pi i k (n-1) / L) (lag 0 at index 0) over 1024
+//     bands of L/1024 bins;
+//   3 features: the 1024-point inverse DFT of the band sums, c[m] = (1/B) sum over lags l = m
+//     (mod 1024) of r(l) D(l), D the band's Dirichlet kernel (|D/B| >= 0.64 for |l| <= 512,
+//     zero at nonzero multiples of 1024), so |c| is a coarse GCC-PHAT magnitude over lags
+//     [-512, 512); feature i < 256 = max of |c| over lags -512 + 4i .. -512 + 4i + 3, divided
+//     by the largest feature; the coarse proposal is the bin of the largest |c|;
+//   4 proposal: the network's most probable of 64 bins of 16 lags (bin b: lags -512 + 16b ..
+//     -512 + 16b + 15) and its probability, or the coarse proposal with confidence 1;
+//   5 refine (confidence >= threshold): r at the 64 lags -512 + 16b - 24 .. -512 + 16b + 39
+//     as a pruned inverse DFT of W, r(n0 + m) = sum_p w^(pm) sum_q V[p + Rq] exp(2 pi i qm / 64)
+//     with V[k] = W[k] w^(k n0), w = exp(2 pi i / L), R = L / 64; the peak is the first largest
+//     |r| in the window; at either window edge the pair falls back;
+//   6 fallback: inverse row and column passes with the fused peak search of rf_gcc_batch.
+// Peaks and the sub-sample offset follow rf_gcc_batch (first largest |r|; f = m1 / (m0 + m1)
+// towards the larger neighbour).
+
+#include "gcc_batch_kernels.hpp"
+#include "fir_internal.hpp"
+#include "twiddle.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+struct rf_delay_mlp;
+
+namespace rf {
+void synth_pairs(uint64_t seed, int first, int count, int n, float2* a, float2* b, float* delays,
+                 hipStream_t stream);
+void delay_mlp_forward(rf_delay_mlp& h, const float* x, int batch, float* probs,
+                       hipStream_t stream);
+void delay_mlp_train_step(rf_delay_mlp& h, const float* x, const int* labels, int batch,
+                          float lr, float* loss, hipStream_t stream);
+int delay_mlp_max_batch(const rf_delay_mlp& h);
+}  // namespace rf
+
+struct rf_delay_search {
+    int count = 0;
+    int n = 0;
+    int log2L = 0;
+    rf::FourStep fs{};
+    std::vector<rf::PairDesc> host;
+    rf::PairDesc* dev = nullptr;
+    int* col_prefix = nullptr;
+    int col_total = 0;
+    float2* work = nullptr;     // count * 2 L
+    float2* tables = nullptr;   // four-step twiddles of L
+    float2* partial = nullptr;  // count * n2 * 1024 band sums per row
+    float* features = nullptr;  // count * 256
+    int* coarse = nullptr;      // count
+    float* probs = nullptr;     // count * 64
+    int* active = nullptr;      // count: 1 = full inverse transform
+    std::size_t bytes = 0;
+};
+
+namespace rf {
+namespace {
+
+constexpr int kBands = 1024;
+constexpr int kFeatures = RF_DELAY_FEATURES;
+constexpr int kBins = RF_DELAY_BINS;
+constexpr int kSpan = RF_DELAY_SPAN;
+constexpr int kWindow = 64;
+constexpr int kWindowMargin = 24;
+constexpr int kRefineThreads = 256;
+constexpr int kRefineBatch = kRefineThreads / 8;  // 64-point transforms per LDS round
+
+__global__ void search_bind_kernel(PairDesc* __restrict__ pairs, int count, const float2* a,
+                                   const float2* b, int n)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < count) {
+        pairs[p].a = a + static_cast<long long>(p) * n;
+        pairs[p].b = b + static_cast<long long>(p) * n;
+    }
+}
+
+// One 4096-point row per item: W into b's row, de-rotated band sums into `partial`.
+__global__ __launch_bounds__(k4096Lanes) void search_row_kernel(
+    const PairDesc* __restrict__ pairs, int total, float2* __restrict__ partial,
+    const float2* __restrict__ stages)
+{
+    __shared__ float2 buf[k4096Slots];
+    xcd_major_items(total, [&](int item) {
+        const PairDesc d = pairs[0];  // every pair has the same shape
+        const int log2n2 = d.log2n2;
+        const int p = item >> log2n2;
+        const long long row = item & ((1 << log2n2) - 1);
+        const PairDesc dp = pairs[p];
+        const FourStep fs = shape_of(dp);
+        const long long L = 1LL << dp.log2L;
+        const float2* wa = dp.work + (row << k4096Log2);
+        float2* wb = dp.work + L + (row << k4096Log2);
+        const int t = threadIdx.x;
+        float2 ua[16];
+        float2 ub[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            ua[r] = wa[t + r * k4096Lanes];
+            ub[r] = wb[t + r * k4096Lanes];
+        }
+        fft4096<false>(buf, t, ua, stages);
+        __syncthreads();
+        fft4096<false>(buf, t, ub, stages);
+        const float scale = 1.0f / static_cast<float>(L);
+        const long long rot = dp.num_a - 1;
+        float2* out = partial + (static_cast<long long>(p) << log2n2 << 10) + (row << 10);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int k1 = t + r * k4096Lanes;
+            const float2 w = weigh(ua[r], ub[r], scale, 1);
+            wb[k1] = w;
+            const long long k = row + (static_cast<long long>(k1) << log2n2);
+            float2 v = cmul(w, four_step_twiddle<true>(fs, (k * rot) & (L - 1)));
+            // band k1 / 4: lanes 4j .. 4j + 3 hold its four bins of this row
+            v.x += __shfl_xor(v.x, 1, 64);
+            v.y += __shfl_xor(v.y, 1, 64);
+            v.x += __shfl_xor(v.x, 2, 64);
+            v.y += __shfl_xor(v.y, 2, 64);
+            if ((t & 3) == 0) {
+                out[k1 >> 2] = v;
+            }
+        }
+        __syncthreads();  // the next item reuses buf
+    });
+}
+
+// One pair per workgroup: band sums -> coarse correlation -> 256 features, coarse proposal.
+__global__ __launch_bounds__(256) void search_features_kernel(
+    const float2* __restrict__ partial, int log2n2, float* __restrict__ features,
+    int* __restrict__ coarse, const float2* __restrict__ table)
+{
+    __shared__ float2 lds[lds_slots(kBands)];
+    __shared__ unsigned long long wave_key[4];
+    __shared__ float wave_max[4];
+    const int p = blockIdx.x;
+    const int t = threadIdx.x;
+    const int rows = 1 << log2n2;
+    const float2* src = partial + (static_cast<long long>(p) << log2n2 << 10);
+    for (int j = t; j < kBands; j += 256) {
+        float2 y = make_float2(0.0f, 0.0f);
+        for (int r = 0; r < rows; ++r) {
+            y = cadd(y, src[(r << 10) + j]);
+        }
+        lds[lds_pad(j)] = y;
+    }
+    __syncthreads();
+    lds_fft<true>(lds, 10, t, t < kBands / 8, table);
+    // lags -512 + 4t + s, s < 4, at index (lag mod 1024)
+    float f = 0.0f;
+    unsigned long long key = 0ULL;
+    for (int s = 0; s < 4; ++s) {
+        const int lag = -kSpan + 4 * t + s;
+        const float2 c = lds[lds_pad(lag & (kBands - 1))];
+        const float mag = hypotf(c.x, c.y);
+        f = fmaxf(f, mag);
+        const unsigned long long k = peak_key(mag, lag + kSpan);
+        key = k > key ? k : key;
+    }
+    float fmax = f;
+    for (int off = 32; off > 0; off >>= 1) {
+        fmax = fmaxf(fmax, __shfl_xor(fmax, off, 64));
+        const unsigned long long other = __shfl_xor(key, off, 64);
+        key = other > key ? other : key;
+    }
+    if ((t & 63) == 0) {
+        wave_max[t >> 6] = fmax;
+        wave_key[t >> 6] = key;
+    }
+    __syncthreads();
+    fmax = fmaxf(fmaxf(wave_max[0], wave_max[1]), fmaxf(wave_max[2], wave_max[3]));
+    features[static_cast<long long>(p) * kFeatures + t] = fmax > 0.0f ? f / fmax : 0.0f;
+    if (t == 0) {
+        for (int w = 1; w < 4; ++w) {
+            key = wave_key[w] > key ? wave_key[w] : key;
+        }
+        const int index = static_cast<int>(0xFFFFFFFFu - static_cast<unsigned>(key & 0xFFFFFFFFULL));
+        coarse[p] = index >> 4;  // (lag + 512) / 16
+    }
+}
+
+// Proposal and path per pair.
+__global__ void search_decide_kernel(const float* __restrict__ probs,
+                                     const int* __restrict__ coarse, int count, float threshold,
+                                     rf_delay_result* __restrict__ results, int* __restrict__ active)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= count) {
+        return;
+    }
+    int bin = coarse[p];
+    float conf = 1.0f;
+    if (probs != nullptr) {
+        const float* q = probs + static_cast<long long>(p) * kBins;
+        bin = 0;
+        conf = q[0];
+        for (int j = 1; j < kBins; ++j) {
+            if (q[j] > conf) {
+                conf = q[j];
+                bin = j;
+            }
+        }
+    }
+    const int path = conf >= threshold ? 0 : 1;
+    results[p].proposal = bin;
+    results[p].confidence = conf;
+    results[p].path = path;
+    active[p] = path;
+}
+
+// Exact GCC-PHAT over the 64-lag window of a refined pair; one workgroup per pair.
+__global__ __launch_bounds__(kRefineThreads) void search_refine_kernel(
+    const PairDesc* __restrict__ pairs, rf_delay_result* __restrict__ results,
+    int* __restrict__ active, const float2* __restrict__ table)
+{
+    constexpr int slots = lds_slots(kWindow);
+    __shared__ float2 lds[kRefineBatch * slots];
+    __shared__ float2 part[4][kWindow];
+    __shared__ float mag[kWindow];
+    const int p = blockIdx.x;
+    if (active[p] != 0) {
+        return;
+    }
+    const int t = threadIdx.x;
+    const PairDesc d = pairs[p];
+    const FourStep fs = shape_of(d);
+    const long long L = 1LL << d.log2L;
+    const int log2n2 = d.log2n2;
+    const int n2mask = (1 << log2n2) - 1;
+    const float2* W = d.work + L;
+    const int bin = results[p].proposal;
+    const long long lo = -kSpan + 16LL * bin - kWindowMargin;   // first lag of the window
+    const long long n0 = (lo + d.num_a - 1 + L) & (L - 1);       // its circular index
+    const int R = static_cast<int>(L >> 6);
+    const int m = t & (kWindow - 1);
+    const int g = t >> 6;
+    float2 acc = make_float2(0.0f, 0.0f);
+    for (int p0 = 0; p0 < R; p0 += kRefineBatch) {
+        for (int e = t; e < kRefineBatch * kWindow; e += kRefineThreads) {
+            const int f = e >> 6;
+            const int q = e & (kWindow - 1);
+            const long long k = p0 + f + static_cast<long long>(R) * q;
+            const float2 w = W[((k & n2mask) << k4096Log2) + (k >> log2n2)];
+            lds[f * slots + lds_pad(q)] = cmul(w, four_step_twiddle<true>(fs, (k * n0) & (L - 1)));
+        }
+        __syncthreads();
+        lds_fft<true>(lds + (t >> 3) * slots, 6, t & 7, true, table);
+#pragma unroll
+        for (int j = 0; j < kRefineBatch / 4; ++j) {
+            const int f = g * (kRefineBatch / 4) + j;
+            const float2 u = lds[f * slots + lds_pad(m)];
+            acc = cadd(acc, cmul(u, four_step_twiddle<true>(fs, static_cast<long long>(p0 + f) * m)));
+        }
+        __syncthreads();
+    }
+    part[g][m] = acc;
+    __syncthreads();
+    if (t < kWindow) {
+        const float2 r = cadd(cadd(part[0][t], part[1][t]), cadd(part[2][t], part[3][t]));
+        const float a = hypotf(r.x, r.y);
+        mag[t] = a;
+        unsigned long long key = peak_key(a, t);
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long other = __shfl_xor(key, off, 64);
+            key = other > key ? other : key;
+        }
+        if (t == 0) {
+            const int best = static_cast<int>(0xFFFFFFFFu - static_cast<unsigned>(key & 0xFFFFFFFFULL));
+            if (best == 0 || best == kWindow - 1) {
+                active[p] = 1;           // the peak may lie outside the window
+                results[p].path = 1;
+            } else {
+                const double m0 = mag[best];
+                const double ml = mag[best - 1];
+                const double mh = mag[best + 1];
+                double frac = 0.0;
+                if (m0 > 0.0) {
+                    frac = mh >= ml ? mh / (m0 + mh) : -ml / (m0 + ml);
+                    frac = frac > 0.5 ? 0.5 : (frac < -0.5 ? -0.5 : frac);
+                }
+                results[p].index = static_cast<int>(lo + best + d.num_a - 1);
+                results[p].magnitude = mag[best];
+                results[p].frac = frac;
+            }
+        }
+    }
+}
+
+// Inverse row pass of the fallback pairs: W row -> inverse 4096-point FFT -> conjugate
+// four-step twiddle, in place (the input of the shared inverse column pass).
+__global__ __launch_bounds__(k4096Lanes) void search_inverse_row_kernel(
+    const PairDesc* __restrict__ pairs, int total, const int* __restrict__ active,
+    const float2* __restrict__ stages)
+{
+    __shared__ float2 buf[k4096Slots];
+    xcd_major_items(total, [&](int item) {
+        const int log2n2 = pairs[0].log2n2;
+        const int p = item >> log2n2;
+        if (active[p] == 0) {
+            return;
+        }
+        const long long row = item & ((1 << log2n2) - 1);
+        const PairDesc d = pairs[p];
+        const FourStep fs = shape_of(d);
+        float2* wb = d.work + (1LL << d.log2L) + (row << k4096Log2);
+        const int t = threadIdx.x;
+        float2 u[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            u[r] = wb[t + r * k4096Lanes];
+        }
+        fft4096<true>(buf, t, u, stages);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int j1 = t + r * k4096Lanes;
+            wb[j1] = cmul(u[r], four_step_twiddle<true>(fs, j1 * row));
+        }
+        __syncthreads();
+    });
+}
+
+__global__ void search_labels_kernel(const float* __restrict__ delays, int count,
+                                     int* __restrict__ labels)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < count) {
+        const int lag = static_cast<int>(rintf(delays[p]));
+        const int bin = (lag + kSpan) >> 4;
+        labels[p] = bin < 0 ? 0 : (bin >= kBins ? kBins - 1 : bin);
+    }
+}
+
+unsigned persistent_grid(long long items, int per_cu)
+{
+    const long long cap = static_cast<long long>(per_cu) * cu_count();
+    return static_cast<unsigned>(ceil_div(std::min(items, cap), 8) * 8);
+}
+
+void features_launch(rf_delay_search& h, const float2* a, const float2* b, hipStream_t stream)
+{
+    const float2* table = twiddle_table_4096();
+    search_bind_kernel<<<static_cast<unsigned>(ceil_div(h.count, 256)), 256, 0, stream>>>(
+        h.dev, h.count, a, b, h.n);
+    check_launch("search_bind_kernel");
+    batch_column_fwd_kernel<<<persistent_grid(2LL * h.col_total, 4), kItemThreads,
+                              column_lds_bytes(), stream>>>(h.dev, h.col_prefix, h.count,
+                                                            h.col_total, table);
+    check_launch("batch_column_fwd_kernel");
+    const int rows = h.count << h.fs.log2n2;
+    search_row_kernel<<<persistent_grid(rows, 2), k4096Lanes, 0, stream>>>(
+        h.dev, rows, h.partial, fft4096_stage_table());
+    check_launch("search_row_kernel");
+    search_features_kernel<<<static_cast<unsigned>(h.count), 256, 0, stream>>>(
+        h.partial, h.fs.log2n2, h.features, h.coarse, table);
+    check_launch("search_features_kernel");
+}
+
+void search_launch(rf_delay_search& h, const float2* a, const float2* b, rf_delay_mlp* mlp,
+                   float threshold, rf_delay_result* results, hipStream_t stream)
+{
+    const float2* table = twiddle_table_4096();
+    batch_init_kernel<<<static_cast<unsigned>(ceil_div(h.count, 256)), 256, 0, stream>>>(
+        results, h.count);
+    check_launch("batch_init_kernel");
+    features_launch(h, a, b, stream);
+    if (mlp != nullptr) {
+        const int chunk = delay_mlp_max_batch(*mlp);
+        for (int first = 0; first < h.count; first += chunk) {
+            const int rows = std::min(chunk, h.count - first);
+            delay_mlp_forward(*mlp, h.features + static_cast<long long>(first) * kFeatures, rows,
+                              h.probs + static_cast<long long>(first) * kBins, stream);
+        }
+    }
+    search_decide_kernel<<<static_cast<unsigned>(ceil_div(h.count, 256)), 256, 0, stream>>>(
+        mlp != nullptr ? h.probs : nullptr, h.coarse, h.count, threshold, results, h.active);
+    check_launch("search_decide_kernel");
+    search_refine_kernel<<<static_cast<unsigned>(h.count), kRefineThreads, 0, stream>>>(
+        h.dev, results, h.active, table);
+    check_launch("search_refine_kernel");
+    const int rows = h.count << h.fs.log2n2;
+    search_inverse_row_kernel<<<persistent_grid(rows, 2), k4096Lanes, 0, stream>>>(
+        h.dev, rows, h.active, fft4096_stage_table());
+    check_launch("search_inverse_row_kernel");
+    batch_column_inv_peak_kernel<<<persistent_grid(h.col_total, 4), kItemThreads,
+                                   column_lds_bytes(), stream>>>(
+        h.dev, h.col_prefix, h.count, h.col_total, results, h.active, table);
+    check_launch("batch_column_inv_peak_kernel");
+    batch_finish_kernel<<<static_cast<unsigned>(h.count), 64, 0, stream>>>(h.dev, results,
+                                                                           h.active);
+    check_launch("batch_finish_kernel");
+}
+
+void destroy_search(rf_delay_search* h)
+{
+    if (h == nullptr) {
+        return;
+    }
+    (void)hipFree(h->dev);
+    (void)hipFree(h->col_prefix);
+    (void)hipFree(h->work);
+    (void)hipFree(h->tables);
+    (void)hipFree(h->partial);
+    (void)hipFree(h->features);
+    (void)hipFree(h->coarse);
+    (void)hipFree(h->probs);
+    (void)hipFree(h->active);
+    delete h;
+}
+
+rf_delay_search* create_search(int count, int n)
+{
+    if (count < 1 || n < 1) fail(RF_INVALID_ARG, "delay search: need count >= 1 and n >= 1");
+    const long long lags = 2LL * n - 1;
+    long long L = 1;
+    while (L < lags) {
+        L *= 2;
+    }
+    if (L < (1LL << 15) || L > (1LL << 20)) {
+        fail(RF_LIMIT, "delay search: n = " + std::to_string(n) + " gives an FFT of "
+                           + std::to_string(L) + " points; supported 2^15 .. 2^20 "
+                           "(8193 <= n <= 524288)");
+    }
+    rf_delay_search* h = new rf_delay_search;
+    try {
+        h->count = count;
+        h->n = n;
+        h->log2L = ilog2(L);
+        h->fs = make_four_step_shape(h->log2L);
+        (void)twiddle_table_4096();
+        (void)fft4096_stage_table();
+        const long long tsize = four_step_table_size(h->log2L);
+        RF_HIP_CHECK(hipMalloc(&h->tables, tsize * sizeof(float2)));
+        upload_four_step_tables(h->log2L, h->tables);
+        const long long work = 2LL * L * count;
+        RF_HIP_CHECK(hipMalloc(&h->work, work * sizeof(float2)));
+        const int n2 = 1 << h->fs.log2n2;
+        const long long partial = static_cast<long long>(count) * n2 * kBands;
+        RF_HIP_CHECK(hipMalloc(&h->partial, partial * sizeof(float2)));
+        RF_HIP_CHECK(hipMalloc(&h->features, static_cast<std::size_t>(count) * kFeatures * sizeof(float)));
+        RF_HIP_CHECK(hipMalloc(&h->coarse, static_cast<std::size_t>(count) * sizeof(int)));
+        RF_HIP_CHECK(hipMalloc(&h->probs, static_cast<std::size_t>(count) * kBins * sizeof(float)));
+        RF_HIP_CHECK(hipMalloc(&h->active, static_cast<std::size_t>(count) * sizeof(int)));
+        h->host.resize(static_cast<std::size_t>(count));
+        std::vector<int> prefix(static_cast<std::size_t>(count) + 1, 0);
+        const int col_items = (1 << h->fs.log2n1) / column_tile(h->fs);
+        for (int i = 0; i < count; ++i) {
+            PairDesc& d = h->host[static_cast<std::size_t>(i)];
+            d = PairDesc{};
+            d.num_a = n;
+            d.num_b = n;
+            d.log2L = h->log2L;
+            d.log2n1 = h->fs.log2n1;
+            d.log2n2 = h->fs.log2n2;
+            d.col_items = col_items;
+            d.row_items = n2;
+            d.work = h->work + 2LL * L * i;
+            d.lo = h->tables;
+            d.hi = h->tables + kTwiddleSize;
+            prefix[static_cast<std::size_t>(i) + 1] = prefix[static_cast<std::size_t>(i)] + col_items;
+        }
+        h->col_total = prefix.back();
+        RF_HIP_CHECK(hipMalloc(&h->dev, h->host.size() * sizeof(PairDesc)));
+        RF_HIP_CHECK(hipMemcpy(h->dev, h->host.data(), h->host.size() * sizeof(PairDesc),
+                               hipMemcpyHostToDevice));
+        RF_HIP_CHECK(hipMalloc(&h->col_prefix, prefix.size() * sizeof(int)));
+        RF_HIP_CHECK(hipMemcpy(h->col_prefix, prefix.data(), prefix.size() * sizeof(int),
+                               hipMemcpyHostToDevice));
+        h->bytes = (tsize + work + partial) * sizeof(float2)
+                   + static_cast<std::size_t>(count) * (kFeatures + kBins + 2) * sizeof(float)
+                   + h->host.size() * sizeof(PairDesc) + prefix.size() * sizeof(int);
+    } catch (...) {
+        destroy_search(h);
+        throw;
+    }
+    return h;
+}
+
+}  // namespace
+}  // namespace rf
+
+using rf::guarded;
+
+extern "C" {
+
+rf_status rf_delay_search_create(int count, int n, rf_delay_search** out)
+{
+    return guarded([&] {
+        if (out == nullptr) rf::fail(RF_INVALID_ARG, "delay search: NULL output handle");
+        *out = nullptr;
+        *out = rf::create_search(count, n);
+    });
+}
+
+rf_status rf_delay_search_destroy(rf_delay_search* h)
+{
+    return guarded([&] { rf::destroy_search(h); });
+}
+
+rf_status rf_delay_search_bytes(const rf_delay_search* h, size_t* bytes)
+{
+    return guarded([&] {
+        if (h == nullptr || bytes == nullptr) rf::fail(RF_INVALID_ARG, "delay search: NULL");
+        *bytes = h->bytes;
+    });
+}
+
+rf_status rf_delay_search_features(rf_delay_search* h, const rf_cf32* a, const rf_cf32* b,
+                                   float* features, int* coarse, rf_stream stream)
+{
+    return guarded([&] {
+        if (h == nullptr || a == nullptr || b == nullptr) {
+            rf::fail(RF_INVALID_ARG, "delay search: NULL handle or input");
+        }
+        hipStream_t s = rf::as_stream(stream);
+        rf::features_launch(*h, reinterpret_cast<const float2*>(a),
+                            reinterpret_cast<const float2*>(b), s);
+        if (features != nullptr) {
+            RF_HIP_CHECK(hipMemcpyAsync(features, h->features,
+                                        static_cast<std::size_t>(h->count) * rf::kFeatures * sizeof(float),
+                                        hipMemcpyDeviceToDevice, s));
+        }
+        if (coarse != nullptr) {
+            RF_HIP_CHECK(hipMemcpyAsync(coarse, h->coarse, static_cast<std::size_t>(h->count) * sizeof(int),
+                                        hipMemcpyDeviceToDevice, s));
+        }
+    });
+}
+
+rf_status rf_delay_search_run(rf_delay_search* h, const rf_cf32* a, const rf_cf32* b,
+                              rf_delay_mlp* mlp, float threshold, rf_delay_result* results,
+                              rf_stream stream)
+{
+    return guarded([&] {
+        if (h == nullptr || a == nullptr || b == nullptr || results == nullptr) {
+            rf::fail(RF_INVALID_ARG, "delay search: NULL handle, input or results");
+        }
+        rf::search_launch(*h, reinterpret_cast<const float2*>(a),
+                          reinterpret_cast<const float2*>(b), mlp, threshold, results,
+                          rf::as_stream(stream));
+    });
+}
+
+rf_status rf_delay_mlp_train(rf_delay_mlp* mlp, unsigned long long seed, int pairs, int batch,
+                             int n, float learning_rate, float* losses, rf_stream stream)
+{
+    return guarded([&] {
+        if (mlp == nullptr) rf::fail(RF_INVALID_ARG, "delay mlp: NULL handle");
+        if (pairs < 1 || batch < 1 || batch > rf::delay_mlp_max_batch(*mlp)) {
+            rf::fail(RF_INVALID_ARG, "delay mlp train: need pairs >= 1 and 1 <= batch <= max_batch");
+        }
+        if (!(learning_rate > 0.0f)) rf::fail(RF_INVALID_ARG, "delay mlp: learning rate <= 0");
+        hipStream_t s = rf::as_stream(stream);
+        rf_delay_search* search = rf::create_search(batch, n);
+        const int steps = static_cast<int>(rf::ceil_div(pairs, batch));
+        float2* a = nullptr;
+        float2* b = nullptr;
+        float* delays = nullptr;
+        int* labels = nullptr;
+        float* loss = nullptr;
+        auto release = [&] {
+            (void)hipFree(a);
+            (void)hipFree(b);
+            (void)hipFree(delays);
+            (void)hipFree(labels);
+            (void)hipFree(loss);
+            rf::destroy_search(search);
+        };
+        try {
+            const std::size_t samples = static_cast<std::size_t>(batch) * n;
+            RF_HIP_CHECK(hipMalloc(&a, samples * sizeof(float2)));
+            RF_HIP_CHECK(hipMalloc(&b, samples * sizeof(float2)));
+            RF_HIP_CHECK(hipMalloc(&delays, static_cast<std::size_t>(batch) * sizeof(float)));
+            RF_HIP_CHECK(hipMalloc(&labels, static_cast<std::size_t>(batch) * sizeof(int)));
+            RF_HIP_CHECK(hipMalloc(&loss, static_cast<std::size_t>(steps) * sizeof(float)));
+            for (int step = 0; step < steps; ++step) {
+                const int rows = std::min(batch, pairs - step * batch);
+                rf::synth_pairs(seed, step * batch, batch, n, a, b, delays, s);
+                rf::features_launch(*search, a, b, s);
+                rf::search_labels_kernel<<<static_cast<unsigned>(rf::ceil_div(batch, 256)), 256, 0,
+                                           s>>>(delays, batch, labels);
+                rf::check_launch("search_labels_kernel");
+                rf::delay_mlp_train_step(*mlp, search->features, labels, rows, learning_rate,
+                                         loss + step, s);
+            }
+            if (losses != nullptr) {
+                RF_HIP_CHECK(hipMemcpyAsync(losses, loss, static_cast<std::size_t>(steps) * sizeof(float),
+                                            hipMemcpyDeviceToHost, s));
+            }
+            RF_HIP_CHECK(hipStreamSynchronize(s));
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            release();
+            throw;
+        }
+        release();
+    });
+}
+
+}  // extern "C"

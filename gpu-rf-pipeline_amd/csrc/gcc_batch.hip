@@ -21,36 +21,15 @@
 // run their row pass as one 4096-point row per item on 256 lanes (register-resident radix-16
 // transforms); shorter rows use the generic LDS transform, F rows per item.
 
-#include "fft4096.hpp"
-#include "fft_plan.hpp"
+#include "gcc_batch_kernels.hpp"
 #include "twiddle.hpp"
-#include "xcorr_common.hpp"
 
 #include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
 
-namespace {
-
-struct PairDesc {
-    const float2* a;
-    const float2* b;
-    float2* r;        // optional full output (num_a + num_b - 1 values)
-    float2* work;     // 2 L values
-    const float2* lo;
-    const float2* hi;
-    int num_a;
-    int num_b;
-    int log2L;
-    int log2n1;
-    int log2n2;
-    int col_items;    // n1 / C
-    int row_items;    // n2 / F
-    int pad;
-};
-
-}  // namespace
+using rf::PairDesc;
 
 struct rf_gcc_batch {
     int count = 0;
@@ -80,84 +59,6 @@ struct rf_gcc_batch {
 
 namespace rf {
 namespace {
-
-constexpr int kItemThreads = 512;
-
-__device__ __forceinline__ int find_pair(const int* __restrict__ prefix, int count, int item)
-{
-    int lo = 0;
-    int hi = count - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (prefix[mid] <= item) {
-            lo = mid;
-        } else {
-            hi = mid - 1;
-        }
-    }
-    return lo;
-}
-
-__device__ __forceinline__ FourStep shape_of(const PairDesc& d)
-{
-    FourStep fs;
-    fs.log2L = d.log2L;
-    fs.log2n1 = d.log2n1;
-    fs.log2n2 = d.log2n2;
-    fs.lo = d.lo;
-    fs.hi = d.hi;
-    return fs;
-}
-
-__global__ void batch_init_kernel(rf_peak_result* results, int count)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) {
-        *reinterpret_cast<unsigned long long*>(&results[i]) = 0ULL;
-        results[i].frac = 0.0;
-    }
-}
-
-__global__ __launch_bounds__(kItemThreads) void batch_column_fwd_kernel(
-    const PairDesc* __restrict__ pairs, const int* __restrict__ prefix, int count, int total,
-    const float2* __restrict__ table)
-{
-    extern __shared__ float2 lds[];
-    xcd_major_items(2 * total, [&](int item) {
-        const int which = item >= total ? 1 : 0;
-        const int local = item - which * total;
-        const int p = find_pair(prefix, count, local);
-        const PairDesc d = pairs[p];
-        const FourStep fs = shape_of(d);
-        const long long L = 1LL << d.log2L;
-        const int log2n1 = d.log2n1;
-        const long long c0 = static_cast<long long>(local - prefix[p]) * column_tile(fs);
-        float2* dst = d.work + (which ? L : 0);
-        auto store = [=](long long j1, int k2, float2 v) {
-            dst[j1 + (static_cast<long long>(k2) << log2n1)] = v;
-        };
-        if (which == 0) {
-            const float2* a = d.a;
-            const int na = d.num_a;
-            column_pass_block<false>(
-                fs, c0,
-                [=](long long j1, int j2) {
-                    return rotated_a(a, na, L, j1 + (static_cast<long long>(j2) << log2n1));
-                },
-                store, lds, table);
-        } else {
-            const float2* b = d.b;
-            const int nb = d.num_b;
-            column_pass_block<false>(
-                fs, c0,
-                [=](long long j1, int j2) {
-                    return padded_b(b, nb, j1 + (static_cast<long long>(j2) << log2n1));
-                },
-                store, lds, table);
-        }
-        __syncthreads();  // the next item reuses the LDS
-    });
-}
 
 __global__ __launch_bounds__(kItemThreads) void batch_row_kernel(
     const PairDesc* __restrict__ pairs, const int* __restrict__ prefix,
@@ -227,127 +128,6 @@ __global__ __launch_bounds__(k4096Lanes) void batch_row4096_kernel(
     });
 }
 
-__global__ __launch_bounds__(kItemThreads) void batch_column_inv_peak_kernel(
-    const PairDesc* __restrict__ pairs, const int* __restrict__ prefix, int count, int total,
-    rf_peak_result* results, const float2* __restrict__ table)
-{
-    extern __shared__ float2 lds[];
-    __shared__ unsigned long long wave_best[kItemThreads / 64];
-    xcd_major_items(total, [&](int item) {
-        const int p = find_pair(prefix, count, item);
-        const PairDesc d = pairs[p];
-        const FourStep fs = shape_of(d);
-        const long long L = 1LL << d.log2L;
-        const int log2n1 = d.log2n1;
-        const int n2 = 1 << d.log2n2;
-        const int C = column_tile(fs);
-        const int slots = lds_slots(n2);
-        const long long c0 = static_cast<long long>(item - prefix[p]) * C;
-        const float2* src = d.work + L;
-        const int total_e = C * n2;
-        for (int e = threadIdx.x; e < total_e; e += blockDim.x) {
-            const int row = e / C;
-            const int col = e - row * C;
-            lds[col * slots + lds_pad(row)] = src[(c0 + col) + (static_cast<long long>(row) << log2n1)];
-        }
-        __syncthreads();
-        const int per = n2 >> 3;
-        const int f = threadIdx.x / per;
-        const int t = threadIdx.x - f * per;
-        lds_fft<true>(lds + f * slots, d.log2n2, t, true, table);
-        const long long lags = static_cast<long long>(d.num_a) + d.num_b - 1;
-        unsigned long long best = 0ULL;
-        for (int e = threadIdx.x; e < total_e; e += blockDim.x) {
-            const int j2 = e / C;
-            const int col = e - j2 * C;
-            const long long n = (c0 + col) + (static_cast<long long>(j2) << log2n1);
-            if (n < lags) {
-                const float2 v = lds[col * slots + lds_pad(j2)];
-                const unsigned long long key = peak_key(hypotf(v.x, v.y), n);
-                best = key > best ? key : best;
-                if (d.r != nullptr) {
-                    d.r[n] = v;
-                }
-            }
-        }
-        for (int off = 32; off > 0; off >>= 1) {
-            const unsigned long long other = __shfl_xor(best, off, 64);
-            best = other > best ? other : best;
-        }
-        if ((threadIdx.x & 63) == 0) {
-            wave_best[threadIdx.x >> 6] = best;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int w = 1; w < kItemThreads / 64; ++w) {
-                best = wave_best[w] > best ? wave_best[w] : best;
-            }
-            atomicMax(reinterpret_cast<unsigned long long*>(&results[p]), best);
-        }
-        __syncthreads();
-    });
-}
-
-// r[n] from column j1 of the pass-2 output: sum_k2 Z[j1 + n1 k2] exp(+2 pi i j2 k2 / n2),
-// one wave, float32 terms summed in float64.
-__device__ double lag_magnitude(const PairDesc& d, const FourStep& fs, long long n)
-{
-    const long long L = 1LL << d.log2L;
-    const int n1 = 1 << d.log2n1;
-    const int n2 = 1 << d.log2n2;
-    const long long j1 = n & (n1 - 1);
-    const long long j2 = n >> d.log2n1;
-    const float2* col = d.work + L + j1;
-    double re = 0.0;
-    double im = 0.0;
-    for (int k2 = threadIdx.x; k2 < n2; k2 += 64) {
-        const long long m = ((j2 * k2) & (n2 - 1)) * n1;
-        const float2 w = four_step_twiddle<true>(fs, m);
-        const float2 z = col[static_cast<long long>(k2) * n1];
-        re += static_cast<double>(z.x) * w.x - static_cast<double>(z.y) * w.y;
-        im += static_cast<double>(z.x) * w.y + static_cast<double>(z.y) * w.x;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        re += __shfl_xor(re, off, 64);
-        im += __shfl_xor(im, off, 64);
-    }
-    return sqrt(re * re + im * im);
-}
-
-__global__ __launch_bounds__(64) void batch_finish_kernel(const PairDesc* __restrict__ pairs,
-                                                          rf_peak_result* results)
-{
-    const PairDesc d = pairs[blockIdx.x];
-    const FourStep fs = shape_of(d);
-    rf_peak_result* res = &results[blockIdx.x];
-    const unsigned long long key = *reinterpret_cast<const unsigned long long*>(res);
-    const long long lags = static_cast<long long>(d.num_a) + d.num_b - 1;
-    const int idx = static_cast<int>(0xFFFFFFFFu - static_cast<unsigned>(key & 0xFFFFFFFFULL));
-    const float mag = __uint_as_float(static_cast<unsigned>(key >> 32));
-    double frac = 0.0;
-    if (idx > 0 && idx + 1 < lags) {
-        const double ml = lag_magnitude(d, fs, idx - 1);
-        const double mh = lag_magnitude(d, fs, idx + 1);
-        const double m0 = static_cast<double>(mag);
-        if (m0 > 0.0) {
-            frac = mh >= ml ? mh / (m0 + mh) : -ml / (m0 + ml);
-            frac = frac > 0.5 ? 0.5 : (frac < -0.5 ? -0.5 : frac);
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        res->index = idx;
-        res->magnitude = mag;
-        res->frac = frac;
-    }
-}
-
-std::size_t column_lds_bytes()
-{
-    // C * lds_slots(n2) with C * n2 = 4096: at most 4096 + 512 + C values, C <= 512.
-    return static_cast<std::size_t>(4096 + 512 + 512) * sizeof(float2);
-}
-
 std::size_t row_lds_bytes()
 {
     // 2 F lds_slots(n1) with F n1 = 4096: at most 2 (4096 + 512 + F), F <= 4.
@@ -384,9 +164,10 @@ void batch_run(rf_gcc_batch& h, rf_peak_result* results, hipStream_t stream)
     }
     const unsigned g3 = grid_for(h.col_total);
     batch_column_inv_peak_kernel<<<g3, kItemThreads, column_lds_bytes(), stream>>>(
-        h.dev, h.col_prefix, h.count, h.col_total, results, table);
+        h.dev, h.col_prefix, h.count, h.col_total, results, nullptr, table);
     check_launch("batch_column_inv_peak_kernel");
-    batch_finish_kernel<<<static_cast<unsigned>(h.count), 64, 0, stream>>>(h.dev, results);
+    batch_finish_kernel<<<static_cast<unsigned>(h.count), 64, 0, stream>>>(h.dev, results,
+                                                                         nullptr);
     check_launch("batch_finish_kernel");
 }
 

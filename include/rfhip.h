@@ -235,6 +235,83 @@ rf_status rf_gcc_batch_launch_graph(rf_gcc_batch* handle, rf_peak_result* result
 rf_status rf_gcc_batch_tune(rf_gcc_batch* handle, rf_peak_result* results, rf_stream stream,
                             int* workgroups_per_cu);
 
+/* ---------------------------------------------------------------------------------------
+ * Learned delay search. No counterpart in the reference: a batched GCC-PHAT (the work of
+ * gcc_phat_cuda, src/xcorr.cu:379-399, plus the peak loop of rfgpu doa, src/main.cpp:390-401)
+ * over `count` equal-length pairs in which a small network proposes the delay bin from cheap
+ * features of the PHAT cross-spectrum; where it is confident, the GCC-PHAT output is evaluated
+ * exactly over a 64-lag window around the proposal instead of by the full inverse transform.
+ * Delays are searched in [-512, 512) samples; 2^15 <= L <= 2^20 (8193 <= n <= 524288).
+ * ------------------------------------------------------------------------------------- */
+#define RF_DELAY_FEATURES 256 /* network input: pooled coarse |GCC-PHAT| over lags -512..511 */
+#define RF_DELAY_BINS 64      /* network output: bins of 16 lags, bin b = lags -512+16b.. */
+#define RF_DELAY_SPAN 512
+#define RF_DELAY_MLP_PARAMS 148032
+
+/* Synthetic capture pairs (the training and evaluation distribution: delay U[-512, 512),
+ * SNR U[-10, 20] dB, AR(1) noise with coefficient U[0, 0.9] per channel, 65-tap windowed-sinc
+ * fractional delay), a pure function of (seed, pair index) by Philox4x32-10. a, b: DEVICE,
+ * count * n values (pair k at k * n); delays: DEVICE, count floats (true delay of b against a,
+ * in samples), or NULL. Pairs first_pair .. first_pair + count - 1 of the seed. */
+rf_status rf_synth_pairs(unsigned long long seed, int first_pair, int count, int n, rf_cf32* a,
+                         rf_cf32* b, float* delays, rf_stream stream);
+
+/* Delay-proposal network 256 -> 256 (GELU) -> 256 (GELU) -> 64 (softmax). FP32 master
+ * parameters (layout W1[256][256], b1[256], W2[256][256], b2[256], W3[64][256], b3[64]); GEMM
+ * operands rounded to BF16 on the matrix cores, FP32 accumulate. Workspaces for `max_batch`
+ * rows. Weights He-uniform from Philox4x32-10 of `seed`, biases 0. */
+typedef struct rf_delay_mlp rf_delay_mlp;
+rf_status rf_delay_mlp_create(int max_batch, unsigned long long seed, rf_delay_mlp** handle);
+rf_status rf_delay_mlp_destroy(rf_delay_mlp* handle);
+/* DEVICE pointer to the RF_DELAY_MLP_PARAMS master parameters (read or overwrite them). */
+rf_status rf_delay_mlp_params(rf_delay_mlp* handle, float** params, size_t* count);
+/* probs: DEVICE batch x 64 softmax probabilities of DEVICE features (batch x 256). */
+rf_status rf_delay_mlp_forward(rf_delay_mlp* handle, const float* features, int batch,
+                               float* probs, rf_stream stream);
+/* Gradients of the mean cross-entropy against DEVICE int labels (0..63): grads (DEVICE,
+ * RF_DELAY_MLP_PARAMS floats, or NULL) and loss (DEVICE, 1 float). Parameters unchanged. */
+rf_status rf_delay_mlp_gradients(rf_delay_mlp* handle, const float* features, const int* labels,
+                                 int batch, float* grads, float* loss, rf_stream stream);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected) on the batch. */
+rf_status rf_delay_mlp_train_step(rf_delay_mlp* handle, const float* features,
+                                  const int* labels, int batch, float learning_rate,
+                                  float* loss, rf_stream stream);
+/* Training driver: ceil(pairs / batch) steps, each synthesising `batch` fresh pairs of n
+ * samples on the GPU (rf_synth_pairs of `seed`, pairs step * batch ..), computing their
+ * features and taking one Adam step towards the bin of the rounded true delay. losses: HOST,
+ * one mean loss per step, or NULL. Synchronises `stream`. */
+rf_status rf_delay_mlp_train(rf_delay_mlp* handle, unsigned long long seed, int pairs,
+                             int batch, int n, float learning_rate, float* losses,
+                             rf_stream stream);
+
+typedef struct rf_delay_search rf_delay_search;
+/* First 16 bytes as rf_peak_result: index = output index of the peak (lag = index - (n - 1)),
+ * magnitude = |r| there, frac = sub-sample offset (f = m1 / (m0 + m1) towards the larger
+ * neighbour m1). proposal = proposed bin, confidence = its probability (1 for the coarse
+ * proposal), path = 0 window refine, 1 full inverse transform. */
+typedef struct rf_delay_result {
+    int32_t index;
+    float magnitude;
+    double frac;
+    int32_t proposal;
+    float confidence;
+    int32_t path;
+    int32_t reserved;
+} rf_delay_result;
+rf_status rf_delay_search_create(int count, int n, rf_delay_search** handle);
+rf_status rf_delay_search_destroy(rf_delay_search* handle);
+rf_status rf_delay_search_bytes(const rf_delay_search* handle, size_t* bytes);
+/* Forward transforms, PHAT weighting and features of DEVICE inputs a, b (count * n each);
+ * features (DEVICE count x 256) and coarse proposals (DEVICE count ints) may be NULL. */
+rf_status rf_delay_search_features(rf_delay_search* handle, const rf_cf32* a, const rf_cf32* b,
+                                   float* features, int* coarse, rf_stream stream);
+/* The search. mlp NULL: the coarse proposal (largest coarse |GCC-PHAT|) for every pair.
+ * Pairs whose confidence is below `threshold` (threshold > 1: every pair) take the full
+ * inverse transform. results: DEVICE, count records. */
+rf_status rf_delay_search_run(rf_delay_search* handle, const rf_cf32* a, const rf_cf32* b,
+                              rf_delay_mlp* mlp, float threshold, rf_delay_result* results,
+                              rf_stream stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -109,3 +109,16 @@ def test_fft_plan_arguments() -> None:
 
 def test_peak_arguments() -> None:
     assert rf.lib().rf_peak(8, 0, 8, None) == rf.RF_INVALID_ARG
+
+
+def test_delay_search_arguments() -> None:
+    L = rf.lib()
+    h = ctypes.c_void_p()
+    assert L.rf_delay_search_create(4, 4000, ctypes.byref(h)) == rf.RF_LIMIT      # L = 2^13
+    assert L.rf_delay_search_create(4, 600000, ctypes.byref(h)) == rf.RF_LIMIT    # L = 2^21
+    assert L.rf_delay_search_create(0, 16384, ctypes.byref(h)) == rf.RF_INVALID_ARG
+    assert L.rf_delay_mlp_create(0, 1, ctypes.byref(h)) == rf.RF_INVALID_ARG
+    assert L.rf_synth_pairs(1, 0, 0, 16384, 8, 8, None, None) == rf.RF_INVALID_ARG
+    assert L.rf_synth_pairs(1, 0, 4, 16384, None, 8, None, None) == rf.RF_INVALID_ARG
+    assert L.rf_delay_search_run(None, 8, 8, None, 0.9, 8, None) == rf.RF_INVALID_ARG
+    assert L.rf_delay_mlp_train(None, 1, 16, 16, 16384, 1e-3, None, None) == rf.RF_INVALID_ARG
