@@ -266,6 +266,14 @@ def bench_gcc_batch(rf, args) -> dict:
     h.bind([x.ptr for x in d_a], [x.ptr for x in d_b])
     res = rf.DeviceBuffer(16 * count, np.uint8)
     tuned = h.tune(res.ptr, stream.handle)
+    group_bytes, groups = h.group_bytes()
+    sweep = {}
+    for budget in (0, 48 << 20, 96 << 20, 160 << 20):
+        h.set_group_bytes(budget)
+        ms = time_steps(rf, lambda i: h.run(res.ptr, stream.handle), max(2, args.steps // 2), 1,
+                        stream.handle) / max(2, args.steps // 2)
+        sweep[f"{budget >> 20}MiB/{h.group_bytes()[1]}groups"] = round(ms, 4)
+    h.set_group_bytes(group_bytes)
     ms_eager = time_steps(rf, lambda i: h.run(res.ptr, stream.handle), args.steps, args.warmup,
                           stream.handle) / args.steps
     ms_graph = time_steps(rf, lambda i: h.run(res.ptr, stream.handle, graph=True), args.steps,
@@ -321,7 +329,9 @@ def bench_gcc_batch(rf, args) -> dict:
                      "peak_tflops": FP32_PEAK_TFLOPS,
                      "frac": round(flop_step / (best * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)},
         "batched": {"eager_ms": round(ms_eager, 4), "graph_ms": round(ms_graph, 4),
-                    "tuned_workgroups_per_cu": tuned},
+                    "tuned_workgroups_per_cu": tuned,
+                    "tuned_group_bytes": group_bytes, "groups": groups,
+                    "group_budget_sweep_ms": sweep},
         "plain_per_pair": {"ms": round(ms_plain, 4), "pairs_per_s": round(count / (ms_plain * 1e-3), 1),
                            "speedup_of_batched": round(ms_plain / best, 2),
                            "same_peak_index": same},

@@ -125,6 +125,8 @@ _proto("rf_gcc_batch_bind", _st, _vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
 _proto("rf_gcc_batch_run", _st, _vp, _vp, _vp)
 _proto("rf_gcc_batch_launch_graph", _st, _vp, _vp, _vp)
 _proto("rf_gcc_batch_tune", _st, _vp, _vp, _vp, ctypes.POINTER(_int))
+_proto("rf_gcc_batch_set_group_bytes", _st, _vp, _size)
+_proto("rf_gcc_batch_group_bytes", _st, _vp, ctypes.POINTER(_size), ctypes.POINTER(_int))
 _ull = ctypes.c_ulonglong
 _fp = ctypes.c_float
 _proto("rf_synth_pairs", _st, _ull, _int, _int, _int, _vp, _vp, _vp, _vp)
@@ -661,6 +663,16 @@ class GccBatch:
         best = _int(0)
         check(_lib.rf_gcc_batch_tune(self.handle, results, _sp(stream), ctypes.byref(best)))
         return best.value
+
+    def set_group_bytes(self, nbytes: int) -> None:
+        check(_lib.rf_gcc_batch_set_group_bytes(self.handle, nbytes))
+
+    def group_bytes(self) -> tuple[int, int]:
+        """(budget in bytes, number of groups) of the current schedule."""
+        n = _size(0)
+        g = _int(0)
+        check(_lib.rf_gcc_batch_group_bytes(self.handle, ctypes.byref(n), ctypes.byref(g)))
+        return n.value, g.value
 
     def bytes(self) -> int:
         n = _size(0)

@@ -18,9 +18,10 @@
 //   4 proposal: the network's most probable of 64 bins of 16 lags (bin b: lags -512 + 16b ..
 //     -512 + 16b + 15) and its probability, or the coarse proposal with confidence 1;
 //   5 refine (confidence >= threshold): r at the 64 lags -512 + 16b - 24 .. -512 + 16b + 39
-//     as a pruned inverse DFT of W, r(n0 + m) = sum_p w^(pm) sum_q V[p + Rq] exp(2 pi i qm / 64)
-//     with V[k] = W[k] w^(k n0), w = exp(2 pi i / L), R = L / 64; the peak is the first largest
-//     |r| in the window; at either window edge the pair falls back;
+//     as a pruned inverse DFT of W, r(n) = sum_p w^(p n) sum_q W[p + R q] exp(2 pi i q n / 64),
+//     w = exp(2 pi i / L), R = L / 64 (64-point transforms of W's rows seen as 64 x 64
+//     matrices); the peak is the first largest |r| in the window; at either window edge the
+//     pair falls back;
 //   6 fallback: inverse row and column passes with the fused peak search of rf_gcc_batch.
 // Peaks and the sub-sample offset follow rf_gcc_batch (first largest |r|; f = m1 / (m0 + m1)
 // towards the larger neighbour).
@@ -62,6 +63,7 @@ struct rf_delay_search {
     int* coarse = nullptr;      // count
     float* probs = nullptr;     // count * 64
     int* active = nullptr;      // count: 1 = full inverse transform
+    int* fallback = nullptr;    // count + 1: number of fallback pairs, then their indices
     std::size_t bytes = 0;
 };
 
@@ -75,7 +77,7 @@ constexpr int kSpan = RF_DELAY_SPAN;
 constexpr int kWindow = 64;
 constexpr int kWindowMargin = 24;
 constexpr int kRefineThreads = 256;
-constexpr int kRefineBatch = kRefineThreads / 8;  // 64-point transforms per LDS round
+constexpr int kRefineBatch = kRefineThreads / 8;  // 64-point transforms per lds_fft call
 
 __global__ void search_bind_kernel(PairDesc* __restrict__ pairs, int count, const float2* a,
                                    const float2* b, int n)
@@ -194,7 +196,8 @@ __global__ __launch_bounds__(256) void search_features_kernel(
 // Proposal and path per pair.
 __global__ void search_decide_kernel(const float* __restrict__ probs,
                                      const int* __restrict__ coarse, int count, float threshold,
-                                     rf_delay_result* __restrict__ results, int* __restrict__ active)
+                                     rf_delay_result* __restrict__ results, int* __restrict__ active,
+                                     int* __restrict__ fallback)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= count) {
@@ -218,15 +221,24 @@ __global__ void search_decide_kernel(const float* __restrict__ probs,
     results[p].confidence = conf;
     results[p].path = path;
     active[p] = path;
+    if (path != 0) {
+        fallback[1 + atomicAdd(&fallback[0], 1)] = p;
+    }
 }
 
 // Exact GCC-PHAT over the 64-lag window of a refined pair; one workgroup per pair.
+// With k = p + R q (p < R = L / 64, q < 64), w^(R q n) = exp(2 pi i q n / 64), so
+//   r(n) = sum_k W[k] w^(k n) = sum_p w^(p n) U_p[n mod 64],  U_p[j] = sum_q W[p + R q] e^(2 pi i q j / 64).
+// W sits in four-step order (bin k at (k mod n2) 4096 + k / n2) and R is a multiple of n2, so
+// the residues p = n2 s + c (s < 64) of row c of W are its 4096 values viewed as a 64 x 64
+// matrix [q][s]: each row is read once, contiguously, transformed as 64 columns of 64 points in
+// LDS, and every lane accumulates one window lag over 16 of the columns.
 __global__ __launch_bounds__(kRefineThreads) void search_refine_kernel(
     const PairDesc* __restrict__ pairs, rf_delay_result* __restrict__ results,
-    int* __restrict__ active, const float2* __restrict__ table)
+    int* __restrict__ active, int* __restrict__ fallback, const float2* __restrict__ table)
 {
-    constexpr int slots = lds_slots(kWindow);
-    __shared__ float2 lds[kRefineBatch * slots];
+    constexpr int slots = column_slots(kWindow);  // odd stride between the 64 columns
+    __shared__ float2 lds[kWindow * slots];
     __shared__ float2 part[4][kWindow];
     __shared__ float mag[kWindow];
     const int p = blockIdx.x;
@@ -238,30 +250,31 @@ __global__ __launch_bounds__(kRefineThreads) void search_refine_kernel(
     const FourStep fs = shape_of(d);
     const long long L = 1LL << d.log2L;
     const int log2n2 = d.log2n2;
-    const int n2mask = (1 << log2n2) - 1;
+    const int n2 = 1 << log2n2;
     const float2* W = d.work + L;
     const int bin = results[p].proposal;
     const long long lo = -kSpan + 16LL * bin - kWindowMargin;   // first lag of the window
     const long long n0 = (lo + d.num_a - 1 + L) & (L - 1);       // its circular index
-    const int R = static_cast<int>(L >> 6);
     const int m = t & (kWindow - 1);
-    const int g = t >> 6;
+    const int g = t >> 6;                                         // columns 16 g .. 16 g + 15
+    const long long n = (n0 + m) & (L - 1);
+    const int j = static_cast<int>(n & (kWindow - 1));
+    const float2 step = four_step_twiddle<true>(fs, (n << log2n2) & (L - 1));  // w^(n2 n)
     float2 acc = make_float2(0.0f, 0.0f);
-    for (int p0 = 0; p0 < R; p0 += kRefineBatch) {
-        for (int e = t; e < kRefineBatch * kWindow; e += kRefineThreads) {
-            const int f = e >> 6;
-            const int q = e & (kWindow - 1);
-            const long long k = p0 + f + static_cast<long long>(R) * q;
-            const float2 w = W[((k & n2mask) << k4096Log2) + (k >> log2n2)];
-            lds[f * slots + lds_pad(q)] = cmul(w, four_step_twiddle<true>(fs, (k * n0) & (L - 1)));
+    for (int c = 0; c < n2; ++c) {
+        const float2* row = W + (static_cast<long long>(c) << k4096Log2);
+        for (int e = t; e < k4096; e += kRefineThreads) {
+            lds[(e & (kWindow - 1)) * slots + lds_pad(e >> 6)] = row[e];  // (q, s) = (e/64, e%64)
         }
         __syncthreads();
         lds_fft<true>(lds + (t >> 3) * slots, 6, t & 7, true, table);
-#pragma unroll
-        for (int j = 0; j < kRefineBatch / 4; ++j) {
-            const int f = g * (kRefineBatch / 4) + j;
-            const float2 u = lds[f * slots + lds_pad(m)];
-            acc = cadd(acc, cmul(u, four_step_twiddle<true>(fs, static_cast<long long>(p0 + f) * m)));
+        lds_fft<true>(lds + (kRefineBatch + (t >> 3)) * slots, 6, t & 7, true, table);
+        const long long p0 = (static_cast<long long>(16 * g) << log2n2) + c;
+        float2 w = four_step_twiddle<true>(fs, (p0 * n) & (L - 1));  // w^(p n), p = n2 s + c
+#pragma unroll 4
+        for (int s = 16 * g; s < 16 * g + 16; ++s) {
+            acc = cadd(acc, cmul(lds[s * slots + lds_pad(j)], w));
+            w = cmul(w, step);
         }
         __syncthreads();
     }
@@ -281,6 +294,7 @@ __global__ __launch_bounds__(kRefineThreads) void search_refine_kernel(
             if (best == 0 || best == kWindow - 1) {
                 active[p] = 1;           // the peak may lie outside the window
                 results[p].path = 1;
+                fallback[1 + atomicAdd(&fallback[0], 1)] = p;
             } else {
                 const double m0 = mag[best];
                 const double ml = mag[best - 1];
@@ -301,16 +315,13 @@ __global__ __launch_bounds__(kRefineThreads) void search_refine_kernel(
 // Inverse row pass of the fallback pairs: W row -> inverse 4096-point FFT -> conjugate
 // four-step twiddle, in place (the input of the shared inverse column pass).
 __global__ __launch_bounds__(k4096Lanes) void search_inverse_row_kernel(
-    const PairDesc* __restrict__ pairs, int total, const int* __restrict__ active,
+    const PairDesc* __restrict__ pairs, const int* __restrict__ fallback,
     const float2* __restrict__ stages)
 {
     __shared__ float2 buf[k4096Slots];
-    xcd_major_items(total, [&](int item) {
-        const int log2n2 = pairs[0].log2n2;
-        const int p = item >> log2n2;
-        if (active[p] == 0) {
-            return;
-        }
+    const int log2n2 = pairs[0].log2n2;
+    xcd_major_items(fallback[0] << log2n2, [&](int item) {
+        const int p = fallback[1 + (item >> log2n2)];
         const long long row = item & ((1 << log2n2) - 1);
         const PairDesc d = pairs[p];
         const FourStep fs = shape_of(d);
@@ -329,6 +340,20 @@ __global__ __launch_bounds__(k4096Lanes) void search_inverse_row_kernel(
         }
         __syncthreads();
     });
+}
+
+// The search's first kernel: clears the result records (peak keys) and the fallback list.
+__global__ void search_init_kernel(rf_delay_result* __restrict__ results, int count,
+                                   int* __restrict__ fallback)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) {
+        *reinterpret_cast<unsigned long long*>(&results[i]) = 0ULL;
+        results[i].frac = 0.0;
+    }
+    if (i == 0) {
+        fallback[0] = 0;
+    }
 }
 
 __global__ void search_labels_kernel(const float* __restrict__ delays, int count,
@@ -355,7 +380,7 @@ void features_launch(rf_delay_search& h, const float2* a, const float2* b, hipSt
         h.dev, h.count, a, b, h.n);
     check_launch("search_bind_kernel");
     batch_column_fwd_kernel<<<persistent_grid(2LL * h.col_total, 4), kItemThreads,
-                              column_lds_bytes(), stream>>>(h.dev, h.col_prefix, h.count,
+                              column_lds_bytes(), stream>>>(h.dev, h.col_prefix, h.count, 0,
                                                             h.col_total, table);
     check_launch("batch_column_fwd_kernel");
     const int rows = h.count << h.fs.log2n2;
@@ -371,9 +396,9 @@ void search_launch(rf_delay_search& h, const float2* a, const float2* b, rf_dela
                    float threshold, rf_delay_result* results, hipStream_t stream)
 {
     const float2* table = twiddle_table_4096();
-    batch_init_kernel<<<static_cast<unsigned>(ceil_div(h.count, 256)), 256, 0, stream>>>(
-        results, h.count);
-    check_launch("batch_init_kernel");
+    search_init_kernel<<<static_cast<unsigned>(ceil_div(h.count, 256)), 256, 0, stream>>>(
+        results, h.count, h.fallback);
+    check_launch("search_init_kernel");
     features_launch(h, a, b, stream);
     if (mlp != nullptr) {
         const int chunk = delay_mlp_max_batch(*mlp);
@@ -384,18 +409,22 @@ void search_launch(rf_delay_search& h, const float2* a, const float2* b, rf_dela
         }
     }
     search_decide_kernel<<<static_cast<unsigned>(ceil_div(h.count, 256)), 256, 0, stream>>>(
-        mlp != nullptr ? h.probs : nullptr, h.coarse, h.count, threshold, results, h.active);
+        mlp != nullptr ? h.probs : nullptr, h.coarse, h.count, threshold, results, h.active,
+        h.fallback);
     check_launch("search_decide_kernel");
     search_refine_kernel<<<static_cast<unsigned>(h.count), kRefineThreads, 0, stream>>>(
-        h.dev, results, h.active, table);
+        h.dev, results, h.active, h.fallback, table);
     check_launch("search_refine_kernel");
+    // Fallback pairs only (their count stays on the device): grids sized for every pair.
     const int rows = h.count << h.fs.log2n2;
     search_inverse_row_kernel<<<persistent_grid(rows, 2), k4096Lanes, 0, stream>>>(
-        h.dev, rows, h.active, fft4096_stage_table());
+        h.dev, h.fallback, fft4096_stage_table());
     check_launch("search_inverse_row_kernel");
+    const int col_items = h.col_total / h.count;
     batch_column_inv_peak_kernel<<<persistent_grid(h.col_total, 4), kItemThreads,
                                    column_lds_bytes(), stream>>>(
-        h.dev, h.col_prefix, h.count, h.col_total, results, h.active, table);
+        h.dev, h.col_prefix, h.count, 0, h.col_total, results,
+        ItemSet{h.fallback + 1, h.fallback, ilog2(col_items)}, table);
     check_launch("batch_column_inv_peak_kernel");
     batch_finish_kernel<<<static_cast<unsigned>(h.count), 64, 0, stream>>>(h.dev, results,
                                                                            h.active);
@@ -416,6 +445,7 @@ void destroy_search(rf_delay_search* h)
     (void)hipFree(h->coarse);
     (void)hipFree(h->probs);
     (void)hipFree(h->active);
+    (void)hipFree(h->fallback);
     delete h;
 }
 
@@ -438,6 +468,7 @@ rf_delay_search* create_search(int count, int n)
         h->n = n;
         h->log2L = ilog2(L);
         h->fs = make_four_step_shape(h->log2L);
+        check_column_tile(h->fs);
         (void)twiddle_table_4096();
         (void)fft4096_stage_table();
         const long long tsize = four_step_table_size(h->log2L);
@@ -452,6 +483,7 @@ rf_delay_search* create_search(int count, int n)
         RF_HIP_CHECK(hipMalloc(&h->coarse, static_cast<std::size_t>(count) * sizeof(int)));
         RF_HIP_CHECK(hipMalloc(&h->probs, static_cast<std::size_t>(count) * kBins * sizeof(float)));
         RF_HIP_CHECK(hipMalloc(&h->active, static_cast<std::size_t>(count) * sizeof(int)));
+        RF_HIP_CHECK(hipMalloc(&h->fallback, (static_cast<std::size_t>(count) + 1) * sizeof(int)));
         h->host.resize(static_cast<std::size_t>(count));
         std::vector<int> prefix(static_cast<std::size_t>(count) + 1, 0);
         const int col_items = (1 << h->fs.log2n1) / column_tile(h->fs);

@@ -160,7 +160,7 @@ void exec_dir(const rf_fft_plan& plan, const float2* in, float2* out, hipStream_
         const int C = column_tile(fs);
         const dim3 cgrid(static_cast<unsigned>((1 << fs.log2n1) / C), static_cast<unsigned>(batch));
         const std::size_t cbytes =
-            static_cast<std::size_t>(C) * lds_slots(1 << fs.log2n2) * sizeof(float2);
+            static_cast<std::size_t>(C) * column_slots(1 << fs.log2n2) * sizeof(float2);
         four_step_column_kernel<INV><<<cgrid, C * (1 << fs.log2n2) / 8, cbytes, stream>>>(
             in, plan.scratch, fs, table);
         check_launch("four_step_column_kernel");

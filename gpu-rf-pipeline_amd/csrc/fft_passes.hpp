@@ -43,11 +43,24 @@ __device__ __forceinline__ float2 four_step_twiddle(const FourStep& fs, long lon
 
 // Columns per workgroup of the column pass: C * n2 = 4096 values (512 lanes), or all n1
 // columns if there are fewer.
+__host__ __device__ __forceinline__ int column_tile_log2(const FourStep& fs)
+{
+    const int c = kTwiddleLog2 - fs.log2n2;
+    return c < fs.log2n1 ? c : fs.log2n1;
+}
+
 __host__ __device__ __forceinline__ int column_tile(const FourStep& fs)
 {
-    const int c = 1 << (kTwiddleLog2 - fs.log2n2);
-    const int n1 = 1 << fs.log2n1;
-    return c < n1 ? c : n1;
+    return 1 << column_tile_log2(fs);
+}
+
+// LDS stride between the columns of a column tile: the padded column, n2 + n2/8 + 1 slots
+// (odd for n2 >= 16, so the lanes that touch one row of consecutive columns fall on
+// different banks). The batched column kernels size their LDS for C * column_slots(n2)
+// = 4096 + 512 + C values.
+__host__ __device__ constexpr int column_slots(int n2)
+{
+    return lds_slots(n2);
 }
 
 // Column pass over one matrix: workgroup handles columns [c0, c0 + C). `load(j1, j2)` gives
@@ -58,22 +71,22 @@ __device__ __forceinline__ void column_pass_block(const FourStep& fs, long long 
                                                   const float2* __restrict__ table)
 {
     const int n2 = 1 << fs.log2n2;
-    const int C = column_tile(fs);
-    const int slots = lds_slots(n2);
+    const int lc = column_tile_log2(fs);
+    const int C = 1 << lc;
+    const int slots = column_slots(n2);
     const int total = C * n2;
     for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        const int row = e / C;
-        const int col = e - row * C;
+        const int row = e >> lc;
+        const int col = e & (C - 1);
         lds[col * slots + lds_pad(row)] = load(c0 + col, row);
     }
     __syncthreads();
-    const int per = n2 >> 3;  // lanes per column transform
-    const int f = threadIdx.x / per;
-    const int t = threadIdx.x - f * per;
+    const int f = threadIdx.x >> (fs.log2n2 - 3);  // n2 / 8 lanes per column transform
+    const int t = threadIdx.x & ((n2 >> 3) - 1);
     lds_fft<INV>(lds + (f < C ? f : 0) * slots, fs.log2n2, t, f < C, table);
     for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        const int k2 = e / C;
-        const int col = e - k2 * C;
+        const int k2 = e >> lc;
+        const int col = e & (C - 1);
         float2 v = lds[col * slots + lds_pad(k2)];
         v = cmul(v, four_step_twiddle<INV>(fs, (c0 + col) * static_cast<long long>(k2)));
         store(c0 + col, k2, v);

@@ -48,6 +48,21 @@ struct rf_gcc_batch {
     };
     RowList rows4096;
     RowList rows_generic;
+    // Host copies of the item prefixes (column items per pair, row items per list entry) and
+    // the pair index of each list entry, for cutting the batch into groups.
+    std::vector<int> col_prefix_host;
+    std::vector<int> row_prefix_host[2];
+    std::vector<int> row_pairs_host[2];
+    // The batch runs as consecutive groups of pairs whose work buffers (2 L values per pair)
+    // fit a byte budget, so that the strided column passes and the row pass of a group find
+    // their intermediate data in the 256 MB Infinity Cache instead of HBM pages. Budget 0:
+    // one group.
+    struct Group {
+        int col_begin, col_items;
+        int row_begin[2], row_items[2];
+    };
+    std::vector<Group> groups;
+    std::size_t group_budget = 0;
     float2* work = nullptr;
     std::map<int, float2*> tables;
     std::size_t bytes = 0;
@@ -62,10 +77,12 @@ namespace {
 
 __global__ __launch_bounds__(kItemThreads) void batch_row_kernel(
     const PairDesc* __restrict__ pairs, const int* __restrict__ prefix,
-    const int* __restrict__ list, int count, int total, const float2* __restrict__ table)
+    const int* __restrict__ list, int count, int begin, int total,
+    const float2* __restrict__ table)
 {
     extern __shared__ float2 lds[];
-    xcd_major_items(total, [&](int item) {
+    xcd_major_items(total, [&](int local) {
+        const int item = begin + local;
         const int k = find_pair(prefix, count, item);
         const int p = list[k];
         const PairDesc d = pairs[p];
@@ -112,10 +129,12 @@ __global__ __launch_bounds__(kItemThreads) void batch_row_kernel(
 // register-resident radix-16 transforms.
 __global__ __launch_bounds__(k4096Lanes) void batch_row4096_kernel(
     const PairDesc* __restrict__ pairs, const int* __restrict__ prefix,
-    const int* __restrict__ list, int count, int total, const float2* __restrict__ stages)
+    const int* __restrict__ list, int count, int begin, int total,
+    const float2* __restrict__ stages)
 {
     __shared__ float2 buf[k4096Slots];
-    xcd_major_items(total, [&](int item) {
+    xcd_major_items(total, [&](int local) {
+        const int item = begin + local;
         const int k = find_pair(prefix, count, item);
         const PairDesc d = pairs[list[k]];
         const FourStep fs = shape_of(d);
@@ -134,6 +153,41 @@ std::size_t row_lds_bytes()
     return static_cast<std::size_t>(2 * (4096 + 512 + 4)) * sizeof(float2);
 }
 
+constexpr std::size_t kDefaultGroupBytes = 0;  // one group: measured fastest on MI355X
+
+void make_groups(rf_gcc_batch& h, std::size_t budget)
+{
+    h.group_budget = budget;
+    h.groups.clear();
+    int p0 = 0;
+    while (p0 < h.count) {
+        int p1 = p0;
+        std::size_t bytes = 0;
+        while (p1 < h.count) {
+            const std::size_t pair_bytes =
+                (static_cast<std::size_t>(2) << h.host[static_cast<std::size_t>(p1)].log2L) * sizeof(float2);
+            if (p1 > p0 && budget != 0 && bytes + pair_bytes > budget) {
+                break;
+            }
+            bytes += pair_bytes;
+            ++p1;
+        }
+        rf_gcc_batch::Group g{};
+        g.col_begin = h.col_prefix_host[static_cast<std::size_t>(p0)];
+        g.col_items = h.col_prefix_host[static_cast<std::size_t>(p1)] - g.col_begin;
+        for (int kind = 0; kind < 2; ++kind) {
+            const std::vector<int>& pairs = h.row_pairs_host[kind];
+            const std::vector<int>& prefix = h.row_prefix_host[kind];
+            const auto lo = std::lower_bound(pairs.begin(), pairs.end(), p0) - pairs.begin();
+            const auto hi = std::lower_bound(pairs.begin(), pairs.end(), p1) - pairs.begin();
+            g.row_begin[kind] = prefix[static_cast<std::size_t>(lo)];
+            g.row_items[kind] = prefix[static_cast<std::size_t>(hi)] - g.row_begin[kind];
+        }
+        h.groups.push_back(g);
+        p0 = p1;
+    }
+}
+
 void batch_run(rf_gcc_batch& h, rf_peak_result* results, hipStream_t stream)
 {
     const float2* table = twiddle_table_4096();
@@ -147,25 +201,29 @@ void batch_run(rf_gcc_batch& h, rf_peak_result* results, hipStream_t stream)
     auto grid_for = [cap](int items) {
         return static_cast<unsigned>(ceil_div(std::min(items, cap), 8) * 8);
     };
-    batch_column_fwd_kernel<<<grid_for(2 * h.col_total), kItemThreads, column_lds_bytes(),
-                              stream>>>(h.dev, h.col_prefix, h.count, h.col_total, table);
-    check_launch("batch_column_fwd_kernel");
-    if (h.rows4096.count > 0) {
-        batch_row4096_kernel<<<grid_for(h.rows4096.total), k4096Lanes, 0, stream>>>(
-            h.dev, h.rows4096.prefix, h.rows4096.pairs, h.rows4096.count, h.rows4096.total,
-            fft4096_stage_table());
-        check_launch("batch_row4096_kernel");
+    for (const rf_gcc_batch::Group& g : h.groups) {
+        batch_column_fwd_kernel<<<grid_for(2 * g.col_items), kItemThreads, column_lds_bytes(),
+                                  stream>>>(h.dev, h.col_prefix, h.count, g.col_begin,
+                                            g.col_items, table);
+        check_launch("batch_column_fwd_kernel");
+        if (g.row_items[0] > 0) {
+            batch_row4096_kernel<<<grid_for(g.row_items[0]), k4096Lanes, 0, stream>>>(
+                h.dev, h.rows4096.prefix, h.rows4096.pairs, h.rows4096.count, g.row_begin[0],
+                g.row_items[0], fft4096_stage_table());
+            check_launch("batch_row4096_kernel");
+        }
+        if (g.row_items[1] > 0) {
+            batch_row_kernel<<<grid_for(g.row_items[1]), kItemThreads, row_lds_bytes(), stream>>>(
+                h.dev, h.rows_generic.prefix, h.rows_generic.pairs, h.rows_generic.count,
+                g.row_begin[1], g.row_items[1], table);
+            check_launch("batch_row_kernel");
+        }
+        batch_column_inv_peak_kernel<<<grid_for(g.col_items), kItemThreads, column_lds_bytes(),
+                                       stream>>>(h.dev, h.col_prefix, h.count, g.col_begin,
+                                                 g.col_items, results, ItemSet{nullptr, nullptr, 0},
+                                                 table);
+        check_launch("batch_column_inv_peak_kernel");
     }
-    if (h.rows_generic.count > 0) {
-        batch_row_kernel<<<grid_for(h.rows_generic.total), kItemThreads, row_lds_bytes(),
-                           stream>>>(h.dev, h.rows_generic.prefix, h.rows_generic.pairs,
-                                     h.rows_generic.count, h.rows_generic.total, table);
-        check_launch("batch_row_kernel");
-    }
-    const unsigned g3 = grid_for(h.col_total);
-    batch_column_inv_peak_kernel<<<g3, kItemThreads, column_lds_bytes(), stream>>>(
-        h.dev, h.col_prefix, h.count, h.col_total, results, nullptr, table);
-    check_launch("batch_column_inv_peak_kernel");
     batch_finish_kernel<<<static_cast<unsigned>(h.count), 64, 0, stream>>>(h.dev, results,
                                                                          nullptr);
     check_launch("batch_finish_kernel");
@@ -240,6 +298,7 @@ rf_status rf_gcc_batch_create(int count, const int* num_a, const int* num_b,
                 }
                 const int log2L = rf::ilog2(L);
                 const rf::FourStep fs = rf::make_four_step_shape(log2L);
+                rf::check_column_tile(fs);
                 PairDesc& d = h->host[static_cast<std::size_t>(i)];
                 d = PairDesc{};
                 d.num_a = num_a[i];
@@ -256,6 +315,12 @@ rf_status rf_gcc_batch_create(int count, const int* num_a, const int* num_b,
                 work_total += 2 * L;
             }
             h->col_total = colp.back();
+            h->col_prefix_host = colp;
+            for (int kind = 0; kind < 2; ++kind) {
+                h->row_prefix_host[kind] = row_prefix[kind];
+                h->row_pairs_host[kind] = row_pairs[kind];
+            }
+            rf::make_groups(*h, rf::kDefaultGroupBytes);
             (void)rf::twiddle_table_4096();
             (void)rf::fft4096_stage_table();
             RF_HIP_CHECK(hipMalloc(&h->work, static_cast<std::size_t>(work_total) * sizeof(float2)));
@@ -385,24 +450,31 @@ rf_status rf_gcc_batch_tune(rf_gcc_batch* h, rf_peak_result* results, rf_stream 
         hipEvent_t stop = nullptr;
         RF_HIP_CHECK(hipEventCreate(&start));
         RF_HIP_CHECK(hipEventCreate(&stop));
-        const int candidates[] = {1, 2, 3, 4, 6, 8};
+        const int grids[] = {2, 4, 8};
+        const std::size_t budgets[] = {0, std::size_t(48) << 20, std::size_t(96) << 20,
+                                       std::size_t(160) << 20};
         float best_ms = 0.0f;
         int best = h->grid_mult;
+        std::size_t best_budget = h->group_budget;
         try {
-            for (int c : candidates) {
-                h->grid_mult = c;
-                rf::batch_run(*h, results, s);  // warm-up
-                RF_HIP_CHECK(hipEventRecord(start, s));
-                for (int rep = 0; rep < 3; ++rep) {
-                    rf::batch_run(*h, results, s);
-                }
-                RF_HIP_CHECK(hipEventRecord(stop, s));
-                RF_HIP_CHECK(hipEventSynchronize(stop));
-                float ms = 0.0f;
-                RF_HIP_CHECK(hipEventElapsedTime(&ms, start, stop));
-                if (best_ms == 0.0f || ms < best_ms) {
-                    best_ms = ms;
-                    best = c;
+            for (std::size_t budget : budgets) {
+                rf::make_groups(*h, budget);
+                for (int c : grids) {
+                    h->grid_mult = c;
+                    rf::batch_run(*h, results, s);  // warm-up
+                    RF_HIP_CHECK(hipEventRecord(start, s));
+                    for (int rep = 0; rep < 3; ++rep) {
+                        rf::batch_run(*h, results, s);
+                    }
+                    RF_HIP_CHECK(hipEventRecord(stop, s));
+                    RF_HIP_CHECK(hipEventSynchronize(stop));
+                    float ms = 0.0f;
+                    RF_HIP_CHECK(hipEventElapsedTime(&ms, start, stop));
+                    if (best_ms == 0.0f || ms < best_ms) {
+                        best_ms = ms;
+                        best = c;
+                        best_budget = budget;
+                    }
                 }
             }
         } catch (...) {
@@ -412,6 +484,7 @@ rf_status rf_gcc_batch_tune(rf_gcc_batch* h, rf_peak_result* results, rf_stream 
         }
         (void)hipEventDestroy(start);
         (void)hipEventDestroy(stop);
+        rf::make_groups(*h, best_budget);
         h->grid_mult = best;
         if (h->graph != nullptr) {  // the captured graph used the old grid
             RF_HIP_CHECK(hipGraphExecDestroy(h->graph));
@@ -419,6 +492,31 @@ rf_status rf_gcc_batch_tune(rf_gcc_batch* h, rf_peak_result* results, rf_stream 
         }
         if (workgroups_per_cu != nullptr) {
             *workgroups_per_cu = best;
+        }
+    });
+}
+
+rf_status rf_gcc_batch_set_group_bytes(rf_gcc_batch* h, size_t bytes)
+{
+    return guarded([&] {
+        if (h == nullptr) rf::fail(RF_INVALID_ARG, "gcc batch: NULL");
+        rf::make_groups(*h, bytes);
+        if (h->graph != nullptr) {  // the captured graph used the old groups
+            RF_HIP_CHECK(hipGraphExecDestroy(h->graph));
+            h->graph = nullptr;
+        }
+    });
+}
+
+rf_status rf_gcc_batch_group_bytes(const rf_gcc_batch* h, size_t* bytes, int* groups)
+{
+    return guarded([&] {
+        if (h == nullptr) rf::fail(RF_INVALID_ARG, "gcc batch: NULL");
+        if (bytes != nullptr) {
+            *bytes = h->group_budget;
+        }
+        if (groups != nullptr) {
+            *groups = static_cast<int>(h->groups.size());
         }
     });
 }

@@ -9,6 +9,8 @@
 #include "fft_plan.hpp"
 #include "xcorr_common.hpp"
 
+#include <string>
+
 namespace rf {
 
 struct PairDesc {
@@ -68,14 +70,15 @@ __global__ void batch_init_kernel(R* results, int count)
     }
 }
 
+// Items begin .. begin + total - 1 of the column prefix, for a and for b.
 __global__ __launch_bounds__(kItemThreads) void batch_column_fwd_kernel(
-    const PairDesc* __restrict__ pairs, const int* __restrict__ prefix, int count, int total,
-    const float2* __restrict__ table)
+    const PairDesc* __restrict__ pairs, const int* __restrict__ prefix, int count, int begin,
+    int total, const float2* __restrict__ table)
 {
     extern __shared__ float2 lds[];
     xcd_major_items(2 * total, [&](int item) {
         const int which = item >= total ? 1 : 0;
-        const int local = item - which * total;
+        const int local = begin + item - which * total;
         const int p = find_pair(prefix, count, local);
         const PairDesc d = pairs[p];
         const FourStep fs = shape_of(d);
@@ -109,44 +112,59 @@ __global__ __launch_bounds__(kItemThreads) void batch_column_fwd_kernel(
     });
 }
 
-// `active` (NULL: every pair) skips the items of pairs whose flag is 0.
+// The items a pass visits: items begin .. begin + total - 1 of the item prefix, or, with
+// `list` set, the 2^log2_items items of each of the pairs list[0 .. *list_count - 1] (pairs of
+// one shape; the count is read on the device, so the launch needs no host round trip).
+struct ItemSet {
+    const int* list;
+    const int* list_count;
+    int log2_items;
+};
+
 template <class R>
 __global__ __launch_bounds__(kItemThreads) void batch_column_inv_peak_kernel(
-    const PairDesc* __restrict__ pairs, const int* __restrict__ prefix, int count, int total,
-    R* results, const int* __restrict__ active, const float2* __restrict__ table)
+    const PairDesc* __restrict__ pairs, const int* __restrict__ prefix, int count, int begin,
+    int total, R* results, ItemSet set, const float2* __restrict__ table)
 {
     extern __shared__ float2 lds[];
     __shared__ unsigned long long wave_best[kItemThreads / 64];
-    xcd_major_items(total, [&](int item) {
-        const int p = find_pair(prefix, count, item);
-        if (active != nullptr && active[p] == 0) {
-            return;  // the whole workgroup: p is the same for every lane
+    const int visits = set.list != nullptr ? (*set.list_count << set.log2_items) : total;
+    xcd_major_items(visits, [&](int local) {
+        int p;
+        int within;
+        if (set.list != nullptr) {
+            p = set.list[local >> set.log2_items];
+            within = local & ((1 << set.log2_items) - 1);
+        } else {
+            const int item = begin + local;
+            p = find_pair(prefix, count, item);
+            within = item - prefix[p];
         }
         const PairDesc d = pairs[p];
         const FourStep fs = shape_of(d);
         const long long L = 1LL << d.log2L;
         const int log2n1 = d.log2n1;
         const int n2 = 1 << d.log2n2;
-        const int C = column_tile(fs);
-        const int slots = lds_slots(n2);
-        const long long c0 = static_cast<long long>(item - prefix[p]) * C;
+        const int lc = column_tile_log2(fs);
+        const int C = 1 << lc;
+        const int slots = column_slots(n2);
+        const long long c0 = static_cast<long long>(within) * C;
         const float2* src = d.work + L;
         const int total_e = C * n2;
         for (int e = threadIdx.x; e < total_e; e += blockDim.x) {
-            const int row = e / C;
-            const int col = e - row * C;
+            const int row = e >> lc;
+            const int col = e & (C - 1);
             lds[col * slots + lds_pad(row)] = src[(c0 + col) + (static_cast<long long>(row) << log2n1)];
         }
         __syncthreads();
-        const int per = n2 >> 3;
-        const int f = threadIdx.x / per;
-        const int t = threadIdx.x - f * per;
+        const int f = threadIdx.x >> (d.log2n2 - 3);
+        const int t = threadIdx.x & ((n2 >> 3) - 1);
         lds_fft<true>(lds + f * slots, d.log2n2, t, true, table);
         const long long lags = static_cast<long long>(d.num_a) + d.num_b - 1;
         unsigned long long best = 0ULL;
         for (int e = threadIdx.x; e < total_e; e += blockDim.x) {
-            const int j2 = e / C;
-            const int col = e - j2 * C;
+            const int j2 = e >> lc;
+            const int col = e & (C - 1);
             const long long n = (c0 + col) + (static_cast<long long>(j2) << log2n1);
             if (n < lags) {
                 const float2 v = lds[col * slots + lds_pad(j2)];
@@ -236,8 +254,19 @@ __global__ __launch_bounds__(64) void batch_finish_kernel(const PairDesc* __rest
 
 std::size_t column_lds_bytes()
 {
-    // C * lds_slots(n2) with C * n2 = 4096: at most 4096 + 512 + C values, C <= 512.
+    // C * column_slots(n2) with C * n2 = 4096: 4096 + 512 + C values, C <= 512.
     return static_cast<std::size_t>(4096 + 512 + 512) * sizeof(float2);
+}
+
+// Host check that a pair's column tile fits the LDS the batched column kernels allocate.
+void check_column_tile(const FourStep& fs)
+{
+    const std::size_t need = static_cast<std::size_t>(column_tile(fs))
+                             * column_slots(1 << fs.log2n2) * sizeof(float2);
+    if (need > column_lds_bytes()) {
+        fail(RF_LIMIT, "column tile of " + std::to_string(need) + " bytes exceeds the "
+                           + std::to_string(column_lds_bytes()) + "-byte LDS allocation");
+    }
 }
 
 }  // namespace

@@ -289,22 +289,22 @@ __global__ __launch_bounds__(512) void xcorr_column_inv_kernel(const float2* __r
     const float2* src = work + L;
     const long long c0 = static_cast<long long>(blockIdx.x) * column_tile(fs);
     const int n2 = 1 << fs.log2n2;
-    const int C = column_tile(fs);
-    const int slots = lds_slots(n2);
+    const int lc = column_tile_log2(fs);
+    const int C = 1 << lc;
+    const int slots = column_slots(n2);
     const int total = C * n2;
     for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        const int row = e / C;
-        const int col = e - row * C;
+        const int row = e >> lc;
+        const int col = e & (C - 1);
         lds[col * slots + lds_pad(row)] = src[(c0 + col) + (static_cast<long long>(row) << log2n1)];
     }
     __syncthreads();
-    const int per = n2 >> 3;
-    const int f = threadIdx.x / per;
-    const int t = threadIdx.x - f * per;
+    const int f = threadIdx.x >> (fs.log2n2 - 3);
+    const int t = threadIdx.x & ((n2 >> 3) - 1);
     lds_fft<true>(lds + (f < C ? f : 0) * slots, fs.log2n2, t, f < C, table);
     for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        const int j2 = e / C;
-        const int col = e - j2 * C;
+        const int j2 = e >> lc;
+        const int col = e & (C - 1);
         const long long n = (c0 + col) + (static_cast<long long>(j2) << log2n1);
         if (n < lags) {
             r[n] = lds[col * slots + lds_pad(j2)];
@@ -413,7 +413,7 @@ void xcorr_fft_run(const rf_xcorr_fft& h, const float2* a, const float2* b, floa
     const FourStep& fs = h.fs;
     const int n2 = 1 << fs.log2n2;
     const int C = column_tile(fs);
-    const std::size_t cbytes = static_cast<std::size_t>(C) * lds_slots(n2) * sizeof(float2);
+    const std::size_t cbytes = static_cast<std::size_t>(C) * column_slots(n2) * sizeof(float2);
     const dim3 cgrid(static_cast<unsigned>((1 << fs.log2n1) / C), 2);
     xcorr_column_fwd_kernel<<<cgrid, C * n2 / 8, cbytes, stream>>>(a, b, h.work, h.num_a,
                                                                    h.num_b, fs, table);

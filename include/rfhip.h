@@ -229,9 +229,16 @@ rf_status rf_gcc_batch_run(rf_gcc_batch* handle, rf_peak_result* results, rf_str
  * replayed with a single launch. */
 rf_status rf_gcc_batch_launch_graph(rf_gcc_batch* handle, rf_peak_result* results,
                                     rf_stream stream);
-/* Autotuner: times the bound batch with 1, 2, 3, 4, 6 and 8 persistent workgroups per CU
- * (3 runs each after a warm-up, device events on `stream`, which it synchronises), keeps the
- * fastest for later runs and reports it. The results buffer is overwritten. */
+/* The passes run over consecutive groups of pairs whose work buffers (2 L complex values per
+ * pair) fit `bytes` (0: one group), so that a group's intermediate data can stay in the
+ * 256 MB Infinity Cache between its passes. Default 0 (measured fastest for 256 uneven
+ * pairs: more groups add launch tails). */
+rf_status rf_gcc_batch_set_group_bytes(rf_gcc_batch* handle, size_t bytes);
+rf_status rf_gcc_batch_group_bytes(const rf_gcc_batch* handle, size_t* bytes, int* groups);
+/* Autotuner: times the bound batch for 2, 4 and 8 persistent workgroups per CU and group
+ * budgets of 0, 48, 96 and 160 MiB (3 runs each after a warm-up, device events on `stream`,
+ * which it synchronises), keeps the fastest pair for later runs and reports the workgroups per
+ * CU (rf_gcc_batch_group_bytes reports the budget). The results buffer is overwritten. */
 rf_status rf_gcc_batch_tune(rf_gcc_batch* handle, rf_peak_result* results, rf_stream stream,
                             int* workgroups_per_cu);
 

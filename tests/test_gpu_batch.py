@@ -143,3 +143,21 @@ def test_short_pairs_are_refused() -> None:
     with pytest.raises(rf.RfError) as e:
         rf.GccBatch([1000, 5000], [1000, 5000])
     assert e.value.status == rf.RF_LIMIT
+
+
+def test_group_schedules_give_identical_results(batch) -> None:
+    """The pairs run in groups sized to a byte budget; every schedule gives the same bits."""
+    pairs, h, _, _, d_r, _, peaks, outs = batch
+    want = [(p.index, p.magnitude, p.frac) for p in peaks]
+    for budget, groups_at_least in ((0, 1), (1 << 20, 2), (96 << 20, 1)):
+        h.set_group_bytes(budget)
+        assert h.group_bytes()[0] == budget
+        if budget == 0:
+            assert h.group_bytes()[1] == 1
+        assert h.group_bytes()[1] >= groups_at_least
+        res = rf.DeviceBuffer(16 * len(pairs), np.uint8)
+        h.run(res.ptr)
+        got = rf.read_peaks(res, len(pairs))
+        assert [(p.index, p.magnitude, p.frac) for p in got] == want
+        for r, o in zip(d_r, outs):
+            np.testing.assert_array_equal(r.download(), o)
