@@ -54,7 +54,7 @@ struct rf_delay_search {
     rf::FourStep fs{};
     std::vector<rf::PairDesc> host;
     rf::PairDesc* dev = nullptr;
-    int* col_prefix = nullptr;
+    int2* col_map = nullptr;    // col_total items: {pair, column tile}
     int col_total = 0;
     float2* work = nullptr;     // count * 2 L
     float2* tables = nullptr;   // four-step twiddles of L
@@ -110,8 +110,8 @@ __global__ __launch_bounds__(k4096Lanes) void search_row_kernel(
         float2 ub[16];
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            ua[r] = wa[t + r * k4096Lanes];
-            ub[r] = wb[t + r * k4096Lanes];
+            ua[r] = gload(wa, t + r * k4096Lanes);
+            ub[r] = gload(wb, t + r * k4096Lanes);
         }
         fft4096<false>(buf, t, ua, stages);
         __syncthreads();
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(k4096Lanes) void search_row_kernel(
         for (int r = 0; r < 16; ++r) {
             const int k1 = t + r * k4096Lanes;
             const float2 w = weigh(ua[r], ub[r], scale, 1);
-            wb[k1] = w;
+            gstore(wb, k1, w);
             const long long k = row + (static_cast<long long>(k1) << log2n2);
             float2 v = cmul(w, four_step_twiddle<true>(fs, (k * rot) & (L - 1)));
             // band k1 / 4: lanes 4j .. 4j + 3 hold its four bins of this row
@@ -263,8 +263,15 @@ __global__ __launch_bounds__(kRefineThreads) void search_refine_kernel(
     float2 acc = make_float2(0.0f, 0.0f);
     for (int c = 0; c < n2; ++c) {
         const float2* row = W + (static_cast<long long>(c) << k4096Log2);
-        for (int e = t; e < k4096; e += kRefineThreads) {
-            lds[(e & (kWindow - 1)) * slots + lds_pad(e >> 6)] = row[e];  // (q, s) = (e/64, e%64)
+        float2 v[k4096 / kRefineThreads];
+#pragma unroll
+        for (int i = 0; i < k4096 / kRefineThreads; ++i) {
+            v[i] = gload(row, t + i * kRefineThreads);
+        }
+#pragma unroll
+        for (int i = 0; i < k4096 / kRefineThreads; ++i) {
+            const int e = t + i * kRefineThreads;                     // (q, s) = (e/64, e%64)
+            lds[(e & (kWindow - 1)) * slots + lds_pad(e >> 6)] = v[i];
         }
         __syncthreads();
         lds_fft<true>(lds + (t >> 3) * slots, 6, t & 7, true, table);
@@ -330,13 +337,13 @@ __global__ __launch_bounds__(k4096Lanes) void search_inverse_row_kernel(
         float2 u[16];
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            u[r] = wb[t + r * k4096Lanes];
+            u[r] = gload(wb, t + r * k4096Lanes);
         }
         fft4096<true>(buf, t, u, stages);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int j1 = t + r * k4096Lanes;
-            wb[j1] = cmul(u[r], four_step_twiddle<true>(fs, j1 * row));
+            gstore(wb, j1, cmul(u[r], four_step_twiddle<true>(fs, j1 * row)));
         }
         __syncthreads();
     });
@@ -380,8 +387,8 @@ void features_launch(rf_delay_search& h, const float2* a, const float2* b, hipSt
         h.dev, h.count, a, b, h.n);
     check_launch("search_bind_kernel");
     batch_column_fwd_kernel<<<persistent_grid(2LL * h.col_total, 4), kItemThreads,
-                              column_lds_bytes(), stream>>>(h.dev, h.col_prefix, h.count, 0,
-                                                            h.col_total, table);
+                              column_lds_bytes(), stream>>>(h.dev, h.col_map, 0, h.col_total,
+                                                            table);
     check_launch("batch_column_fwd_kernel");
     const int rows = h.count << h.fs.log2n2;
     search_row_kernel<<<persistent_grid(rows, 2), k4096Lanes, 0, stream>>>(
@@ -423,7 +430,7 @@ void search_launch(rf_delay_search& h, const float2* a, const float2* b, rf_dela
     const int col_items = h.col_total / h.count;
     batch_column_inv_peak_kernel<<<persistent_grid(h.col_total, 4), kItemThreads,
                                    column_lds_bytes(), stream>>>(
-        h.dev, h.col_prefix, h.count, 0, h.col_total, results,
+        h.dev, h.col_map, 0, h.col_total, results,
         ItemSet{h.fallback + 1, h.fallback, ilog2(col_items)}, table);
     check_launch("batch_column_inv_peak_kernel");
     batch_finish_kernel<<<static_cast<unsigned>(h.count), 64, 0, stream>>>(h.dev, results,
@@ -437,7 +444,7 @@ void destroy_search(rf_delay_search* h)
         return;
     }
     (void)hipFree(h->dev);
-    (void)hipFree(h->col_prefix);
+    (void)hipFree(h->col_map);
     (void)hipFree(h->work);
     (void)hipFree(h->tables);
     (void)hipFree(h->partial);
@@ -485,7 +492,7 @@ rf_delay_search* create_search(int count, int n)
         RF_HIP_CHECK(hipMalloc(&h->active, static_cast<std::size_t>(count) * sizeof(int)));
         RF_HIP_CHECK(hipMalloc(&h->fallback, (static_cast<std::size_t>(count) + 1) * sizeof(int)));
         h->host.resize(static_cast<std::size_t>(count));
-        std::vector<int> prefix(static_cast<std::size_t>(count) + 1, 0);
+        std::vector<int2> col_map;
         const int col_items = (1 << h->fs.log2n1) / column_tile(h->fs);
         for (int i = 0; i < count; ++i) {
             PairDesc& d = h->host[static_cast<std::size_t>(i)];
@@ -500,18 +507,20 @@ rf_delay_search* create_search(int count, int n)
             d.work = h->work + 2LL * L * i;
             d.lo = h->tables;
             d.hi = h->tables + kTwiddleSize;
-            prefix[static_cast<std::size_t>(i) + 1] = prefix[static_cast<std::size_t>(i)] + col_items;
+            for (int c = 0; c < col_items; ++c) {
+                col_map.push_back(make_int2(i, c));
+            }
         }
-        h->col_total = prefix.back();
+        h->col_total = static_cast<int>(col_map.size());
         RF_HIP_CHECK(hipMalloc(&h->dev, h->host.size() * sizeof(PairDesc)));
         RF_HIP_CHECK(hipMemcpy(h->dev, h->host.data(), h->host.size() * sizeof(PairDesc),
                                hipMemcpyHostToDevice));
-        RF_HIP_CHECK(hipMalloc(&h->col_prefix, prefix.size() * sizeof(int)));
-        RF_HIP_CHECK(hipMemcpy(h->col_prefix, prefix.data(), prefix.size() * sizeof(int),
+        RF_HIP_CHECK(hipMalloc(&h->col_map, col_map.size() * sizeof(int2)));
+        RF_HIP_CHECK(hipMemcpy(h->col_map, col_map.data(), col_map.size() * sizeof(int2),
                                hipMemcpyHostToDevice));
         h->bytes = (tsize + work + partial) * sizeof(float2)
                    + static_cast<std::size_t>(count) * (kFeatures + kBins + 2) * sizeof(float)
-                   + h->host.size() * sizeof(PairDesc) + prefix.size() * sizeof(int);
+                   + h->host.size() * sizeof(PairDesc) + col_map.size() * sizeof(int2);
     } catch (...) {
         destroy_search(h);
         throw;

@@ -250,7 +250,7 @@ __device__ __forceinline__ void lds_pass(float2* buf, int P, int t, int p, bool 
 // workgroup must call this (it has barriers); `t` is the thread's index within the transform
 // and `active` is false for threads that only take part in the barriers.
 template <bool INV>
-__device__ void lds_fft(float2* buf, int log2P, int t, bool active,
+__device__ __forceinline__ void lds_fft(float2* buf, int log2P, int t, bool active,
                         const float2* __restrict__ table)
 {
     const int P = 1 << log2P;

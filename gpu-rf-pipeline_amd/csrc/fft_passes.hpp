@@ -37,7 +37,7 @@ inline FourStep make_four_step_shape(int log2L)
 template <bool INV>
 __device__ __forceinline__ float2 four_step_twiddle(const FourStep& fs, long long m)
 {
-    const float2 w = cmul(fs.lo[m & (kTwiddleSize - 1)], fs.hi[m >> kTwiddleLog2]);
+    const float2 w = cmul(gload(fs.lo, m & (kTwiddleSize - 1)), gload(fs.hi, m >> kTwiddleLog2));
     return INV ? make_float2(w.x, -w.y) : w;
 }
 
@@ -64,7 +64,9 @@ __host__ __device__ constexpr int column_slots(int n2)
 }
 
 // Column pass over one matrix: workgroup handles columns [c0, c0 + C). `load(j1, j2)` gives
-// the input at n = j1 + n1 j2; `store(j1, k2, v)` takes the twiddled result. 512 lanes.
+// the input at n = j1 + n1 j2; `store(j1, k2, v)` takes the twiddled result. The workgroup has
+// C n2 / 8 lanes and every lane moves 8 elements, all 8 loads in flight before the first LDS
+// write and all 8 twiddle lookups before the first store.
 template <bool INV, class Load, class Store>
 __device__ __forceinline__ void column_pass_block(const FourStep& fs, long long c0, Load load,
                                                   Store store, float2* lds,
@@ -74,22 +76,34 @@ __device__ __forceinline__ void column_pass_block(const FourStep& fs, long long 
     const int lc = column_tile_log2(fs);
     const int C = 1 << lc;
     const int slots = column_slots(n2);
-    const int total = C * n2;
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        const int row = e >> lc;
-        const int col = e & (C - 1);
-        lds[col * slots + lds_pad(row)] = load(c0 + col, row);
+    const int lanes = (C * n2) >> 3;
+    float2 v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int e = threadIdx.x + i * lanes;
+        v[i] = load(c0 + (e & (C - 1)), e >> lc);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int e = threadIdx.x + i * lanes;
+        lds[(e & (C - 1)) * slots + lds_pad(e >> lc)] = v[i];
     }
     __syncthreads();
     const int f = threadIdx.x >> (fs.log2n2 - 3);  // n2 / 8 lanes per column transform
     const int t = threadIdx.x & ((n2 >> 3) - 1);
     lds_fft<INV>(lds + (f < C ? f : 0) * slots, fs.log2n2, t, f < C, table);
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int e = threadIdx.x + i * lanes;
         const int k2 = e >> lc;
         const int col = e & (C - 1);
-        float2 v = lds[col * slots + lds_pad(k2)];
-        v = cmul(v, four_step_twiddle<INV>(fs, (c0 + col) * static_cast<long long>(k2)));
-        store(c0 + col, k2, v);
+        v[i] = cmul(lds[col * slots + lds_pad(k2)],
+                    four_step_twiddle<INV>(fs, (c0 + col) * static_cast<long long>(k2)));
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int e = threadIdx.x + i * lanes;
+        store(c0 + (e & (C - 1)), e >> lc, v[i]);
     }
 }
 

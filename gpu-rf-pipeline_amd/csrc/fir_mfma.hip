@@ -16,9 +16,12 @@
 // back exactly. BF16 has FP32's exponent range and needs no scaling.
 //
 // LDS: the sample tile as four 16-bit arrays (re/im x hi/lo), so a lane's eight consecutive
-// samples are one 16-byte read; the taps reversed and stored as eight copies shifted by one
-// element each (hi and lo), so a lane's eight taps, whose start is not a multiple of eight, are
-// still one aligned 16-byte read from the copy with the matching shift.
+// samples are one 16-byte read. The 32 lanes of an A read start 32 samples (64 bytes) apart,
+// which would put every fourth lane on the same banks, so within each block of 32 samples the
+// four groups of 8 are rotated by (block / 4) mod 4: lanes i .. i + 15 then read 16 different
+// 16-byte bank groups, and no space is added. The taps are reversed and stored as eight copies shifted by one element each (hi and
+// lo), so a lane's eight taps, whose start is not a multiple of eight, are still one aligned
+// 16-byte read from the copy with the matching shift.
 
 #include "fir_internal.hpp"
 
@@ -37,6 +40,19 @@ constexpr int kMfmaOutputs = 4 * 1024;           // per workgroup
 __host__ __device__ constexpr int mfma_kp(int num_taps)
 {
     return ((num_taps + 31 + 15) / 16) * 16;
+}
+
+// LDS slot of tile sample u = 32 b + 8 g + e: group g moves to (g + b / 4) mod 4 within block
+// b. A bijection on every block of 32; an 8-aligned group of 8 samples stays contiguous and
+// 16-byte aligned.
+__host__ __device__ constexpr int tile_slot(int u)
+{
+    return (u & ~31) | ((((u >> 3) + (u >> 7)) & 3) << 3) | (u & 7);
+}
+
+__host__ __device__ constexpr int tile_slots(int samples)
+{
+    return (samples + 31) & ~31;
 }
 
 // Length of one shifted tap copy: indices up to T + 78 are read.
@@ -146,12 +162,13 @@ __global__ __launch_bounds__(kMfmaThreads) void fir_mfma_kernel(
     __shared__ float scratch[kMfmaThreads / 64];
     const int kp = mfma_kp(num_taps);
     const int U = kMfmaOutputs + kp - 32;
+    const int Us = tile_slots(U);
     const int len = mfma_copy_len(num_taps);
     H* rh = reinterpret_cast<H*>(smem);
-    H* rl = rh + U;
-    H* ih = rl + U;
-    H* il = ih + U;
-    H* th = il + U;           // 8 copies
+    H* rl = rh + Us;
+    H* ih = rl + Us;
+    H* il = ih + Us;
+    H* th = il + Us;          // 8 copies
     H* tl = th + 8 * len;     // 8 copies
     const long long n0 = static_cast<long long>(blockIdx.x) * kMfmaOutputs;
     const long long x0 = n0 - kp + 32;
@@ -169,8 +186,9 @@ __global__ __launch_bounds__(kMfmaThreads) void fir_mfma_kernel(
     for (int u = threadIdx.x; u < U; u += blockDim.x) {
         const long long s = x0 + u;
         const float2 v = (s >= 0 && s < n) ? x[s] : make_float2(0.0f, 0.0f);
-        Split<BF16>::split(ldexpf(v.x, ex), rh[u], rl[u]);
-        Split<BF16>::split(ldexpf(v.y, ex), ih[u], il[u]);
+        const int slot = tile_slot(u);
+        Split<BF16>::split(ldexpf(v.x, ex), rh[slot], rl[slot]);
+        Split<BF16>::split(ldexpf(v.y, ex), ih[slot], il[slot]);
     }
     for (int idx = threadIdx.x; idx < 16 * len; idx += blockDim.x) {
         th[idx] = ws[idx];  // th and tl are contiguous, as in the workspace
@@ -187,7 +205,7 @@ __global__ __launch_bounds__(kMfmaThreads) void fir_mfma_kernel(
     f32x16 acc_im1 = {};
     const int steps = kp / 16;
     for (int s = 0; s < steps; ++s) {
-        const int u0 = 1024 * w + 32 * i - 16 * s - 8 * g + kp - 8;
+        const int u0 = tile_slot(1024 * w + 32 * i - 16 * s - 8 * g + kp - 8);
         const V arh = *reinterpret_cast<const V*>(rh + u0);
         const V arl = *reinterpret_cast<const V*>(rl + u0);
         const V aih = *reinterpret_cast<const V*>(ih + u0);
@@ -227,8 +245,8 @@ __global__ __launch_bounds__(kMfmaThreads) void fir_mfma_kernel(
 
 std::size_t mfma_lds_bytes(int num_taps)
 {
-    const std::size_t U = kMfmaOutputs + mfma_kp(num_taps) - 32;
-    return (4 * U + 16 * static_cast<std::size_t>(mfma_copy_len(num_taps))) * 2;
+    const std::size_t Us = tile_slots(kMfmaOutputs + mfma_kp(num_taps) - 32);
+    return (4 * Us + 16 * static_cast<std::size_t>(mfma_copy_len(num_taps))) * 2;
 }
 
 template <bool BF16>

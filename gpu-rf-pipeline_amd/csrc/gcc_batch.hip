@@ -35,16 +35,14 @@ struct rf_gcc_batch {
     int count = 0;
     std::vector<PairDesc> host;
     PairDesc* dev = nullptr;
-    int* col_prefix = nullptr;  // count + 1
+    int2* col_map = nullptr;    // col_total items: {pair, column tile}
     int col_total = 0;
     // Row-pass items, split by row length: pairs with 4096-point rows run the register-resident
-    // radix-16 kernel, the others the generic LDS kernel. Each list is a set of pair indices
-    // with an item prefix over them.
+    // radix-16 kernel, the others the generic LDS kernel, each over its own item map.
     struct RowList {
         int count = 0;
         int total = 0;
-        int* prefix = nullptr;  // count + 1
-        int* pairs = nullptr;   // count
+        int2* map = nullptr;    // total items: {pair, row item within the pair}
     };
     RowList rows4096;
     RowList rows_generic;
@@ -76,22 +74,19 @@ namespace rf {
 namespace {
 
 __global__ __launch_bounds__(kItemThreads) void batch_row_kernel(
-    const PairDesc* __restrict__ pairs, const int* __restrict__ prefix,
-    const int* __restrict__ list, int count, int begin, int total,
+    const PairDesc* __restrict__ pairs, const int2* __restrict__ map, int begin, int total,
     const float2* __restrict__ table)
 {
     extern __shared__ float2 lds[];
     xcd_major_items(total, [&](int local) {
-        const int item = begin + local;
-        const int k = find_pair(prefix, count, item);
-        const int p = list[k];
-        const PairDesc d = pairs[p];
+        const int2 m = map[begin + local];
+        const PairDesc d = pairs[m.x];
         const FourStep fs = shape_of(d);
         const long long L = 1LL << d.log2L;
         const int n1 = 1 << d.log2n1;
         const int F = lds_batch_per_block(d.log2n1);
         const int slots = lds_slots(n1);
-        const int row0 = (item - prefix[k]) * F;
+        const int row0 = m.y * F;
         float2* wa = d.work + static_cast<long long>(row0) * n1;
         float2* wb = d.work + L + static_cast<long long>(row0) * n1;
         float2* la = lds;
@@ -99,8 +94,8 @@ __global__ __launch_bounds__(kItemThreads) void batch_row_kernel(
         const int total_e = F * n1;
         for (int e = threadIdx.x; e < total_e; e += blockDim.x) {
             const int s = (e >> d.log2n1) * slots + lds_pad(e & (n1 - 1));
-            la[s] = wa[e];
-            lb[s] = wb[e];
+            la[s] = gload(wa, e);
+            lb[s] = gload(wb, e);
         }
         __syncthreads();
         const int per = n1 >> 3;
@@ -119,7 +114,7 @@ __global__ __launch_bounds__(kItemThreads) void batch_row_kernel(
             const int rr = e >> d.log2n1;
             const int j1 = e & (n1 - 1);
             const float2 v = lb[rr * slots + lds_pad(j1)];
-            wb[e] = cmul(v, four_step_twiddle<true>(fs, static_cast<long long>(j1) * (row0 + rr)));
+            gstore(wb, e, cmul(v, four_step_twiddle<true>(fs, static_cast<long long>(j1) * (row0 + rr))));
         }
         __syncthreads();
     });
@@ -128,18 +123,16 @@ __global__ __launch_bounds__(kItemThreads) void batch_row_kernel(
 // Row pass of the pairs whose rows are 4096 points: one row per item, 256 lanes, the
 // register-resident radix-16 transforms.
 __global__ __launch_bounds__(k4096Lanes) void batch_row4096_kernel(
-    const PairDesc* __restrict__ pairs, const int* __restrict__ prefix,
-    const int* __restrict__ list, int count, int begin, int total,
+    const PairDesc* __restrict__ pairs, const int2* __restrict__ map, int begin, int total,
     const float2* __restrict__ stages)
 {
     __shared__ float2 buf[k4096Slots];
     xcd_major_items(total, [&](int local) {
-        const int item = begin + local;
-        const int k = find_pair(prefix, count, item);
-        const PairDesc d = pairs[list[k]];
+        const int2 m = map[begin + local];
+        const PairDesc d = pairs[m.x];
         const FourStep fs = shape_of(d);
         const long long L = 1LL << d.log2L;
-        const long long row = item - prefix[k];
+        const long long row = m.y;
         const float2* wa = d.work + (row << k4096Log2);
         float2* wb = d.work + L + (row << k4096Log2);
         xcorr_row4096(wa, wb, fs, row, 1, 1.0f / static_cast<float>(L), buf, stages);
@@ -203,25 +196,21 @@ void batch_run(rf_gcc_batch& h, rf_peak_result* results, hipStream_t stream)
     };
     for (const rf_gcc_batch::Group& g : h.groups) {
         batch_column_fwd_kernel<<<grid_for(2 * g.col_items), kItemThreads, column_lds_bytes(),
-                                  stream>>>(h.dev, h.col_prefix, h.count, g.col_begin,
-                                            g.col_items, table);
+                                  stream>>>(h.dev, h.col_map, g.col_begin, g.col_items, table);
         check_launch("batch_column_fwd_kernel");
         if (g.row_items[0] > 0) {
             batch_row4096_kernel<<<grid_for(g.row_items[0]), k4096Lanes, 0, stream>>>(
-                h.dev, h.rows4096.prefix, h.rows4096.pairs, h.rows4096.count, g.row_begin[0],
-                g.row_items[0], fft4096_stage_table());
+                h.dev, h.rows4096.map, g.row_begin[0], g.row_items[0], fft4096_stage_table());
             check_launch("batch_row4096_kernel");
         }
         if (g.row_items[1] > 0) {
             batch_row_kernel<<<grid_for(g.row_items[1]), kItemThreads, row_lds_bytes(), stream>>>(
-                h.dev, h.rows_generic.prefix, h.rows_generic.pairs, h.rows_generic.count,
-                g.row_begin[1], g.row_items[1], table);
+                h.dev, h.rows_generic.map, g.row_begin[1], g.row_items[1], table);
             check_launch("batch_row_kernel");
         }
         batch_column_inv_peak_kernel<<<grid_for(g.col_items), kItemThreads, column_lds_bytes(),
-                                       stream>>>(h.dev, h.col_prefix, h.count, g.col_begin,
-                                                 g.col_items, results, ItemSet{nullptr, nullptr, 0},
-                                                 table);
+                                       stream>>>(h.dev, h.col_map, g.col_begin, g.col_items,
+                                                 results, ItemSet{nullptr, nullptr, 0}, table);
         check_launch("batch_column_inv_peak_kernel");
     }
     batch_finish_kernel<<<static_cast<unsigned>(h.count), 64, 0, stream>>>(h.dev, results,
@@ -241,10 +230,9 @@ void destroy_batch(rf_gcc_batch* h)
         (void)hipStreamDestroy(h->capture);
     }
     (void)hipFree(h->dev);
-    (void)hipFree(h->col_prefix);
+    (void)hipFree(h->col_map);
     for (auto* list : {&h->rows4096, &h->rows_generic}) {
-        (void)hipFree(list->prefix);
-        (void)hipFree(list->pairs);
+        (void)hipFree(list->map);
     }
     (void)hipFree(h->work);
     for (auto& kv : h->tables) {
@@ -341,9 +329,19 @@ rf_status rf_gcc_batch_create(int count, const int* num_a, const int* num_b,
                 d.hi = it->second + rf::kTwiddleSize;
             }
             RF_HIP_CHECK(hipMalloc(&h->dev, h->host.size() * sizeof(PairDesc)));
-            RF_HIP_CHECK(hipMalloc(&h->col_prefix, colp.size() * sizeof(int)));
-            RF_HIP_CHECK(hipMemcpy(h->col_prefix, colp.data(), colp.size() * sizeof(int),
-                                   hipMemcpyHostToDevice));
+            auto upload_map = [&](const std::vector<int2>& map, int2** dev) {
+                RF_HIP_CHECK(hipMalloc(dev, map.size() * sizeof(int2)));
+                RF_HIP_CHECK(hipMemcpy(*dev, map.data(), map.size() * sizeof(int2),
+                                       hipMemcpyHostToDevice));
+                h->bytes += map.size() * sizeof(int2);
+            };
+            std::vector<int2> col_map;
+            for (int i = 0; i < count; ++i) {
+                for (int c = 0; c < h->host[static_cast<std::size_t>(i)].col_items; ++c) {
+                    col_map.push_back(make_int2(i, c));
+                }
+            }
+            upload_map(col_map, &h->col_map);
             rf_gcc_batch::RowList* lists[2] = {&h->rows4096, &h->rows_generic};
             for (int kind = 0; kind < 2; ++kind) {
                 rf_gcc_batch::RowList& list = *lists[kind];
@@ -352,20 +350,18 @@ rf_status rf_gcc_batch_create(int count, const int* num_a, const int* num_b,
                 if (list.count == 0) {
                     continue;
                 }
-                RF_HIP_CHECK(hipMalloc(&list.prefix, row_prefix[kind].size() * sizeof(int)));
-                RF_HIP_CHECK(hipMalloc(&list.pairs, row_pairs[kind].size() * sizeof(int)));
-                RF_HIP_CHECK(hipMemcpy(list.prefix, row_prefix[kind].data(),
-                                       row_prefix[kind].size() * sizeof(int),
-                                       hipMemcpyHostToDevice));
-                RF_HIP_CHECK(hipMemcpy(list.pairs, row_pairs[kind].data(),
-                                       row_pairs[kind].size() * sizeof(int),
-                                       hipMemcpyHostToDevice));
-                h->bytes += (row_prefix[kind].size() + row_pairs[kind].size()) * sizeof(int);
+                std::vector<int2> map;
+                for (int p : row_pairs[kind]) {
+                    for (int r = 0; r < h->host[static_cast<std::size_t>(p)].row_items; ++r) {
+                        map.push_back(make_int2(p, r));
+                    }
+                }
+                upload_map(map, &list.map);
             }
             RF_HIP_CHECK(hipMemcpy(h->dev, h->host.data(), h->host.size() * sizeof(PairDesc),
                                    hipMemcpyHostToDevice));
             h->bytes += static_cast<std::size_t>(work_total) * sizeof(float2)
-                        + h->host.size() * sizeof(PairDesc) + colp.size() * sizeof(int);
+                        + h->host.size() * sizeof(PairDesc);
         } catch (...) {
             rf::destroy_batch(h);
             throw;

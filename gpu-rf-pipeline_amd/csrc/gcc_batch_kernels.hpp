@@ -34,21 +34,8 @@ namespace {
 
 constexpr int kItemThreads = 512;
 
-__device__ __forceinline__ int find_pair(const int* __restrict__ prefix, int count, int item)
-{
-    int lo = 0;
-    int hi = count - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (prefix[mid] <= item) {
-            lo = mid;
-        } else {
-            hi = mid - 1;
-        }
-    }
-    return lo;
-}
-
+// Item maps: item -> {pair, index of the item within its pair}, built on the host, so a
+// persistent workgroup finds its item's pair with one load.
 __device__ __forceinline__ FourStep shape_of(const PairDesc& d)
 {
     FourStep fs;
@@ -70,24 +57,23 @@ __global__ void batch_init_kernel(R* results, int count)
     }
 }
 
-// Items begin .. begin + total - 1 of the column prefix, for a and for b.
+// Column items begin .. begin + total - 1, for a and for b.
 __global__ __launch_bounds__(kItemThreads) void batch_column_fwd_kernel(
-    const PairDesc* __restrict__ pairs, const int* __restrict__ prefix, int count, int begin,
-    int total, const float2* __restrict__ table)
+    const PairDesc* __restrict__ pairs, const int2* __restrict__ map, int begin, int total,
+    const float2* __restrict__ table)
 {
     extern __shared__ float2 lds[];
     xcd_major_items(2 * total, [&](int item) {
         const int which = item >= total ? 1 : 0;
-        const int local = begin + item - which * total;
-        const int p = find_pair(prefix, count, local);
-        const PairDesc d = pairs[p];
+        const int2 m = map[begin + item - which * total];
+        const PairDesc d = pairs[m.x];
         const FourStep fs = shape_of(d);
         const long long L = 1LL << d.log2L;
         const int log2n1 = d.log2n1;
-        const long long c0 = static_cast<long long>(local - prefix[p]) * column_tile(fs);
+        const long long c0 = static_cast<long long>(m.y) * column_tile(fs);
         float2* dst = d.work + (which ? L : 0);
         auto store = [=](long long j1, int k2, float2 v) {
-            dst[j1 + (static_cast<long long>(k2) << log2n1)] = v;
+            gstore(dst, j1 + (static_cast<long long>(k2) << log2n1), v);
         };
         if (which == 0) {
             const float2* a = d.a;
@@ -112,7 +98,7 @@ __global__ __launch_bounds__(kItemThreads) void batch_column_fwd_kernel(
     });
 }
 
-// The items a pass visits: items begin .. begin + total - 1 of the item prefix, or, with
+// The items a pass visits: items begin .. begin + total - 1 of the item map, or, with
 // `list` set, the 2^log2_items items of each of the pairs list[0 .. *list_count - 1] (pairs of
 // one shape; the count is read on the device, so the launch needs no host round trip).
 struct ItemSet {
@@ -123,8 +109,8 @@ struct ItemSet {
 
 template <class R>
 __global__ __launch_bounds__(kItemThreads) void batch_column_inv_peak_kernel(
-    const PairDesc* __restrict__ pairs, const int* __restrict__ prefix, int count, int begin,
-    int total, R* results, ItemSet set, const float2* __restrict__ table)
+    const PairDesc* __restrict__ pairs, const int2* __restrict__ map, int begin, int total,
+    R* results, ItemSet set, const float2* __restrict__ table)
 {
     extern __shared__ float2 lds[];
     __shared__ unsigned long long wave_best[kItemThreads / 64];
@@ -136,9 +122,9 @@ __global__ __launch_bounds__(kItemThreads) void batch_column_inv_peak_kernel(
             p = set.list[local >> set.log2_items];
             within = local & ((1 << set.log2_items) - 1);
         } else {
-            const int item = begin + local;
-            p = find_pair(prefix, count, item);
-            within = item - prefix[p];
+            const int2 m = map[begin + local];
+            p = m.x;
+            within = m.y;
         }
         const PairDesc d = pairs[p];
         const FourStep fs = shape_of(d);
@@ -150,11 +136,17 @@ __global__ __launch_bounds__(kItemThreads) void batch_column_inv_peak_kernel(
         const int slots = column_slots(n2);
         const long long c0 = static_cast<long long>(within) * C;
         const float2* src = d.work + L;
-        const int total_e = C * n2;
-        for (int e = threadIdx.x; e < total_e; e += blockDim.x) {
-            const int row = e >> lc;
-            const int col = e & (C - 1);
-            lds[col * slots + lds_pad(row)] = src[(c0 + col) + (static_cast<long long>(row) << log2n1)];
+        // C n2 = 4096 values, 8 per lane, all loads in flight before the LDS writes
+        float2 v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int e = threadIdx.x + i * kItemThreads;
+            v[i] = gload(src, (c0 + (e & (C - 1))) + (static_cast<long long>(e >> lc) << log2n1));
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int e = threadIdx.x + i * kItemThreads;
+            lds[(e & (C - 1)) * slots + lds_pad(e >> lc)] = v[i];
         }
         __syncthreads();
         const int f = threadIdx.x >> (d.log2n2 - 3);
@@ -162,16 +154,18 @@ __global__ __launch_bounds__(kItemThreads) void batch_column_inv_peak_kernel(
         lds_fft<true>(lds + f * slots, d.log2n2, t, true, table);
         const long long lags = static_cast<long long>(d.num_a) + d.num_b - 1;
         unsigned long long best = 0ULL;
-        for (int e = threadIdx.x; e < total_e; e += blockDim.x) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int e = threadIdx.x + i * kItemThreads;
             const int j2 = e >> lc;
             const int col = e & (C - 1);
             const long long n = (c0 + col) + (static_cast<long long>(j2) << log2n1);
             if (n < lags) {
-                const float2 v = lds[col * slots + lds_pad(j2)];
-                const unsigned long long key = peak_key(hypotf(v.x, v.y), n);
+                const float2 val = lds[col * slots + lds_pad(j2)];
+                const unsigned long long key = peak_key(hypotf(val.x, val.y), n);
                 best = key > best ? key : best;
                 if (d.r != nullptr) {
-                    d.r[n] = v;
+                    gstore(d.r, n, val);
                 }
             }
         }

@@ -124,4 +124,21 @@ __device__ __forceinline__ float2 cscale(float2 a, float s)
     return make_float2(a.x * s, a.y * s);
 }
 
+// Complex loads and stores through the global address space. A pointer read from memory (a
+// descriptor struct) is generic to the compiler, which then emits flat instructions; those
+// also count against the LDS wait counter, so every LDS wait in a loop would wait for the
+// HBM loads in flight too. These keep such accesses global_load / global_store.
+typedef __attribute__((address_space(1))) const double global_cdouble;
+typedef __attribute__((address_space(1))) double global_double;
+
+__device__ __forceinline__ float2 gload(const float2* p, long long i)
+{
+    return __builtin_bit_cast(float2, ((global_cdouble*)p)[i]);  // address-space cast
+}
+
+__device__ __forceinline__ void gstore(float2* p, long long i, float2 v)
+{
+    ((global_double*)p)[i] = __builtin_bit_cast(double, v);  // address-space cast
+}
+
 }  // namespace rf

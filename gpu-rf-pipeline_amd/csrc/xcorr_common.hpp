@@ -15,12 +15,12 @@ __device__ __forceinline__ float2 rotated_a(const float2* __restrict__ a, int nu
     if (m >= L) {
         m -= L;
     }
-    return m < num_a ? a[m] : make_float2(0.0f, 0.0f);
+    return m < num_a ? gload(a, m) : make_float2(0.0f, 0.0f);
 }
 
 __device__ __forceinline__ float2 padded_b(const float2* __restrict__ b, int num_b, long long n)
 {
-    return n < num_b ? b[n] : make_float2(0.0f, 0.0f);
+    return n < num_b ? gload(b, n) : make_float2(0.0f, 0.0f);
 }
 
 // B conj(A) / L, or for GCC-PHAT (B conj(A)) * (1/L) / |B conj(A)| (0 where that is 0),
@@ -59,8 +59,8 @@ __device__ __forceinline__ void xcorr_row4096(const float2* __restrict__ wa,
     float2 ub[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        ua[r] = wa[t + r * k4096Lanes];
-        ub[r] = wb[t + r * k4096Lanes];
+        ua[r] = gload(wa, t + r * k4096Lanes);
+        ub[r] = gload(wb, t + r * k4096Lanes);
     }
     fft4096<false>(buf, t, ua, stages);
     __syncthreads();  // every lane has read a's last pass from buf
@@ -74,7 +74,7 @@ __device__ __forceinline__ void xcorr_row4096(const float2* __restrict__ wa,
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int j1 = t + r * k4096Lanes;
-        wb[j1] = cmul(ub[r], four_step_twiddle<true>(fs, j1 * row));
+        gstore(wb, j1, cmul(ub[r], four_step_twiddle<true>(fs, j1 * row)));
     }
 }
 
