@@ -98,6 +98,17 @@ _proto("rf_fir", _st, _int, _vp, _vp, _vp, _int, _int, _int, _vp)
 _proto("rf_fir_batched", _st, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
        ctypes.POINTER(_int), _int, _vp, _int, _vp)
 _proto("rf_magnitude", _st, _vp, _vp, _int, _vp)
+
+
+class FirConfig(ctypes.Structure):
+    """rf_fir_config: variant, lanes per workgroup, outputs per lane, measured ms."""
+    _fields_ = [("impl", ctypes.c_int32), ("block", ctypes.c_int32),
+                ("outputs_per_lane", ctypes.c_int32), ("ms", ctypes.c_float)]
+
+
+_proto("rf_fir_tune", _st, _vp, _vp, _vp, _int, _int, ctypes.c_uint, _vp,
+       ctypes.POINTER(FirConfig))
+_proto("rf_fir_run_config", _st, ctypes.POINTER(FirConfig), _vp, _vp, _vp, _int, _int, _vp)
 _proto("rf_fft_plan_create", _st, _int, _int, ctypes.POINTER(_vp))
 _proto("rf_fft_plan_destroy", _st, _vp)
 _proto("rf_fft_plan_bytes", _st, _vp, ctypes.POINTER(_size))
@@ -691,6 +702,22 @@ class GccBatch:
                 self.handle = None
         except Exception:
             pass
+
+
+def fir_tune(x: int, taps: int, y: int, n: int, num_taps: int, impls: Sequence[str],
+             stream=None) -> FirConfig:
+    """Autotunes the FIR variants named in `impls` on device buffers; returns the fastest."""
+    mask = 0
+    for name in impls:
+        mask |= 1 << fir_impl_index(name)
+    best = FirConfig()
+    check(_lib.rf_fir_tune(x, taps, y, n, num_taps, mask, _sp(stream), ctypes.byref(best)))
+    return best
+
+
+def fir_run_config(config: FirConfig, x: int, taps: int, y: int, n: int, num_taps: int,
+                   stream=None) -> None:
+    check(_lib.rf_fir_run_config(ctypes.byref(config), x, taps, y, n, num_taps, _sp(stream)))
 
 
 DELAY_FEATURES = 256

@@ -24,6 +24,7 @@
 
 #include <cstring>
 #include <string>
+#include <vector>
 
 namespace rf {
 namespace {
@@ -518,11 +519,20 @@ void allow_lds(Kernel kernel, std::size_t bytes)
     allow_large_lds(reinterpret_cast<const void*>(kernel), bytes);
 }
 
+// `forced_r` (4, 8 or 16) overrides the outputs-per-lane choice; 0 keeps it.
 void launch_register(const float2* x, const float* taps, float2* y, int n, int num_taps,
-                     int block, hipStream_t stream)
+                     int block, hipStream_t stream, int forced_r = 0)
 {
     std::size_t bytes = 0;
-    const int r = register_outputs(num_taps, block, n, &bytes);
+    int r = register_outputs(num_taps, block, n, &bytes);
+    if (forced_r != 0) {
+        r = forced_r;
+        bytes = register_bytes_for(r, num_taps, block);
+        if (bytes > static_cast<std::size_t>(max_lds_per_block())) {
+            fail(RF_LIMIT, "fir register: " + std::to_string(r) + " outputs per lane need "
+                               + std::to_string(bytes) + " bytes of LDS");
+        }
+    }
     const unsigned grid = static_cast<unsigned>(ceil_div(n, static_cast<long long>(block) * r));
     if (r == 16) {
         allow_lds(fir_register_kernel<16>, bytes);
@@ -735,6 +745,107 @@ rf_status rf_fir(int impl, const rf_cf32* x, const float* taps, rf_cf32* y, int 
         rf::fir_launch(impl, reinterpret_cast<const float2*>(x), taps,
                        reinterpret_cast<float2*>(y), num_samples, num_taps, b,
                        rf::as_stream(stream));
+    });
+}
+
+rf_status rf_fir_run_config(const rf_fir_config* config, const rf_cf32* x, const float* taps,
+                            rf_cf32* y, int num_samples, int num_taps, rf_stream stream)
+{
+    return guarded([&] {
+        if (config == nullptr) rf::fail(RF_INVALID_ARG, "fir config: NULL");
+        const int r = config->outputs_per_lane;
+        if (r != 0 && r != 4 && r != 8 && r != 16) {
+            rf::fail(RF_INVALID_ARG, "fir config: outputs_per_lane must be 0, 4, 8 or 16");
+        }
+        rf::check_fir_args(config->impl, x, taps, y, num_samples, num_taps, config->block);
+        if (num_samples == 0) {
+            return;
+        }
+        if (config->impl == RF_FIR_REGISTER && r != 0) {
+            rf::launch_register(reinterpret_cast<const float2*>(x), taps,
+                                reinterpret_cast<float2*>(y), num_samples, num_taps,
+                                config->block, rf::as_stream(stream), r);
+            return;
+        }
+        rf::fir_launch(config->impl, reinterpret_cast<const float2*>(x), taps,
+                       reinterpret_cast<float2*>(y), num_samples, num_taps, config->block,
+                       rf::as_stream(stream));
+    });
+}
+
+rf_status rf_fir_tune(const rf_cf32* x, const float* taps, rf_cf32* y, int num_samples,
+                      int num_taps, unsigned impl_mask, rf_stream stream, rf_fir_config* best)
+{
+    return guarded([&] {
+        if (best == nullptr) rf::fail(RF_INVALID_ARG, "fir tune: NULL result");
+        rf::check_fir_args(RF_FIR_REGISTER, x, taps, y, num_samples, num_taps,
+                           RF_DEFAULT_FIR_BLOCK);
+        if ((impl_mask & ((1u << RF_FIR_IMPL_COUNT) - 1)) == 0) {
+            rf::fail(RF_INVALID_ARG, "fir tune: no variant selected");
+        }
+        std::vector<rf_fir_config> candidates;
+        for (int impl = 0; impl < RF_FIR_IMPL_COUNT; ++impl) {
+            if ((impl_mask >> impl & 1u) == 0) {
+                continue;
+            }
+            if (impl == RF_FIR_TILED_CONST && num_taps > RF_MAX_CONST_TAPS) {
+                continue;
+            }
+            if (impl == RF_FIR_REGISTER) {
+                for (int block : {64, 128, 256, 512}) {
+                    for (int r : {4, 8, 16}) {
+                        if (rf::register_bytes_for(r, num_taps, block)
+                            <= static_cast<std::size_t>(rf::max_lds_per_block())) {
+                            candidates.push_back(rf_fir_config{impl, block, r, 0.0f});
+                        }
+                    }
+                }
+            } else if (impl == RF_FIR_NAIVE || impl == RF_FIR_TILED || impl == RF_FIR_TILED_CONST) {
+                for (int block : {128, 256, 512}) {
+                    candidates.push_back(rf_fir_config{impl, block, 0, 0.0f});
+                }
+            } else {
+                candidates.push_back(rf_fir_config{impl, RF_DEFAULT_FIR_BLOCK, 0, 0.0f});
+            }
+        }
+        hipStream_t s = rf::as_stream(stream);
+        hipEvent_t start = nullptr;
+        hipEvent_t stop = nullptr;
+        RF_HIP_CHECK(hipEventCreate(&start));
+        RF_HIP_CHECK(hipEventCreate(&stop));
+        bool found = false;
+        try {
+            for (rf_fir_config& c : candidates) {
+                // A candidate that does not fit (LDS) is skipped, not an error of the tuner.
+                if (rf_fir_run_config(&c, x, taps, y, num_samples, num_taps, stream) != RF_OK) {
+                    continue;
+                }
+                RF_HIP_CHECK(hipEventRecord(start, s));
+                for (int rep = 0; rep < 3; ++rep) {
+                    const rf_status st = rf_fir_run_config(&c, x, taps, y, num_samples, num_taps,
+                                                           stream);
+                    if (st != RF_OK) {
+                        rf::fail(st, std::string("fir tune: ") + rf_last_error());
+                    }
+                }
+                RF_HIP_CHECK(hipEventRecord(stop, s));
+                RF_HIP_CHECK(hipEventSynchronize(stop));
+                float ms = 0.0f;
+                RF_HIP_CHECK(hipEventElapsedTime(&ms, start, stop));
+                c.ms = ms / 3.0f;
+                if (!found || c.ms < best->ms) {
+                    *best = c;
+                    found = true;
+                }
+            }
+        } catch (...) {
+            (void)hipEventDestroy(start);
+            (void)hipEventDestroy(stop);
+            throw;
+        }
+        (void)hipEventDestroy(start);
+        (void)hipEventDestroy(stop);
+        if (!found) rf::fail(RF_LIMIT, "fir tune: no selected variant fits this filter");
     });
 }
 

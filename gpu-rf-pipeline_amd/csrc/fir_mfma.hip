@@ -33,9 +33,15 @@ namespace {
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMfmaThreads = 256;                // 4 waves, 1024 outputs each
 constexpr int kMfmaOutputs = 4 * 1024;           // per workgroup
+constexpr int kMfmaMaxTaps = 2049;
+// Per-lane load counts of the prologue at the largest supported tap count: tile samples
+// (4096 + Kp - 32 <= 6144) and 16-byte tap-table vectors (2 copy_len <= 2 * 2136).
+constexpr int kTileLoads = 24;
+constexpr int kTapLoads = 17;
 
 __host__ __device__ constexpr int mfma_kp(int num_taps)
 {
@@ -173,25 +179,45 @@ __global__ __launch_bounds__(kMfmaThreads) void fir_mfma_kernel(
     const long long n0 = static_cast<long long>(blockIdx.x) * kMfmaOutputs;
     const long long x0 = n0 - kp + 32;
 
-    // Samples: block max for the FP16 scale, then the split tile.
+    // Samples: every lane loads its (at most kTileLoads) tile samples at once, then the block
+    // max for the FP16 scale, then the split tile.
+    float2 xs[kTileLoads];
     float m = 0.0f;
-    for (int u = threadIdx.x; u < U; u += blockDim.x) {
+#pragma unroll
+    for (int j = 0; j < kTileLoads; ++j) {
+        const int u = threadIdx.x + j * kMfmaThreads;
         const long long s = x0 + u;
-        if (s >= 0 && s < n) {
-            const float2 v = x[s];
-            m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+        xs[j] = (u < U && s >= 0 && s < n) ? x[s] : make_float2(0.0f, 0.0f);
+        m = fmaxf(m, fmaxf(fabsf(xs[j].x), fabsf(xs[j].y)));
+    }
+    // Taps: 16-byte copies of the hi/lo tables (16 len 16-bit values), all in flight.
+    const int vecs = 2 * len;  // 16 len values / 8 per 16-byte vector
+    const u32x4* wsv = reinterpret_cast<const u32x4*>(ws);
+    u32x4* thv = reinterpret_cast<u32x4*>(th);
+    u32x4 tv[kTapLoads];
+#pragma unroll
+    for (int j = 0; j < kTapLoads; ++j) {
+        const int idx = threadIdx.x + j * kMfmaThreads;
+        if (idx < vecs) {
+            tv[j] = wsv[idx];
         }
     }
     const int ex = BF16 ? 0 : fp16_scale_exponent(block_max(m, scratch));
-    for (int u = threadIdx.x; u < U; u += blockDim.x) {
-        const long long s = x0 + u;
-        const float2 v = (s >= 0 && s < n) ? x[s] : make_float2(0.0f, 0.0f);
-        const int slot = tile_slot(u);
-        Split<BF16>::split(ldexpf(v.x, ex), rh[slot], rl[slot]);
-        Split<BF16>::split(ldexpf(v.y, ex), ih[slot], il[slot]);
+#pragma unroll
+    for (int j = 0; j < kTileLoads; ++j) {
+        const int u = threadIdx.x + j * kMfmaThreads;
+        if (u < U) {
+            const int slot = tile_slot(u);
+            Split<BF16>::split(ldexpf(xs[j].x, ex), rh[slot], rl[slot]);
+            Split<BF16>::split(ldexpf(xs[j].y, ex), ih[slot], il[slot]);
+        }
     }
-    for (int idx = threadIdx.x; idx < 16 * len; idx += blockDim.x) {
-        th[idx] = ws[idx];  // th and tl are contiguous, as in the workspace
+#pragma unroll
+    for (int j = 0; j < kTapLoads; ++j) {
+        const int idx = threadIdx.x + j * kMfmaThreads;
+        if (idx < vecs) {
+            thv[idx] = tv[j];  // th and tl are contiguous, as in the workspace
+        }
     }
     __syncthreads();
 
@@ -203,8 +229,9 @@ __global__ __launch_bounds__(kMfmaThreads) void fir_mfma_kernel(
     f32x16 acc_im0 = {};
     f32x16 acc_re1 = {};
     f32x16 acc_im1 = {};
-    const int steps = kp / 16;
-    for (int s = 0; s < steps; ++s) {
+    // Step s covers kk = 16 s .. 16 s + 15; even steps accumulate into acc*0, odd ones into
+    // acc*1, two steps per iteration with no branch, so the accumulators stay in place.
+    auto step = [&](int s, f32x16& acc_re, f32x16& acc_im) {
         const int u0 = tile_slot(1024 * w + 32 * i - 16 * s - 8 * g + kp - 8);
         const V arh = *reinterpret_cast<const V*>(rh + u0);
         const V arl = *reinterpret_cast<const V*>(rl + u0);
@@ -215,21 +242,21 @@ __global__ __launch_bounds__(kMfmaThreads) void fir_mfma_kernel(
         const int base = r * len + (b - r);
         const V bh = *reinterpret_cast<const V*>(th + base);
         const V bl = *reinterpret_cast<const V*>(tl + base);
-        if (s & 1) {
-            acc_re1 = Split<BF16>::mma(arh, bh, acc_re1);
-            acc_im1 = Split<BF16>::mma(aih, bh, acc_im1);
-            acc_re1 = Split<BF16>::mma(arh, bl, acc_re1);
-            acc_im1 = Split<BF16>::mma(aih, bl, acc_im1);
-            acc_re1 = Split<BF16>::mma(arl, bh, acc_re1);
-            acc_im1 = Split<BF16>::mma(ail, bh, acc_im1);
-        } else {
-            acc_re0 = Split<BF16>::mma(arh, bh, acc_re0);
-            acc_im0 = Split<BF16>::mma(aih, bh, acc_im0);
-            acc_re0 = Split<BF16>::mma(arh, bl, acc_re0);
-            acc_im0 = Split<BF16>::mma(aih, bl, acc_im0);
-            acc_re0 = Split<BF16>::mma(arl, bh, acc_re0);
-            acc_im0 = Split<BF16>::mma(ail, bh, acc_im0);
-        }
+        acc_re = Split<BF16>::mma(arh, bh, acc_re);
+        acc_im = Split<BF16>::mma(aih, bh, acc_im);
+        acc_re = Split<BF16>::mma(arh, bl, acc_re);
+        acc_im = Split<BF16>::mma(aih, bl, acc_im);
+        acc_re = Split<BF16>::mma(arl, bh, acc_re);
+        acc_im = Split<BF16>::mma(ail, bh, acc_im);
+    };
+    const int steps = kp / 16;
+    int s = 0;
+    for (; s + 1 < steps; s += 2) {
+        step(s, acc_re0, acc_im0);
+        step(s + 1, acc_re1, acc_im1);
+    }
+    if (s < steps) {
+        step(s, acc_re0, acc_im0);
     }
     const float scale = ldexpf(*tap_descale, -ex);
     const long long nw = n0 + 1024 * w;
@@ -253,6 +280,9 @@ template <bool BF16>
 void launch_mfma(const float2* x, const float* taps, float2* y, int n, int num_taps,
                  hipStream_t stream)
 {
+    if (num_taps > kMfmaMaxTaps) {
+        fail(RF_LIMIT, "fir mfma: at most " + std::to_string(kMfmaMaxTaps) + " taps");
+    }
     const std::size_t bytes = mfma_lds_bytes(num_taps);
     if (bytes > static_cast<std::size_t>(max_lds_per_block())) {
         fail(RF_LIMIT, "fir mfma: " + std::to_string(num_taps) + " taps need "

@@ -123,6 +123,24 @@ rf_status rf_fir_batched(int impl, const rf_cf32* const* x, rf_cf32* const* y,
                          const int* num_samples, int count, const float* taps, int num_taps,
                          rf_stream stream);
 
+/* FIR autotuner. No counterpart in the reference (which fixes block 256 and R by tap count,
+ * src/fir.cu:449-485): times every variant allowed by impl_mask (bit i = RF_FIR_* value i) on
+ * the caller's buffers (y is overwritten) with its tile shapes, lanes per workgroup
+ * {64, 128, 256, 512} x outputs per lane {4, 8, 16} for `register`, {128, 256, 512} for the
+ * other direct variants, and reports the fastest; device events on `stream`, which it
+ * synchronises. The direct variants give identical bits for every shape. */
+typedef struct rf_fir_config {
+    int32_t impl;             /* RF_FIR_* */
+    int32_t block;            /* lanes per workgroup of the direct variants */
+    int32_t outputs_per_lane; /* register variant: 4, 8 or 16; 0 = the library's choice */
+    float ms;                 /* measured time of one call */
+} rf_fir_config;
+rf_status rf_fir_tune(const rf_cf32* x, const float* taps, rf_cf32* y, int num_samples,
+                      int num_taps, unsigned impl_mask, rf_stream stream, rf_fir_config* best);
+/* rf_fir with a tuned configuration. */
+rf_status rf_fir_run_config(const rf_fir_config* config, const rf_cf32* x, const float* taps,
+                            rf_cf32* y, int num_samples, int num_taps, rf_stream stream);
+
 /* ---------------------------------------------------------------------------------------
  * Magnitude: mag[n] = |x[n]| (src/magnitude.cu:17-39, tools/reference.py:98-100).
  * ------------------------------------------------------------------------------------- */

@@ -255,6 +255,35 @@ def test_fir_config_plain_2e20_x_63() -> None:
         assert_fir_close(impl, rf.fir(x, h, impl), x, h, want)
 
 
+def test_fir_autotuner_configs_give_identical_bits() -> None:
+    """rf_fir_tune times the direct variants' tile shapes; every shape, and the tuned one, gives
+    the same bits as rf_fir (same per-output summation order)."""
+    rng = np.random.default_rng(11)
+    n, t = 1 << 18, 255
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    h = rng.standard_normal(t).astype(np.float32)
+    d_x = rf.DeviceBuffer.from_array(x)
+    d_h = rf.DeviceBuffer.from_array(h)
+    d_y = rf.DeviceBuffer(n)
+    rf.fir_device(rf.fir_impl_index("register"), d_x.ptr, d_h.ptr, d_y.ptr, n, t)
+    want = d_y.download()
+    best = rf.fir_tune(d_x.ptr, d_h.ptr, d_y.ptr, n, t, ["register", "tiled", "naive"])
+    assert best.impl in (rf.fir_impl_index(v) for v in ("register", "tiled", "naive"))
+    assert best.ms > 0 and best.block in (64, 128, 256, 512)
+    d_y.zero()
+    rf.fir_run_config(best, d_x.ptr, d_h.ptr, d_y.ptr, n, t)
+    assert d_y.download().tobytes() == want.tobytes()
+    for block in (64, 512):
+        for r in (4, 8, 16):
+            cfg = rf.FirConfig(rf.fir_impl_index("register"), block, r, 0.0)
+            d_y.zero()
+            rf.fir_run_config(cfg, d_x.ptr, d_h.ptr, d_y.ptr, n, t)
+            assert d_y.download().tobytes() == want.tobytes(), (block, r)
+    with pytest.raises(rf.RfError):
+        rf.fir_run_config(rf.FirConfig(rf.fir_impl_index("register"), 256, 5, 0.0),
+                          d_x.ptr, d_h.ptr, d_y.ptr, n, t)
+
+
 @pytest.mark.parametrize("impl", ["register", "fft_ola", "mfma_f16x3", "mfma_bf16x3"])
 def test_fir_full_size_long_filter_sampled_outputs(impl: str) -> None:
     # 2^24 samples x 1025 Kaiser taps: 3000 outputs recomputed in float64 on the host.
