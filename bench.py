@@ -178,11 +178,48 @@ def bench_fir(rf, args, long_filter: bool) -> dict:
     }
 
 
+def channelize_input(n: int = 1 << 24, seed: int = 0xC3):
+    """40 QPSK carriers at distinct channel centres k fs / 64, powers U[-30, 0] dB, symbols
+    (+-1 +-1j)/sqrt(2) held for 128 samples, plus complex AWGN at -40 dB (power 1e-4)."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    centres = rng.choice(64, 40, replace=False)
+    amps = 10.0 ** (rng.uniform(-30.0, 0.0, 40) / 20.0)
+    hold = 128
+    blocks = n // hold
+    sym = ((rng.choice([-1.0, 1.0], (blocks, 40)) + 1j * rng.choice([-1.0, 1.0], (blocks, 40)))
+           / np.sqrt(2.0))
+    u = np.arange(hold)
+    phasors = amps[:, None] * np.exp(2j * np.pi * np.outer(centres, u) / 64.0)  # 40 x 128
+    x = (sym @ phasors).reshape(-1)                                               # n samples
+    noise = (rng.standard_normal(n, dtype=np.float32)
+             + 1j * rng.standard_normal(n, dtype=np.float32)) * np.float32(np.sqrt(0.5e-4))
+    return (x + noise).astype(np.complex64)
+
+
+def channelize_errors(got: np.ndarray, x: np.ndarray, h: np.ndarray, m: int,
+                      frames: np.ndarray) -> dict:
+    """Max abs error and max error / sum_l |h x| over the given frames, against float64."""
+    xs = x.astype(np.complex128)
+    l = np.arange(h.size)
+    k = np.arange(m)
+    dft = np.exp(2j * np.pi * np.outer(l, k) / m)
+    worst_abs, worst_rel = 0.0, 0.0
+    for f in frames:
+        idx = f * m - l
+        ok = idx >= 0
+        terms = np.zeros(h.size, np.complex128)
+        terms[ok] = h[ok] * xs[idx[ok]]
+        want = terms @ dft
+        err = np.max(np.abs(got[f] - want))
+        worst_abs = max(worst_abs, float(err))
+        worst_rel = max(worst_rel, float(err / np.sum(np.abs(terms))))
+    return {"max_abs_err": worst_abs, "max_err_over_term_sum": worst_rel,
+            "frames_checked": int(len(frames))}
+
+
 def bench_channelize(rf, args) -> dict:
     n, m, t = 1 << 24, 64, 1024
-    rng = np.random.Generator(np.random.Philox(0xC3))
-    x = ((rng.standard_normal(n, dtype=np.float32) + 1j * rng.standard_normal(n, dtype=np.float32))
-         * np.float32(0.01)).astype(np.complex64)
+    x = channelize_input(n)
     h = kaiser_lowpass(t, 1.0 / 128, 8.0)
     frames = rf.channelize_frames(n, m)
     sets = 3
@@ -195,8 +232,18 @@ def bench_channelize(rf, args) -> dict:
         rf.channelize_device(None, d_x[i % sets].ptr, d_h.ptr, d_z[i % sets].ptr, n, m, t,
                              stream.handle)
 
+    def step_mfma(i):
+        rf.channelize_mfma_device(d_x[i % sets].ptr, d_h.ptr, d_z[i % sets].ptr, n, m, t,
+                                  stream.handle)
+
     ms = time_steps(rf, step, args.steps, args.warmup, stream.handle)
     ms_step = ms / args.steps
+    check_frames = np.sort(np.random.default_rng(5).choice(frames, 48, replace=False))
+    z_fused = d_z[(args.warmup + args.steps - 1) % sets].download().reshape(-1, m)
+    err_fused = channelize_errors(z_fused, x, h, m, check_frames)
+    ms_mfma = time_steps(rf, step_mfma, args.steps, args.warmup, stream.handle) / args.steps
+    z_mfma = d_z[(args.warmup + args.steps - 1) % sets].download().reshape(-1, m)
+    err_mfma = channelize_errors(z_mfma, x, h, m, check_frames)
     bytes_step = 16 * n + 4 * t
     achieved = bytes_step / (ms_step * 1e-3) / 1e9
     cpu = None
@@ -211,12 +258,17 @@ def bench_channelize(rf, args) -> dict:
     return {
         "metric": METRIC, "value": round(n / ms_step / 1e3, 1), "unit": "Msamples/s",
         "ms_per_step": round(ms_step, 5), "dtype": "f32",
-        "config": {"workload": f"polyphase channelizer {n} complex64 samples, M={m}, "
-                               f"{t}-tap Kaiser prototype", "samples": n, "channels": m,
-                   "taps": t, "parallelism": "single-gpu"},
+        "config": {"workload": f"polyphase channelizer {n} complex64 samples (40 QPSK carriers, "
+                               f"-30..0 dB, AWGN -40 dB), M={m}, {t}-tap Kaiser prototype",
+                   "samples": n, "channels": m, "taps": t, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profiles("channelize_fused")},
+        "accuracy": err_fused,
+        "mfma_f16x3": {"ms_per_step": round(ms_mfma, 5),
+                       "msamples_per_s": round(n / ms_mfma / 1e3, 1),
+                       "hbm_frac": round(bytes_step / (ms_mfma * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                       **err_mfma},
         "cpu_baseline": cpu,
     }
 

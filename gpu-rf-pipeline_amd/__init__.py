@@ -116,6 +116,7 @@ _proto("rf_fft_exec", _st, _vp, _vp, _vp, _int, _vp)
 _proto("rf_spectrum", _st, _vp, _vp, _vp, _vp)
 _proto("rf_channelize_frames", _int, _int, _int)
 _proto("rf_channelize", _st, _vp, _vp, _vp, _vp, _int, _int, _int, _vp)
+_proto("rf_channelize_mfma", _st, _vp, _vp, _vp, _int, _int, _int, _vp)
 _proto("rf_xcorr_impl_count", _int)
 _proto("rf_xcorr_impl_name", ctypes.c_char_p, _int)
 _proto("rf_xcorr_impl_summary", ctypes.c_char_p, _int)
@@ -433,6 +434,26 @@ def channelize_device(plan: FftPlan | None, x: int, taps: int, out: int, num_sam
                       channels: int, num_taps: int, stream=None) -> None:
     check(_lib.rf_channelize(plan.handle if plan else None, x, taps, out, num_samples, channels,
                              num_taps, _sp(stream)))
+
+
+def channelize_mfma_device(x: int, taps: int, out: int, num_samples: int, channels: int,
+                           num_taps: int, stream=None) -> None:
+    """Channelizer with the DFT as a split-FP16 matrix-core GEMM (64 channels, <= 1024 taps)."""
+    check(_lib.rf_channelize_mfma(x, taps, out, num_samples, channels, num_taps, _sp(stream)))
+
+
+def channelize_mfma(x, taps, channels: int) -> np.ndarray:
+    """Host-array form of rf_channelize_mfma; flat output, frame after frame."""
+    x = _cf32(x)
+    taps = _f32(taps)
+    frames = channelize_frames(x.size, channels)
+    if frames == 0:
+        return np.zeros(0, dtype=np.complex64)
+    d_x = DeviceBuffer.from_array(x)
+    d_t = DeviceBuffer.from_array(taps)
+    d_z = DeviceBuffer(frames * channels)
+    channelize_mfma_device(d_x.ptr, d_t.ptr, d_z.ptr, x.size, channels, taps.size)
+    return d_z.download()
 
 
 class XcorrFft:

@@ -67,6 +67,11 @@ __host__ __device__ constexpr int mfma_copy_len(int num_taps)
     return ((num_taps + 80 + 7) / 8) * 8;
 }
 
+static_assert(kMfmaOutputs + mfma_kp(kMfmaMaxTaps) - 32 <= kTileLoads * kMfmaThreads,
+              "tile samples per lane");
+static_assert(2 * mfma_copy_len(kMfmaMaxTaps) <= kTapLoads * kMfmaThreads,
+              "tap-table vectors per lane");
+
 template <bool BF16>
 struct Split;
 

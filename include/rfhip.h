@@ -180,6 +180,13 @@ int rf_channelize_frames(int num_samples, int channels);
 rf_status rf_channelize(const rf_fft_plan* plan, const rf_cf32* x, const float* taps,
                         rf_cf32* out, int num_samples, int channels, int num_taps,
                         rf_stream stream);
+/* The same channelizer with the filterbank's DFT on the matrix cores (no reference
+ * counterpart; channelize_device's definition, src/channelize.cu:3-11): FP32 branch sums, then
+ * the 64-point DFT as a split-FP16 GEMM (hi*hi + hi*lo + lo*hi, FP32 accumulation; power-of-two
+ * scaling per 64 frames). |error| <= 2^-15 * sum_l |h[l] x[tM - l]| per output.
+ * channels must be 64 and num_taps <= 1024 (RF_LIMIT otherwise). */
+rf_status rf_channelize_mfma(const rf_cf32* x, const float* taps, rf_cf32* out,
+                             int num_samples, int channels, int num_taps, rf_stream stream);
 
 /* ---------------------------------------------------------------------------------------
  * Cross-correlation (src/xcorr.cu:1-7, tools/reference.py:19-22,114-125):

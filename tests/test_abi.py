@@ -122,3 +122,13 @@ def test_delay_search_arguments() -> None:
     assert L.rf_synth_pairs(1, 0, 4, 16384, None, 8, None, None) == rf.RF_INVALID_ARG
     assert L.rf_delay_search_run(None, 8, 8, None, 0.9, 8, None) == rf.RF_INVALID_ARG
     assert L.rf_delay_mlp_train(None, 1, 16, 16, 16384, 1e-3, None, None) == rf.RF_INVALID_ARG
+
+
+def test_channelize_mfma_and_fir_tune_arguments() -> None:
+    L = rf.lib()
+    assert L.rf_channelize_mfma(8, 8, 8, 4096, 32, 1024, None) == rf.RF_LIMIT
+    assert L.rf_channelize_mfma(8, 8, 8, 4096, 64, 1025, None) == rf.RF_LIMIT
+    assert L.rf_channelize_mfma(8, 8, 8, 4096, 0, 1024, None) == rf.RF_INVALID_ARG
+    assert L.rf_fir_tune(8, 8, 8, 1024, 63, 1 << 3, None, None) == rf.RF_INVALID_ARG
+    cfg = rf.FirConfig(3, 256, 5, 0.0)
+    assert L.rf_fir_run_config(ctypes.byref(cfg), 8, 8, 8, 1024, 63, None) == rf.RF_INVALID_ARG

@@ -440,3 +440,62 @@ def test_channelize_config_shape_64_channels_1024_taps() -> None:
     got = rf.channelize(x, h, 64)
     want = oracle.channelize(x, h, 64).ravel()
     np.testing.assert_allclose(got, want, rtol=CHANNELIZE_RTOL, atol=CHANNELIZE_ATOL)
+
+
+# --- channelizer, DFT on the matrix cores -----------------------------------------------------
+
+def channelize_term_sum(x: np.ndarray, h: np.ndarray, channels: int) -> np.ndarray:
+    """S[t] = sum_l |h[l]| |x[tM - l]|: the scale of frame t's terms (float64)."""
+    conv = np.convolve(np.abs(x.astype(np.complex128)), np.abs(h.astype(np.float64)))
+    frames = -(-x.size // channels)
+    return conv[np.arange(frames) * channels]
+
+
+@pytest.mark.parametrize("name,taps_name", [("tone", "channelizer_proto"), ("tone_odd", "lowpass"),
+                                            ("short", "identity"), ("ch0", "channelizer_proto")])
+def test_channelize_mfma_matches_oracle(name: str, taps_name: str) -> None:
+    x, h = cf32(name), taps(taps_name)
+    got = rf.channelize_mfma(x, h, 64).reshape(-1, 64)
+    want = oracle.channelize(x, h, 64)
+    assert got.shape == want.shape
+    bound = 2.0 ** -15 * channelize_term_sum(x, h, 64)[:, None]
+    assert np.all(np.abs(got - want) <= bound + 1e-30), float(np.max(np.abs(got - want) / (bound + 1e-30)))
+
+
+def test_channelize_mfma_config_shape_and_dynamic_range() -> None:
+    """2^20 samples of the C3 kind (64 channels, 1024-tap Kaiser prototype), QPSK carriers over
+    a 40 dB power range: per-output error within the stated bound, also for weak frames."""
+    rng = np.random.default_rng(0xC3)
+    n = 1 << 20
+    t = np.arange(n)
+    x = np.zeros(n, np.complex128)
+    for k, p_db in zip(rng.choice(64, 8, replace=False), rng.uniform(-40, 0, 8)):
+        sym = (rng.choice([-1, 1], n // 128 + 1) + 1j * rng.choice([-1, 1], n // 128 + 1)) / np.sqrt(2)
+        x += 10 ** (p_db / 20) * np.repeat(sym, 128)[:n] * np.exp(2j * np.pi * k * t / 64)
+    x += 1e-2 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    x = x.astype(np.complex64)
+    m = np.arange(1024) - 511.5
+    h = np.sinc(m / 128) * np.kaiser(1024, 8.0)
+    h = (h / h.sum()).astype(np.float32)
+    got = rf.channelize_mfma(x, h, 64).reshape(-1, 64)
+    frames = np.sort(rng.choice(got.shape[0], 64, replace=False))
+    xs = x.astype(np.complex128)
+    for f in frames:
+        l = np.arange(1024)
+        idx = f * 64 - l
+        ok = idx >= 0
+        terms = np.zeros(1024, np.complex128)
+        terms[ok] = h[ok] * xs[idx[ok]]
+        want = np.array([np.sum(terms * np.exp(2j * np.pi * k * l / 64)) for k in range(64)])
+        bound = 2.0 ** -15 * np.sum(np.abs(terms))
+        assert np.all(np.abs(got[f] - want) <= bound + 1e-30), f
+    fused = rf.channelize(x, h, 64).reshape(-1, 64)
+    np.testing.assert_allclose(got, fused, rtol=0, atol=2e-4 * np.max(np.abs(fused)))
+
+
+def test_channelize_mfma_limits() -> None:
+    x, h = cf32("tone"), taps("lowpass_long")
+    with pytest.raises(rf.RfError):
+        rf.channelize_mfma(x, h, 32)                  # only 64 channels
+    with pytest.raises(rf.RfError):
+        rf.channelize_mfma(x, np.ones(1025, np.float32), 64)   # more than 16 taps per branch
