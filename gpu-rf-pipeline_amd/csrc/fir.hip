@@ -307,27 +307,30 @@ __global__ void fir_register_batched_kernel(FirBatch batch, const float* __restr
 // 256 lanes per segment with the register-resident radix-16 transform of fft4096.hpp: the
 // forward transform reads x straight from HBM, the spectrum multiply sits between the forward
 // and the inverse transform in registers, and the inverse transform writes y straight back.
-// The tap spectrum is made once per call by a 512-lane generic LDS transform.
-constexpr int kOlaLog2 = k4096Log2;
+// The tap spectrum is made once per call by one workgroup with the same transform.
 constexpr int kOlaP = k4096;
-constexpr int kOlaThreads = kOlaP / 8;
 constexpr int kOlaMinTaps = 65;   // at or below this, the direct register kernel is cheaper
 constexpr int kOlaMaxTaps = 2049; // keeps Q >= P / 2
 
-__global__ __launch_bounds__(kOlaThreads) void fir_ola_taps_kernel(
+// Tap spectrum, scaled by 1/P (exact: a power of two): one workgroup, the register-resident
+// radix-16 transform.
+__global__ __launch_bounds__(k4096Lanes) void fir_ola_taps_kernel(
     const float* __restrict__ taps, int num_taps, float2* __restrict__ spec,
-    const float2* __restrict__ table)
+    const float2* __restrict__ stages)
 {
-    __shared__ float2 buf[lds_slots(kOlaP)];
+    __shared__ float2 buf[k4096Slots];
     const int t = threadIdx.x;
-    const float scale = 1.0f / static_cast<float>(kOlaP);  // exact: a power of two
-    for (int i = t; i < kOlaP; i += kOlaThreads) {
-        buf[lds_pad(i)] = make_float2(i < num_taps ? taps[i] * scale : 0.0f, 0.0f);
+    const float scale = 1.0f / static_cast<float>(kOlaP);
+    float2 u[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int i = t + r * k4096Lanes;
+        u[r] = make_float2(i < num_taps ? taps[i] * scale : 0.0f, 0.0f);
     }
-    __syncthreads();
-    lds_fft<false>(buf, kOlaLog2, t, true, table);
-    for (int i = t; i < kOlaP; i += kOlaThreads) {
-        spec[i] = buf[lds_pad(i)];
+    fft4096<false>(buf, t, u, stages);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        spec[t + r * k4096Lanes] = u[r];
     }
 }
 
@@ -559,10 +562,10 @@ bool mfma_applies(int num_taps)
     return num_taps > kOlaMinTaps && num_taps <= kOlaMaxTaps;
 }
 
-void launch_ola_taps(const float* taps, int num_taps, float2* spec, const float2* table,
-                     hipStream_t stream)
+void launch_ola_taps(const float* taps, int num_taps, float2* spec, hipStream_t stream)
 {
-    fir_ola_taps_kernel<<<1, kOlaThreads, 0, stream>>>(taps, num_taps, spec, table);
+    fir_ola_taps_kernel<<<1, k4096Lanes, 0, stream>>>(taps, num_taps, spec,
+                                                      fft4096_stage_table());
     check_launch("fir_ola_taps_kernel");
 }
 
@@ -602,9 +605,8 @@ void fir_launch(int impl, const float2* x, const float* taps, float2* y, int n, 
             launch_register(x, taps, y, n, num_taps, block, stream);
             return;
         }
-        const float2* table = twiddle_table_4096();
         StreamScratch spec(kOlaP * sizeof(float2), stream);
-        launch_ola_taps(taps, num_taps, static_cast<float2*>(spec.ptr), table, stream);
+        launch_ola_taps(taps, num_taps, static_cast<float2*>(spec.ptr), stream);
         const long long q = kOlaP - (num_taps - 1);
         const long long segments = ceil_div(n, q);
         const long long per_xcd = ceil_div(segments, 8);
@@ -665,9 +667,8 @@ void fir_batched_launch(int impl, const FirBatch& batch, int count, int longest,
     case RF_FIR_REGISTER:
     case RF_FIR_FFT_OLA: {
         if (impl == RF_FIR_FFT_OLA && ola_applies(num_taps)) {
-            const float2* table = twiddle_table_4096();
             StreamScratch spec(kOlaP * sizeof(float2), stream);
-            launch_ola_taps(taps, num_taps, static_cast<float2*>(spec.ptr), table, stream);
+            launch_ola_taps(taps, num_taps, static_cast<float2*>(spec.ptr), stream);
             const long long q = kOlaP - (num_taps - 1);
             const dim3 grid(static_cast<unsigned>(ceil_div(longest, q)), count);
             fir_ola_batched_kernel<<<grid, k4096Lanes, 0, stream>>>(
