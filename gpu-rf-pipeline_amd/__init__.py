@@ -114,6 +114,7 @@ _proto("rf_fft_plan_destroy", _st, _vp)
 _proto("rf_fft_plan_bytes", _st, _vp, ctypes.POINTER(_size))
 _proto("rf_fft_exec", _st, _vp, _vp, _vp, _int, _vp)
 _proto("rf_spectrum", _st, _vp, _vp, _vp, _vp)
+_proto("rf_spectrum_magnitude", _st, _vp, _vp, _vp, _vp)
 _proto("rf_channelize_frames", _int, _int, _int)
 _proto("rf_channelize", _st, _vp, _vp, _vp, _vp, _int, _int, _int, _vp)
 _proto("rf_channelize_mfma", _st, _vp, _vp, _vp, _int, _int, _int, _vp)
@@ -624,6 +625,25 @@ def spectrum(x, nfft: int, bench: Bench | None = None) -> np.ndarray:
     synchronize()
     plan.destroy()
     return out[-1]
+
+
+def spectrum_magnitude(x, nfft: int) -> np.ndarray:
+    """|framed forward FFT| in one pass (rf_spectrum_magnitude). Flat float32 output."""
+    x = _cf32(x)
+    if nfft < 1:
+        raise RfError(RF_INVALID_ARG, "nfft must be at least 1")
+    frames = -(-x.size // nfft)
+    if frames == 0:
+        return np.zeros(0, dtype=np.float32)
+    d_in = DeviceBuffer(frames * nfft)
+    d_in.zero()
+    d_in.upload(x)
+    d_mag = DeviceBuffer(frames * nfft, np.float32)
+    plan = FftPlan(nfft, frames)
+    check(_lib.rf_spectrum_magnitude(plan.handle, d_in.ptr, d_mag.ptr, None))
+    out = d_mag.download()
+    plan.destroy()
+    return out
 
 
 def magnitude(x, bench: Bench | None = None) -> np.ndarray:

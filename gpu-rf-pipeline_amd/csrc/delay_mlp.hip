@@ -82,11 +82,39 @@ __device__ __forceinline__ float gelu_grad(float z)
     return cdf + z * pdf;
 }
 
+// Eight consecutive k values of row `row` of a view (k-contiguous when VEC: two 16-byte
+// loads), rows >= rows and depths >= K read as zero without leaving the buffer (indices are
+// clamped, values zeroed by select, no branch per element).
+template <bool VEC>
+__device__ __forceinline__ void load8(const View& v, int row, int rows, int k, int K, float* out)
+{
+    const bool row_ok = row < rows;
+    const int rc = row_ok ? row : rows - 1;
+    if (VEC && k + 8 <= K) {
+        const float4* p = reinterpret_cast<const float4*>(v.p + rc * v.sr + k);
+        const float4 lo = p[0];
+        const float4 hi = p[1];
+        const float z = row_ok ? 1.0f : 0.0f;
+        out[0] = lo.x * z; out[1] = lo.y * z; out[2] = lo.z * z; out[3] = lo.w * z;
+        out[4] = hi.x * z; out[5] = hi.y * z; out[6] = hi.z * z; out[7] = hi.w * z;
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const bool ok = row_ok && k + q < K;
+        const int kc = k + q < K ? k + q : K - 1;
+        const float val = v.p[rc * v.sr + kc * v.sc];
+        out[q] = ok ? val : 0.0f;
+    }
+}
+
 // C[M x N] = A[M x K] B[K x N], one wave per 32 x 32 tile of C, grid (N / 32, ceil(M / 32)).
-// Rows m >= M and depths k >= K read as zero; N is a multiple of 32.
+// Rows m >= M and depths k >= K read as zero; N is a multiple of 32. VEC_A / VEC_B: the
+// operand is contiguous along k (A: sc == 1, B viewed as rows n: sr == 1).
 // MFMA 32x32x16 operand layout: lane l holds A(l % 32, 8 (l / 32) + q) and
 // B(8 (l / 32) + q, l % 32) for q < 8; accumulator v of lane l is C(row, l % 32) with
 // row = (v % 4) + 8 (v / 4) + 4 (l / 32).
+template <bool VEC_A, bool VEC_B>
 __global__ __launch_bounds__(64) void mlp_gemm_kernel(View a, View b, int M, int K, int N,
                                                       int epilogue, const float* __restrict__ bias,
                                                       float* __restrict__ c, float* __restrict__ h)
@@ -97,21 +125,24 @@ __global__ __launch_bounds__(64) void mlp_gemm_kernel(View a, View b, int M, int
     const int i = l & 31;
     const int kq = 8 * (l >> 5);
     const int am = m0 + i;
+    // B as rows n (k along the row): element (k, n) at p[k sr + n sc].
+    const View bt{b.p, b.sc, b.sr};
     f32x16 acc;
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
         acc[v] = 0.0f;
     }
     for (int k0 = 0; k0 < K; k0 += 16) {
+        float x[8];
+        float y[8];
+        load8<VEC_A>(a, am, M, k0 + kq, K, x);
+        load8<VEC_B>(bt, n0 + i, N, k0 + kq, K, y);
         bf16x8 av;
         bf16x8 bv;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            const int k = k0 + kq + q;
-            const float x = (am < M && k < K) ? a.p[am * a.sr + k * a.sc] : 0.0f;
-            const float y = k < K ? b.p[k * b.sr + (n0 + i) * b.sc] : 0.0f;
-            av[q] = static_cast<__bf16>(x);
-            bv[q] = static_cast<__bf16>(y);
+            av[q] = static_cast<__bf16>(x[q]);
+            bv[q] = static_cast<__bf16>(y[q]);
         }
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
     }
@@ -138,7 +169,18 @@ void gemm(View a, View b, int M, int K, int N, int epilogue, const float* bias, 
           float* h, hipStream_t stream)
 {
     const dim3 grid(static_cast<unsigned>(N / 32), static_cast<unsigned>(ceil_div(M, 32)));
-    mlp_gemm_kernel<<<grid, 64, 0, stream>>>(a, b, M, K, N, epilogue, bias, c, h);
+    // 16-byte loads need k-contiguous operands at 16-byte aligned rows (strides multiple of 4)
+    const bool va = a.sc == 1 && a.sr % 4 == 0;
+    const bool vb = b.sr == 1 && b.sc % 4 == 0;
+    if (va && vb) {
+        mlp_gemm_kernel<true, true><<<grid, 64, 0, stream>>>(a, b, M, K, N, epilogue, bias, c, h);
+    } else if (va) {
+        mlp_gemm_kernel<true, false><<<grid, 64, 0, stream>>>(a, b, M, K, N, epilogue, bias, c, h);
+    } else if (vb) {
+        mlp_gemm_kernel<false, true><<<grid, 64, 0, stream>>>(a, b, M, K, N, epilogue, bias, c, h);
+    } else {
+        mlp_gemm_kernel<false, false><<<grid, 64, 0, stream>>>(a, b, M, K, N, epilogue, bias, c, h);
+    }
     check_launch("mlp_gemm_kernel");
 }
 

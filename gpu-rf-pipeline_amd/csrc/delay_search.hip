@@ -59,6 +59,7 @@ struct rf_delay_search {
     float2* work = nullptr;     // count * 2 L
     float2* tables = nullptr;   // four-step twiddles of L
     float2* partial = nullptr;  // count * n2 * 1024 band sums per row
+    float2* derotate = nullptr; // L values: [row][k1] = exp(+2 pi i (row + n2 k1)(n - 1) / L)
     float* features = nullptr;  // count * 256
     int* coarse = nullptr;      // count
     float* probs = nullptr;     // count * 64
@@ -89,10 +90,21 @@ __global__ void search_bind_kernel(PairDesc* __restrict__ pairs, int count, cons
     }
 }
 
+// Sum over each quad of lanes (4j .. 4j + 3) with DPP quad permutes (xor 1, then xor 2), which
+// stay in the VALU instead of going through the LDS crossbar like a general shuffle.
+__device__ __forceinline__ float quad_sum(float v)
+{
+    v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1,
+                                                            0xF, 0xF, false));  // [1,0,3,2]
+    v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x4E,
+                                                            0xF, 0xF, false));  // [2,3,0,1]
+    return v;
+}
+
 // One 4096-point row per item: W into b's row, de-rotated band sums into `partial`.
 __global__ __launch_bounds__(k4096Lanes) void search_row_kernel(
     const PairDesc* __restrict__ pairs, int total, float2* __restrict__ partial,
-    const float2* __restrict__ stages)
+    const float2* __restrict__ derotate, const float2* __restrict__ stages)
 {
     __shared__ float2 buf[k4096Slots];
     xcd_major_items(total, [&](int item) {
@@ -101,7 +113,6 @@ __global__ __launch_bounds__(k4096Lanes) void search_row_kernel(
         const int p = item >> log2n2;
         const long long row = item & ((1 << log2n2) - 1);
         const PairDesc dp = pairs[p];
-        const FourStep fs = shape_of(dp);
         const long long L = 1LL << dp.log2L;
         const float2* wa = dp.work + (row << k4096Log2);
         float2* wb = dp.work + L + (row << k4096Log2);
@@ -117,20 +128,17 @@ __global__ __launch_bounds__(k4096Lanes) void search_row_kernel(
         __syncthreads();
         fft4096<false>(buf, t, ub, stages);
         const float scale = 1.0f / static_cast<float>(L);
-        const long long rot = dp.num_a - 1;
         float2* out = partial + (static_cast<long long>(p) << log2n2 << 10) + (row << 10);
+        const float2* drow = derotate + (row << k4096Log2);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int k1 = t + r * k4096Lanes;
             const float2 w = weigh(ua[r], ub[r], scale, 1);
             gstore(wb, k1, w);
-            const long long k = row + (static_cast<long long>(k1) << log2n2);
-            float2 v = cmul(w, four_step_twiddle<true>(fs, (k * rot) & (L - 1)));
+            float2 v = cmul(w, drow[k1]);  // exp(+2 pi i k (n-1) / L), k = row + n2 k1
             // band k1 / 4: lanes 4j .. 4j + 3 hold its four bins of this row
-            v.x += __shfl_xor(v.x, 1, 64);
-            v.y += __shfl_xor(v.y, 1, 64);
-            v.x += __shfl_xor(v.x, 2, 64);
-            v.y += __shfl_xor(v.y, 2, 64);
+            v.x = quad_sum(v.x);
+            v.y = quad_sum(v.y);
             if ((t & 3) == 0) {
                 out[k1 >> 2] = v;
             }
@@ -392,7 +400,7 @@ void features_launch(rf_delay_search& h, const float2* a, const float2* b, hipSt
     check_launch("batch_column_fwd_kernel");
     const int rows = h.count << h.fs.log2n2;
     search_row_kernel<<<persistent_grid(rows, 2), k4096Lanes, 0, stream>>>(
-        h.dev, rows, h.partial, fft4096_stage_table());
+        h.dev, rows, h.partial, h.derotate, fft4096_stage_table());
     check_launch("search_row_kernel");
     search_features_kernel<<<static_cast<unsigned>(h.count), 256, 0, stream>>>(
         h.partial, h.fs.log2n2, h.features, h.coarse, table);
@@ -448,6 +456,7 @@ void destroy_search(rf_delay_search* h)
     (void)hipFree(h->work);
     (void)hipFree(h->tables);
     (void)hipFree(h->partial);
+    (void)hipFree(h->derotate);
     (void)hipFree(h->features);
     (void)hipFree(h->coarse);
     (void)hipFree(h->probs);
@@ -484,6 +493,24 @@ rf_delay_search* create_search(int count, int n)
         const long long work = 2LL * L * count;
         RF_HIP_CHECK(hipMalloc(&h->work, work * sizeof(float2)));
         const int n2 = 1 << h->fs.log2n2;
+        {
+            // de-rotation of the row pass, in its storage order, float64 on the host
+            std::vector<float2> host(static_cast<std::size_t>(L));
+            const long long rot = n - 1;
+            const double two_pi = 6.283185307179586476925286766559;
+            for (int row = 0; row < n2; ++row) {
+                for (int k1 = 0; k1 < kTwiddleSize; ++k1) {
+                    const long long k = row + static_cast<long long>(n2) * k1;
+                    const long long m = (k % L) * (rot % L) % L;  // exact in 64 bits: L <= 2^20
+                    const double a = two_pi * static_cast<double>(m) / static_cast<double>(L);
+                    host[static_cast<std::size_t>(row) * kTwiddleSize + k1] =
+                        make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+                }
+            }
+            RF_HIP_CHECK(hipMalloc(&h->derotate, static_cast<std::size_t>(L) * sizeof(float2)));
+            RF_HIP_CHECK(hipMemcpy(h->derotate, host.data(), host.size() * sizeof(float2),
+                                   hipMemcpyHostToDevice));
+        }
         const long long partial = static_cast<long long>(count) * n2 * kBands;
         RF_HIP_CHECK(hipMalloc(&h->partial, partial * sizeof(float2)));
         RF_HIP_CHECK(hipMalloc(&h->features, static_cast<std::size_t>(count) * kFeatures * sizeof(float)));
@@ -518,7 +545,7 @@ rf_delay_search* create_search(int count, int n)
         RF_HIP_CHECK(hipMalloc(&h->col_map, col_map.size() * sizeof(int2)));
         RF_HIP_CHECK(hipMemcpy(h->col_map, col_map.data(), col_map.size() * sizeof(int2),
                                hipMemcpyHostToDevice));
-        h->bytes = (tsize + work + partial) * sizeof(float2)
+        h->bytes = (tsize + work + partial + L) * sizeof(float2)
                    + static_cast<std::size_t>(count) * (kFeatures + kBins + 2) * sizeof(float)
                    + h->host.size() * sizeof(PairDesc) + col_map.size() * sizeof(int2);
     } catch (...) {

@@ -3,8 +3,10 @@
 // transforms; see fft_core.hpp and fft_passes.hpp for the method.
 
 #include "fft_plan.hpp"
+#include "fir_internal.hpp"
 #include "twiddle.hpp"
 
+#include <cstdint>
 #include <string>
 #include <vector>
 
@@ -126,6 +128,35 @@ __global__ __launch_bounds__(512) void four_step_column_kernel(const float2* __r
         [=](long long j1, int j2) { return src[j1 + (static_cast<long long>(j2) << log2n1)]; },
         [=](long long j1, int k2, float2 v) { dst[j1 + (static_cast<long long>(k2) << log2n1)] = v; },
         lds, table);
+}
+
+// The same batched transform with |X| = sqrtf(re^2 + im^2) as the store epilogue (the
+// magnitude kernel's formula, so the result equals spectrum-then-magnitude bit for bit).
+__global__ __launch_bounds__(512) void fft_lds_batch_magnitude_kernel(
+    const float2* in, float* out, int log2P, long long batch, const float2* __restrict__ table)
+{
+    extern __shared__ float2 lds[];
+    const int P = 1 << log2P;
+    const int F = lds_batch_per_block(log2P);
+    const int slots = lds_slots(P);
+    const long long first = static_cast<long long>(blockIdx.x) * F;
+    const int count = static_cast<int>(batch - first < F ? batch - first : F);
+    const long long base = first * P;
+    const int total = count * P;
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        const int f = e >> log2P;
+        lds[f * slots + lds_pad(e & (P - 1))] = in[base + e];
+    }
+    __syncthreads();
+    const int per = P >> 3;
+    const int f = threadIdx.x / per;
+    const int t = threadIdx.x - f * per;
+    lds_fft<false>(lds + (f < count ? f : 0) * slots, log2P, t, f < count, table);
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        const int fe = e >> log2P;
+        const float2 v = lds[fe * slots + lds_pad(e & (P - 1))];
+        out[base + e] = sqrtf(v.x * v.x + v.y * v.y);
+    }
 }
 
 std::size_t lds_batch_bytes(int log2P)
@@ -331,6 +362,40 @@ rf_status rf_spectrum(const rf_fft_plan* plan, const rf_cf32* in, rf_cf32* out,
                       rf_stream stream)
 {
     return rf_fft_exec(plan, in, out, RF_FFT_FORWARD, stream);
+}
+
+rf_status rf_spectrum_magnitude(const rf_fft_plan* plan, const rf_cf32* in, float* mag,
+                                rf_stream stream)
+{
+    return guarded([&] {
+        if (plan == nullptr || in == nullptr || mag == nullptr) {
+            rf::fail(RF_INVALID_ARG, "spectrum magnitude: NULL plan or buffer");
+        }
+        if (plan->batch == 0) {
+            return;
+        }
+        hipStream_t s = rf::as_stream(stream);
+        const long long total = static_cast<long long>(plan->batch) * plan->nfft;
+        if (plan->kind == rf::kPlanLds) {
+            const int F = rf::lds_batch_per_block(plan->log2n);
+            const int threads = F * (plan->nfft >> 3);
+            const unsigned grid = static_cast<unsigned>(rf::ceil_div(plan->batch, F));
+            rf::fft_lds_batch_magnitude_kernel<<<grid, threads, rf::lds_batch_bytes(plan->log2n),
+                                                 s>>>(reinterpret_cast<const float2*>(in), mag,
+                                                      plan->log2n, plan->batch,
+                                                      rf::twiddle_table_4096());
+            rf::check_launch("fft_lds_batch_magnitude_kernel");
+            return;
+        }
+        // Other lengths: the spectrum into stream-ordered scratch, then the magnitude pass.
+        rf::StreamScratch spec(static_cast<std::size_t>(total) * sizeof(float2), s);
+        rf::fft_exec(*plan, reinterpret_cast<const float2*>(in), static_cast<float2*>(spec.ptr),
+                     RF_FFT_FORWARD, s);
+        if (total > INT32_MAX) rf::fail(RF_LIMIT, "spectrum magnitude: more than 2^31 values");
+        const rf_status st = rf_magnitude(static_cast<const rf_cf32*>(spec.ptr), mag,
+                                          static_cast<int>(total), stream);
+        if (st != RF_OK) rf::fail(st, rf_last_error());
+    });
 }
 
 }  // extern "C"

@@ -167,6 +167,12 @@ rf_status rf_fft_exec(const rf_fft_plan* plan, const rf_cf32* in, rf_cf32* out, 
  * zero-padded to batch * nfft samples) to `out`. */
 rf_status rf_spectrum(const rf_fft_plan* plan, const rf_cf32* in, rf_cf32* out,
                       rf_stream stream);
+/* Spectrum then magnitude in one pass (no reference counterpart; the fusion deferred in
+ * docs/ROADMAP.md:57-59 of the reference): mag (DEVICE, nfft * batch floats) = |rf_spectrum|,
+ * sqrtf(re^2 + im^2) as rf_magnitude computes it. Plans with 8 <= nfft <= 4096 run as one
+ * kernel whose store writes the magnitude; other lengths go through stream-ordered scratch. */
+rf_status rf_spectrum_magnitude(const rf_fft_plan* plan, const rf_cf32* in, float* mag,
+                                rf_stream stream);
 
 /* ---------------------------------------------------------------------------------------
  * Polyphase channelizer (src/channelize.cu:1-72, tools/reference.py:24-30,128-146):

@@ -132,3 +132,7 @@ def test_channelize_mfma_and_fir_tune_arguments() -> None:
     assert L.rf_fir_tune(8, 8, 8, 1024, 63, 1 << 3, None, None) == rf.RF_INVALID_ARG
     cfg = rf.FirConfig(3, 256, 5, 0.0)
     assert L.rf_fir_run_config(ctypes.byref(cfg), 8, 8, 8, 1024, 63, None) == rf.RF_INVALID_ARG
+
+
+def test_spectrum_magnitude_arguments() -> None:
+    assert rf.lib().rf_spectrum_magnitude(None, 8, 8, None) == rf.RF_INVALID_ARG

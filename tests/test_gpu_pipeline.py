@@ -95,6 +95,19 @@ def test_spectrum_any_length_matches_oracle(nfft: int) -> None:
     np.testing.assert_allclose(got, want, rtol=SPECTRUM_RTOL, atol=SPECTRUM_ATOL * scale)
 
 
+@pytest.mark.parametrize("nfft", [1, 8, 64, 100, 1024, 4096, 16384])
+def test_spectrum_magnitude_fusion(nfft: int) -> None:
+    """One-pass |spectrum| equals spectrum then magnitude bit for bit, and the oracle's |X|
+    within the spectrum tolerances."""
+    x = cf32("tone")
+    fused = rf.spectrum_magnitude(x, nfft)
+    two_pass = rf.magnitude(rf.spectrum(x, nfft))
+    assert fused.tobytes() == two_pass.tobytes()
+    want = np.abs(oracle.spectrum(x, nfft).ravel())
+    scale = max(1.0, np.sqrt(nfft / 1024))
+    np.testing.assert_allclose(fused, want, rtol=SPECTRUM_RTOL, atol=SPECTRUM_ATOL * scale)
+
+
 def test_fft_inverse_round_trip() -> None:
     for nfft, batch in [(4096, 3), (1 << 15, 2), (1 << 18, 1), (48, 5)]:
         x = noise(nfft, nfft * batch)
