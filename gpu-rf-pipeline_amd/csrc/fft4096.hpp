@@ -7,9 +7,13 @@
 // operation between two transforms without touching LDS, and write its outputs straight back;
 // each transform makes two LDS round trips. LDS slot of element j: j + j/16, which puts the 16
 // lanes of each stride-16 write and of each (t/16)*256 + t%16 write on 16 different banks.
+// Every LDS access of a pass is one per-lane base plus a compile-time offset (slot(t + 256 r)
+// = slot(t) + 272 r, slot(16 t + r) = 17 t + r for r < 16, slot(j + 16 r) = slot(j) + 17 r),
+// so a pass keeps one address register instead of sixteen.
 // Twiddles: per-device table [span][k][r] = exp(-2 pi i r k / (16 span)), r < 16, k < span,
-// span 16 at offset 0 and span 256 at offset 256 (fft4096_stage_table()); a lane reads its 16
-// factors as 8 16-byte loads.
+// span 16 at offset 0 and span 256 at offset 256 (fft4096_stage_table()); a lane needs the 15
+// factors r = 1..15 of row k = t % 16 of the span-16 table and of row k = t of the span-256
+// table.
 #pragma once
 
 #include "fft_core.hpp"
@@ -26,19 +30,45 @@ __device__ __forceinline__ int pad16(int j)
     return j + (j >> 4);
 }
 
-template <bool INV>
-__device__ __forceinline__ void twiddle_dft16(float2* u, int k, const float2* stage)
+// Factors 1..15 of a 16-entry table row as seven 16-byte loads and one 8-byte load: entry 0
+// (= 1) is not fetched, so no load writes a register that nothing reads (such a load would
+// make the next write of that register wait for it). Global-address loads, also when the
+// caller's pointer has lost its address space.
+__device__ __forceinline__ void load_factors(const float2* row, float2* w)
 {
-    const float4* tw = reinterpret_cast<const float4*>(stage + 16 * k);
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(1))) const f32x4 global_f32x4;
+    typedef __attribute__((address_space(1))) const double global_f64;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const float4 w = tw[q];
-        if (q > 0) {
-            u[2 * q] = INV ? cmulconj(u[2 * q], make_float2(w.x, w.y))
-                           : cmul(u[2 * q], make_float2(w.x, w.y));
-        }
-        u[2 * q + 1] = INV ? cmulconj(u[2 * q + 1], make_float2(w.z, w.w))
-                           : cmul(u[2 * q + 1], make_float2(w.z, w.w));
+    for (int q = 0; q < 7; ++q) {
+        const f32x4 v = *(global_f32x4*)(row + 1 + 2 * q);  // address-space casts
+        w[1 + 2 * q] = make_float2(v.x, v.y);
+        w[2 + 2 * q] = make_float2(v.z, v.w);
+    }
+    w[15] = __builtin_bit_cast(float2, *(global_f64*)(row + 15));
+}
+
+// The 30 factors one lane applies in a transform. Loading them before a transform lets their
+// latency overlap the caller's own loads, and one load serves a forward and an inverse
+// transform (the inverse applies the conjugates).
+struct Fft4096Twiddles {
+    float2 w16[16];   // span-16 pass: exp(-2 pi i r (t % 16) / 256); entry 0 unused
+    float2 w256[16];  // span-256 pass: exp(-2 pi i r t / 4096); entry 0 unused
+};
+
+__device__ __forceinline__ void fft4096_load_twiddles(int t, const float2* stages,
+                                                      Fft4096Twiddles& tw)
+{
+    load_factors(stages + 16 * (t & 15), tw.w16);
+    load_factors(stages + 256 + 16 * t, tw.w256);
+}
+
+template <bool INV>
+__device__ __forceinline__ void twiddled_dft16(float2* u, const float2* w)
+{
+#pragma unroll
+    for (int r = 1; r < 16; ++r) {
+        u[r] = INV ? cmulconj(u[r], w[r]) : cmul(u[r], w[r]);
     }
     dft16<INV>(u);
 }
@@ -48,29 +78,31 @@ template <bool INV>
 __device__ __forceinline__ void fft4096_first(float2* buf, int t, float2* u)
 {
     dft16<INV>(u);
+    float2* dst = buf + 17 * t;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        buf[pad16(16 * t + r)] = u[r];
+        dst[r] = u[r];
     }
     __syncthreads();
 }
 
-// Span-16 pass through LDS.
+// Span-16 pass through LDS with the lane's factors in registers.
 template <bool INV>
-__device__ __forceinline__ void fft4096_middle(float2* buf, int t, const float2* stages)
+__device__ __forceinline__ void fft4096_middle(float2* buf, int t, const float2* w16)
 {
     float2 u[16];
+    const float2* src = buf + pad16(t);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        u[r] = buf[pad16(t + r * k4096Lanes)];
+        u[r] = src[272 * r];
     }
     __syncthreads();
+    twiddled_dft16<INV>(u, w16);
     const int k = t & 15;
-    twiddle_dft16<INV>(u, k, stages);
-    const int j = (t - k) * 16 + k;
+    float2* dst = buf + pad16((t - k) * 16 + k);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        buf[pad16(j + 16 * r)] = u[r];
+        dst[17 * r] = u[r];
     }
     __syncthreads();
 }
@@ -78,24 +110,34 @@ __device__ __forceinline__ void fft4096_middle(float2* buf, int t, const float2*
 // Span-256 pass from LDS; outputs t + 256 r stay in registers.
 template <bool INV>
 __device__ __forceinline__ void fft4096_last(const float2* buf, int t, float2* u,
-                                             const float2* stages)
+                                             const float2* w256)
 {
+    const float2* src = buf + pad16(t);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        u[r] = buf[pad16(t + r * k4096Lanes)];
+        u[r] = src[272 * r];
     }
-    twiddle_dft16<INV>(u, t, stages + 256);
+    twiddled_dft16<INV>(u, w256);
 }
 
-// The whole transform, unnormalized: u[r] = x[t + 256 r] in, X[t + 256 r] out. `buf` holds
-// k4096Slots values; the caller separates two uses of the same buffer with a barrier once
-// every lane has read what the previous transform left there.
+// The whole transform, unnormalized, with preloaded factors: u[r] = x[t + 256 r] in,
+// X[t + 256 r] out. `buf` holds k4096Slots values; the caller separates two uses of the same
+// buffer with a barrier once every lane has read what the previous transform left there.
+template <bool INV>
+__device__ __forceinline__ void fft4096(float2* buf, int t, float2* u, const Fft4096Twiddles& tw)
+{
+    fft4096_first<INV>(buf, t, u);
+    fft4096_middle<INV>(buf, t, tw.w16);
+    fft4096_last<INV>(buf, t, u, tw.w256);
+}
+
+// The same, loading the factors first (they arrive while the first pass computes).
 template <bool INV>
 __device__ __forceinline__ void fft4096(float2* buf, int t, float2* u, const float2* stages)
 {
-    fft4096_first<INV>(buf, t, u);
-    fft4096_middle<INV>(buf, t, stages);
-    fft4096_last<INV>(buf, t, u, stages);
+    Fft4096Twiddles tw;
+    fft4096_load_twiddles(t, stages, tw);
+    fft4096<INV>(buf, t, u, tw);
 }
 
 }  // namespace rf

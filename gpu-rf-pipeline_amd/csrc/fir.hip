@@ -334,8 +334,33 @@ __global__ __launch_bounds__(k4096Lanes) void fir_ola_taps_kernel(
     }
 }
 
+// Unchecked buffer accesses for blocks known to lie inside their arrays: address = base +
+// soffset + voffset, with the workgroup-uniform parts (base, block offset) in scalar registers
+// and one per-lane offset register shared by all 16 blocks of a segment, instead of a 64-bit
+// address per access. The record count is the maximum, so the range check never applies.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t unchecked_buffer(const void* base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), static_cast<short>(0), -1,
+                                             0x00020000);
+}
+
+__device__ __forceinline__ float2 buffer_load2(__amdgpu_buffer_rsrc_t r, unsigned voffset,
+                                               unsigned soffset)
+{
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voffset, soffset, 0));
+}
+
+__device__ __forceinline__ void buffer_store2(float2 v, __amdgpu_buffer_rsrc_t r,
+                                              unsigned voffset, unsigned soffset)
+{
+    using u32x2 = unsigned __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voffset, soffset, 0);
+}
+
 // One segment: seg[i] = x[in0 + i], i < 4096 (zero outside the input), filtered by the
-// spectrum, outputs y[out0 + i - (T-1)] for T-1 <= i < 4096.
+// spectrum, outputs y[out0 + i - (T-1)] for T-1 <= i < 4096. Lane t holds positions t + 256 r.
+// The twiddle factors are loaded after the samples, so they arrive while the first pass runs,
+// and serve both transforms.
 __device__ __forceinline__ void ola_segment(const float2* __restrict__ x,
                                             const float2* __restrict__ spec,
                                             float2* __restrict__ y, int n, int halo,
@@ -344,12 +369,14 @@ __device__ __forceinline__ void ola_segment(const float2* __restrict__ x,
 {
     const int t = threadIdx.x;
     const long long in0 = out0 - halo;
+    const unsigned lane_bytes = static_cast<unsigned>(t) * 8u;
+    constexpr unsigned kBlockBytes = k4096Lanes * 8u;
     float2 u[16];
     if (in0 >= 0 && in0 + kOlaP <= n) {
-        const float2* src = x + in0 + t;
+        const __amdgpu_buffer_rsrc_t src = unchecked_buffer(x + in0);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            u[r] = src[r * k4096Lanes];
+            u[r] = buffer_load2(src, lane_bytes, r * kBlockBytes);
         }
     } else {
 #pragma unroll
@@ -357,19 +384,27 @@ __device__ __forceinline__ void ola_segment(const float2* __restrict__ x,
             u[r] = load_sample(x, in0 + t + r * k4096Lanes, n);
         }
     }
-    fft4096<false>(buf, t, u, stages);
+    Fft4096Twiddles tw;
+    fft4096_load_twiddles(t, stages, tw);
+    fft4096<false>(buf, t, u, tw);
+    const __amdgpu_buffer_rsrc_t h = unchecked_buffer(spec);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        u[r] = cmul(u[r], spec[t + r * k4096Lanes]);
+        u[r] = cmul(u[r], buffer_load2(h, lane_bytes, r * kBlockBytes));
     }
     __syncthreads();  // every lane has read the forward result before it is overwritten
-    fft4096<true>(buf, t, u, stages);
+    fft4096<true>(buf, t, u, tw);
     if (out0 + kOlaP - halo <= n) {
+        // Position p = t + 256 r goes to y[out0 + p - halo] when p >= halo; a block wholly at or
+        // above the halo stores without a per-lane test.
+        const __amdgpu_buffer_rsrc_t dst = unchecked_buffer(y + out0);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const int pos = t + r * k4096Lanes;
-            if (pos >= halo) {
-                y[out0 + pos - halo] = u[r];
+            const int first = r * k4096Lanes;
+            if (first >= halo) {
+                buffer_store2(u[r], dst, lane_bytes, static_cast<unsigned>(first - halo) * 8u);
+            } else if (first + k4096Lanes > halo && t + first >= halo) {
+                buffer_store2(u[r], dst, static_cast<unsigned>(t + first - halo) * 8u, 0u);
             }
         }
     } else {
@@ -562,6 +597,22 @@ bool mfma_applies(int num_taps)
     return num_taps > kOlaMinTaps && num_taps <= kOlaMaxTaps;
 }
 
+// Dynamic LDS that holds the overlap-save kernels at three resident workgroups per CU. The
+// kernel needs 35 KiB of LDS and 126 registers, so four would fit; measured on MI355X
+// (2^24 samples, 1025 taps) three per CU run the whole launch about 5 % faster than four,
+// which contend for the same CU's VALU and memory pipes without hiding more latency. The
+// reserve is unused memory: each workgroup asks for a quarter of the CU's LDS plus 4 KiB.
+std::size_t ola_lds_reserve()
+{
+    const long long per_cu = lds_per_cu();
+    const long long used = static_cast<long long>(k4096Slots) * sizeof(float2);
+    const long long total = per_cu / 4 + 4096;
+    if (total <= used || 3 * total > per_cu || total > max_lds_per_block()) {
+        return 0;
+    }
+    return static_cast<std::size_t>(total - used);
+}
+
 void launch_ola_taps(const float* taps, int num_taps, float2* spec, hipStream_t stream)
 {
     fir_ola_taps_kernel<<<1, k4096Lanes, 0, stream>>>(taps, num_taps, spec,
@@ -610,7 +661,8 @@ void fir_launch(int impl, const float2* x, const float* taps, float2* y, int n, 
         const long long q = kOlaP - (num_taps - 1);
         const long long segments = ceil_div(n, q);
         const long long per_xcd = ceil_div(segments, 8);
-        fir_ola_kernel<<<static_cast<unsigned>(8 * per_xcd), k4096Lanes, 0, stream>>>(
+        fir_ola_kernel<<<static_cast<unsigned>(8 * per_xcd), k4096Lanes, ola_lds_reserve(),
+                         stream>>>(
             x, static_cast<const float2*>(spec.ptr), y, n, num_taps, segments, per_xcd,
             fft4096_stage_table());
         check_launch("fir_ola_kernel");
@@ -671,7 +723,7 @@ void fir_batched_launch(int impl, const FirBatch& batch, int count, int longest,
             launch_ola_taps(taps, num_taps, static_cast<float2*>(spec.ptr), stream);
             const long long q = kOlaP - (num_taps - 1);
             const dim3 grid(static_cast<unsigned>(ceil_div(longest, q)), count);
-            fir_ola_batched_kernel<<<grid, k4096Lanes, 0, stream>>>(
+            fir_ola_batched_kernel<<<grid, k4096Lanes, ola_lds_reserve(), stream>>>(
                 batch, static_cast<const float2*>(spec.ptr), num_taps, fft4096_stage_table());
             check_launch("fir_ola_batched_kernel");
             return;

@@ -77,6 +77,8 @@ inline void check_launch(const char* what)
 
 // LDS bytes one workgroup may use on the current device.
 int max_lds_per_block();
+// LDS bytes of one compute unit of the current device.
+int lds_per_cu();
 // Compute units of the current device.
 int cu_count();
 // Enqueues the library's streaming device-to-device copy.

@@ -34,6 +34,16 @@ int max_lds_per_block()
     return bytes;
 }
 
+int lds_per_cu()
+{
+    int device = 0;
+    RF_HIP_CHECK(hipGetDevice(&device));
+    int bytes = 0;
+    RF_HIP_CHECK(hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                       device));
+    return bytes;
+}
+
 int cu_count()
 {
     int device = 0;

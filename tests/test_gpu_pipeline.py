@@ -236,6 +236,27 @@ def test_fir_long_random_filter(impl: str) -> None:
     assert_fir_close(impl, rf.fir(x, h, impl), x, h)
 
 
+@pytest.mark.parametrize("num_taps", [66, 300, 777, 1500])
+def test_fir_ola_uneven_halo_and_lengths(num_taps: int) -> None:
+    # Overlap-save segments of 4096 keep 4097 - T outputs; a halo T - 1 that is not a multiple
+    # of the 256-position blocks splits one block between discarded and kept positions, and the
+    # first segment reads before x[0]. Lengths: shorter than the halo, one segment plus a few,
+    # and many segments.
+    h = (np.random.default_rng(num_taps).standard_normal(num_taps) / 20).astype(np.float32)
+    lens = [1, num_taps - 1, 4097 - num_taps + 5, 20011]
+    for i, n in enumerate(lens):
+        x = noise(300 + i, n)
+        assert_fir_close("fft_ola", rf.fir(x, h, "fft_ola"), x, h)
+    xs = [noise(400 + i, n) for i, n in enumerate(lens)]
+    d_x = [rf.DeviceBuffer.from_array(x) for x in xs]
+    d_y = [rf.DeviceBuffer(n) for n in lens]
+    d_h = rf.DeviceBuffer.from_array(h)
+    rf.fir_batched_device(rf.fir_impl_index("fft_ola"), [b.ptr for b in d_x],
+                          [b.ptr for b in d_y], lens, d_h.ptr, h.size)
+    for x, y in zip(xs, d_y):
+        assert_fir_close("fft_ola", y.download(), x, h)
+
+
 @pytest.mark.parametrize("impl", ["mfma_f16x3", "mfma_bf16x3"])
 def test_fir_wide_dynamic_range(impl: str) -> None:
     # Samples of 1e-6 in the first half and 1e4 in the second, taps from 1e-9 to 1e-1: the FP16
