@@ -109,6 +109,9 @@ class FirConfig(ctypes.Structure):
 _proto("rf_fir_tune", _st, _vp, _vp, _vp, _int, _int, ctypes.c_uint, _vp,
        ctypes.POINTER(FirConfig))
 _proto("rf_fir_run_config", _st, ctypes.POINTER(FirConfig), _vp, _vp, _vp, _int, _int, _vp)
+_proto("rf_fir_plan_create", _st, _int, _vp, _int, _int, _vp, ctypes.POINTER(_vp))
+_proto("rf_fir_plan_run", _st, _vp, _vp, _vp, _int, _vp)
+_proto("rf_fir_plan_destroy", _st, _vp)
 _proto("rf_fft_plan_create", _st, _int, _int, ctypes.POINTER(_vp))
 _proto("rf_fft_plan_destroy", _st, _vp)
 _proto("rf_fft_plan_bytes", _st, _vp, ctypes.POINTER(_size))
@@ -759,6 +762,34 @@ def fir_tune(x: int, taps: int, y: int, n: int, num_taps: int, impls: Sequence[s
 def fir_run_config(config: FirConfig, x: int, taps: int, y: int, n: int, num_taps: int,
                    stream=None) -> None:
     check(_lib.rf_fir_run_config(ctypes.byref(config), x, taps, y, n, num_taps, _sp(stream)))
+
+
+class FirPlan:
+    """A filter prepared once (device taps copied; fft_ola: the tap spectrum made) for many
+    rf_fir_plan_run calls, each the same bits as rf_fir with that variant and block."""
+
+    def __init__(self, impl: str | int, taps: int, num_taps: int, block: int = 0, stream=None):
+        idx = fir_impl_index(impl) if isinstance(impl, str) else impl
+        p = _vp()
+        check(_lib.rf_fir_plan_create(idx, taps, num_taps, block, _sp(stream), ctypes.byref(p)))
+        self.handle = p.value
+        self.num_taps = num_taps
+
+    def run(self, x: int, y: int, n: int, stream=None) -> None:
+        check(_lib.rf_fir_plan_run(self.handle, x, y, n, _sp(stream)))
+
+    def destroy(self) -> None:
+        if self.handle:
+            check(_lib.rf_fir_plan_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.rf_fir_plan_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
 
 
 DELAY_FEATURES = 256

@@ -350,11 +350,13 @@ __device__ __forceinline__ float2 buffer_load2(__amdgpu_buffer_rsrc_t r, unsigne
     return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voffset, soffset, 0));
 }
 
+// Non-temporal (cache policy bit 1, "nt"): the outputs are streamed out and never re-read by
+// the kernel, so they should not displace the input halos the next segments read from L2.
 __device__ __forceinline__ void buffer_store2(float2 v, __amdgpu_buffer_rsrc_t r,
                                               unsigned voffset, unsigned soffset)
 {
     using u32x2 = unsigned __attribute__((ext_vector_type(2)));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voffset, soffset, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voffset, soffset, 2);
 }
 
 // One segment: seg[i] = x[in0 + i], i < 4096 (zero outside the input), filtered by the
@@ -620,6 +622,18 @@ void launch_ola_taps(const float* taps, int num_taps, float2* spec, hipStream_t 
     check_launch("fir_ola_taps_kernel");
 }
 
+// The overlap-save filtering kernel over x with a ready tap spectrum.
+void launch_ola(const float2* x, const float2* spec, float2* y, int n, int num_taps,
+                hipStream_t stream)
+{
+    const long long q = kOlaP - (num_taps - 1);
+    const long long segments = ceil_div(n, q);
+    const long long per_xcd = ceil_div(segments, 8);
+    fir_ola_kernel<<<static_cast<unsigned>(8 * per_xcd), k4096Lanes, ola_lds_reserve(), stream>>>(
+        x, spec, y, n, num_taps, segments, per_xcd, fft4096_stage_table());
+    check_launch("fir_ola_kernel");
+}
+
 void fir_launch(int impl, const float2* x, const float* taps, float2* y, int n, int num_taps,
                 int block, hipStream_t stream)
 {
@@ -658,14 +672,7 @@ void fir_launch(int impl, const float2* x, const float* taps, float2* y, int n, 
         }
         StreamScratch spec(kOlaP * sizeof(float2), stream);
         launch_ola_taps(taps, num_taps, static_cast<float2*>(spec.ptr), stream);
-        const long long q = kOlaP - (num_taps - 1);
-        const long long segments = ceil_div(n, q);
-        const long long per_xcd = ceil_div(segments, 8);
-        fir_ola_kernel<<<static_cast<unsigned>(8 * per_xcd), k4096Lanes, ola_lds_reserve(),
-                         stream>>>(
-            x, static_cast<const float2*>(spec.ptr), y, n, num_taps, segments, per_xcd,
-            fft4096_stage_table());
-        check_launch("fir_ola_kernel");
+        launch_ola(x, static_cast<const float2*>(spec.ptr), y, n, num_taps, stream);
         return;
     }
     case RF_FIR_MFMA_F16X3:
@@ -931,6 +938,104 @@ rf_status rf_fir_batched(int impl, const rf_cf32* const* x, rf_cf32* const* y,
         rf::fir_batched_launch(impl, batch, count, longest, taps, num_taps,
                                rf::as_stream(stream));
     });
+}
+
+}  // extern "C"
+
+// A filter prepared once for many calls: device copies of the taps and, when overlap-save
+// applies, their 4096-point spectrum (made by the same kernel rf_fir runs per call, so the
+// results are the same bits).
+struct rf_fir_plan {
+    int impl = 0;
+    int num_taps = 0;
+    int block = RF_DEFAULT_FIR_BLOCK;
+    float* taps = nullptr;
+    float2* spec = nullptr;
+};
+
+namespace {
+
+void destroy_fir_plan(rf_fir_plan* plan)
+{
+    if (plan == nullptr) {
+        return;
+    }
+    if (plan->taps != nullptr) {
+        (void)hipFree(plan->taps);
+    }
+    if (plan->spec != nullptr) {
+        (void)hipFree(plan->spec);
+    }
+    delete plan;
+}
+
+}  // namespace
+
+extern "C" {
+
+rf_status rf_fir_plan_create(int impl, const float* taps, int num_taps, int block,
+                             rf_stream stream, rf_fir_plan** out)
+{
+    return guarded([&] {
+        if (out == nullptr) rf::fail(RF_INVALID_ARG, "fir plan: NULL output handle");
+        *out = nullptr;
+        const int b = block == 0 ? RF_DEFAULT_FIR_BLOCK : block;
+        // The sample buffers come with each run; check the filter half now.
+        static const float2 probe{};
+        rf::check_fir_args(impl, &probe, taps, &probe, 1, num_taps, b);
+        if (impl == RF_FIR_TILED_CONST || impl == RF_FIR_REGISTER) {
+            rf::check_const_taps(num_taps, impl == RF_FIR_REGISTER ? "register" : "tiled_const");
+        }
+        if (impl == RF_FIR_TILED || impl == RF_FIR_TILED_CONST) {
+            (void)rf::tiled_bytes(num_taps, b, impl == RF_FIR_TILED ? "tiled" : "tiled_const");
+        }
+        hipStream_t s = rf::as_stream(stream);
+        rf_fir_plan* plan = new rf_fir_plan;
+        plan->impl = impl;
+        plan->num_taps = num_taps;
+        plan->block = b;
+        try {
+            RF_HIP_CHECK(hipMalloc(&plan->taps, sizeof(float) * num_taps));
+            RF_HIP_CHECK(hipMemcpyAsync(plan->taps, taps, sizeof(float) * num_taps,
+                                        hipMemcpyDeviceToDevice, s));
+            if (impl == RF_FIR_FFT_OLA && rf::ola_applies(num_taps)) {
+                RF_HIP_CHECK(hipMalloc(&plan->spec, sizeof(float2) * rf::kOlaP));
+                rf::launch_ola_taps(plan->taps, num_taps, plan->spec, s);
+            }
+            // Ready for any stream once create returns.
+            RF_HIP_CHECK(hipStreamSynchronize(s));
+        } catch (...) {
+            destroy_fir_plan(plan);
+            throw;
+        }
+        *out = plan;
+    });
+}
+
+rf_status rf_fir_plan_run(const rf_fir_plan* plan, const rf_cf32* x, rf_cf32* y,
+                          int num_samples, rf_stream stream)
+{
+    return guarded([&] {
+        if (plan == nullptr) rf::fail(RF_INVALID_ARG, "fir plan: NULL plan");
+        rf::check_fir_args(plan->impl, x, plan->taps, y, num_samples, plan->num_taps,
+                           plan->block);
+        if (num_samples == 0) {
+            return;
+        }
+        const float2* xs = reinterpret_cast<const float2*>(x);
+        float2* ys = reinterpret_cast<float2*>(y);
+        if (plan->spec != nullptr) {
+            rf::launch_ola(xs, plan->spec, ys, num_samples, plan->num_taps, rf::as_stream(stream));
+        } else {
+            rf::fir_launch(plan->impl, xs, plan->taps, ys, num_samples, plan->num_taps,
+                           plan->block, rf::as_stream(stream));
+        }
+    });
+}
+
+rf_status rf_fir_plan_destroy(rf_fir_plan* plan)
+{
+    return guarded([&] { destroy_fir_plan(plan); });
 }
 
 }  // extern "C"

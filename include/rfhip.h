@@ -141,6 +141,19 @@ rf_status rf_fir_tune(const rf_cf32* x, const float* taps, rf_cf32* y, int num_s
 rf_status rf_fir_run_config(const rf_fir_config* config, const rf_cf32* x, const float* taps,
                             rf_cf32* y, int num_samples, int num_taps, rf_stream stream);
 
+/* Prepared filter. No counterpart in the reference, whose host wrappers upload the taps on
+ * every call (src/fir.cu:62-85): for one filter applied to many blocks, create copies the
+ * device taps into memory the plan owns and, for fft_ola, makes their 4096-point spectrum once
+ * (then synchronises `stream`), so each run launches only the filtering kernel. Arguments and
+ * errors as rf_fir (impl, taps, num_taps, block checked at create); a run gives the same bits
+ * as rf_fir with that impl and block. Runs on one plan may overlap on different streams. */
+typedef struct rf_fir_plan rf_fir_plan;
+rf_status rf_fir_plan_create(int impl, const float* taps, int num_taps, int block,
+                             rf_stream stream, rf_fir_plan** plan);
+rf_status rf_fir_plan_run(const rf_fir_plan* plan, const rf_cf32* x, rf_cf32* y,
+                          int num_samples, rf_stream stream);
+rf_status rf_fir_plan_destroy(rf_fir_plan* plan);
+
 /* ---------------------------------------------------------------------------------------
  * Magnitude: mag[n] = |x[n]| (src/magnitude.cu:17-39, tools/reference.py:98-100).
  * ------------------------------------------------------------------------------------- */

@@ -136,3 +136,18 @@ def test_channelize_mfma_and_fir_tune_arguments() -> None:
 
 def test_spectrum_magnitude_arguments() -> None:
     assert rf.lib().rf_spectrum_magnitude(None, 8, 8, None) == rf.RF_INVALID_ARG
+
+
+def test_fir_plan_arguments() -> None:
+    # Checked before any device call: NULL handle, unknown variant, bad block, NULL taps,
+    # constant-cache tap limit; a NULL plan to run or destroy.
+    L = rf.lib()
+    p = ctypes.c_void_p()
+    assert L.rf_fir_plan_create(4, 8, 1025, 0, None, None) == rf.RF_INVALID_ARG
+    assert L.rf_fir_plan_create(9, 8, 1025, 0, None, ctypes.byref(p)) == rf.RF_INVALID_ARG
+    assert L.rf_fir_plan_create(0, 8, 1025, 33, None, ctypes.byref(p)) == rf.RF_INVALID_ARG
+    assert L.rf_fir_plan_create(0, None, 1025, 0, None, ctypes.byref(p)) == rf.RF_INVALID_ARG
+    assert L.rf_fir_plan_create(3, 8, 8193, 0, None, ctypes.byref(p)) == rf.RF_LIMIT
+    assert p.value is None
+    assert L.rf_fir_plan_run(None, 8, 8, 16, None) == rf.RF_INVALID_ARG
+    assert L.rf_fir_plan_destroy(None) == rf.RF_OK

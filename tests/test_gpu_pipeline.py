@@ -236,6 +236,27 @@ def test_fir_long_random_filter(impl: str) -> None:
     assert_fir_close(impl, rf.fir(x, h, impl), x, h)
 
 
+@pytest.mark.parametrize("impl", FIR_IMPLS)
+def test_fir_plan_gives_the_bits_of_fir(impl: str) -> None:
+    # The plan holds its own copy of the taps (the caller's buffer is zeroed after create) and
+    # serves inputs of different lengths, on the default and on a created stream.
+    cases = [(cf32("tone_odd"), taps("lowpass_long")), (cf32("short"), taps("lowpass")),
+             (noise(21, 20011), (np.random.default_rng(22).standard_normal(300) / 20)
+              .astype(np.float32))]
+    stream = rf.Stream()
+    for x, h in cases:
+        d_h = rf.DeviceBuffer.from_array(h)
+        plan = rf.FirPlan(impl, d_h.ptr, h.size)
+        d_h.zero()
+        for n, s in [(x.size, None), (x.size // 3 + 1, stream)]:
+            d_x = rf.DeviceBuffer.from_array(x[:n])
+            d_y = rf.DeviceBuffer(n)
+            plan.run(d_x.ptr, d_y.ptr, n, s)
+            np.testing.assert_array_equal(d_y.download(stream=s), rf.fir(x[:n], h, impl))
+        plan.destroy()
+    stream.destroy()
+
+
 @pytest.mark.parametrize("num_taps", [66, 300, 777, 1500])
 def test_fir_ola_uneven_halo_and_lengths(num_taps: int) -> None:
     # Overlap-save segments of 4096 keep 4097 - T outputs; a halo T - 1 that is not a multiple
