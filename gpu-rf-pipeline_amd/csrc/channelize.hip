@@ -131,8 +131,37 @@ __global__ __launch_bounds__(kChanThreads) void channelize_fused_kernel(
     const long long t0 = group * F;
     const long long base = t0 * M - static_cast<long long>(pb) * M + 1;
     const int len = (F + pb - 1) * M;
-    for (int i = threadIdx.x; i < len; i += kChanThreads) {
-        lds[i] = load_sample(x, base + i, n);
+    if constexpr (LOG2M != 0 && PB != 0) {
+        // Fixed shape: every lane's loads issued before the first LDS store, so the staging
+        // costs one HBM round trip instead of one per loop iteration.
+        constexpr int kLen = ((kChanOutputs >> LOG2M) + PB - 1) << LOG2M;
+        constexpr int kIters = (kLen + kChanThreads - 1) / kChanThreads;
+        float2 v[kIters];
+        if (base >= 0 && base + kLen <= n) {
+            const float2* src = x + base;
+#pragma unroll
+            for (int k = 0; k < kIters; ++k) {
+                const int i = threadIdx.x + k * kChanThreads;
+                v[k] = i < kLen ? gload(src, i) : make_float2(0.0f, 0.0f);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < kIters; ++k) {
+                const int i = threadIdx.x + k * kChanThreads;
+                v[k] = i < kLen ? load_sample(x, base + i, n) : make_float2(0.0f, 0.0f);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kIters; ++k) {
+            const int i = threadIdx.x + k * kChanThreads;
+            if (i < kLen) {
+                lds[i] = v[k];
+            }
+        }
+    } else {
+        for (int i = threadIdx.x; i < len; i += kChanThreads) {
+            lds[i] = load_sample(x, base + i, n);
+        }
     }
     __syncthreads();
 
@@ -181,6 +210,21 @@ __global__ __launch_bounds__(kChanThreads) void channelize_fused_kernel(
     }
     const int total = count * M;
     float2* dst = out + t0 * M;
+    if (total == kChanOutputs) {
+        // A full workgroup: the LDS reads of all eight rounds in flight together.
+        constexpr int kRounds = kChanOutputs / kChanThreads;
+        float2 z[kRounds];
+#pragma unroll
+        for (int k = 0; k < kRounds; ++k) {
+            const int e = threadIdx.x + k * kChanThreads;
+            z[k] = lds[(e >> log2M) * slots + lds_pad(e & (M - 1))];
+        }
+#pragma unroll
+        for (int k = 0; k < kRounds; ++k) {
+            gstore(dst, threadIdx.x + k * kChanThreads, z[k]);
+        }
+        return;
+    }
     for (int e = threadIdx.x; e < total; e += kChanThreads) {
         dst[e] = lds[(e >> log2M) * slots + lds_pad(e & (M - 1))];
     }

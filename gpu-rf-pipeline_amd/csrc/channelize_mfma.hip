@@ -125,17 +125,15 @@ __global__ __launch_bounds__(kThreads) void channelize_mfma_kernel(
     const int t = threadIdx.x;
     const int l = t & 63;
     const int w = t >> 6;
-    const long long t0 = static_cast<long long>(blockIdx.x) * kFramesPerBlock;
-
-    // D fragments of this wave: issued first, used last.
-    f16x8 dh[kSteps];
-    f16x8 dl[kSteps];
-    const f16x8* tab = reinterpret_cast<const f16x8*>(table);
-#pragma unroll
-    for (int s = 0; s < kSteps; ++s) {
-        dh[s] = tab[(w * kSteps + s) * 64 + l];
-        dl[s] = tab[kTableHalves / 8 + (w * kSteps + s) * 64 + l];
+    // XCD-major order: workgroup b runs on XCD b % 8, which gets consecutive frame blocks, so
+    // the 15-frame history each block re-reads was just loaded into that XCD's L2.
+    const long long blocks = (frames + kFramesPerBlock - 1) / kFramesPerBlock;
+    const long long per_xcd = (blocks + 7) / 8;
+    const long long block = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (block >= blocks) {
+        return;  // the whole workgroup: padding of the grid to a multiple of 8
     }
+    const long long t0 = block * kFramesPerBlock;
 
     // 1. branch sums: lane (r, g) = (t % 64, t / 64), frames f = 16 g + j.
     const int r = l;
@@ -166,6 +164,17 @@ __global__ __launch_bounds__(kThreads) void channelize_mfma_kernel(
         }
         v[j] = acc;
         vmax = fmaxf(vmax, fmaxf(fabsf(acc.x), fabsf(acc.y)));
+    }
+
+    // D fragments of this wave, issued after the branch sums (whose sample window holds most
+    // of the registers) and used by the GEMM after two barriers.
+    f16x8 dh[kSteps];
+    f16x8 dl[kSteps];
+    const f16x8* tab = reinterpret_cast<const f16x8*>(table);
+#pragma unroll
+    for (int s = 0; s < kSteps; ++s) {
+        dh[s] = tab[(w * kSteps + s) * 64 + l];
+        dl[s] = tab[kTableHalves / 8 + (w * kSteps + s) * 64 + l];
     }
 
     // 2. block scale, FP16 split into LDS rows 16 g + j, columns 2 r, 2 r + 1.
@@ -254,7 +263,8 @@ extern "C" rf_status rf_channelize_mfma(const rf_cf32* x, const float* taps, rf_
             rf::fail(RF_INVALID_ARG, "channelize mfma: NULL buffer");
         }
         const _Float16* table = rf::dft_table();
-        const unsigned grid = static_cast<unsigned>(rf::ceil_div(frames, rf::kFramesPerBlock));
+        const unsigned grid =
+            static_cast<unsigned>(8 * rf::ceil_div(rf::ceil_div(frames, rf::kFramesPerBlock), 8));
         rf::channelize_mfma_kernel<<<grid, rf::kThreads, 0, rf::as_stream(stream)>>>(
             reinterpret_cast<const float2*>(x), taps, reinterpret_cast<float*>(out),
             num_samples, num_taps, frames, table);

@@ -548,6 +548,22 @@ def test_channelize_mfma_config_shape_and_dynamic_range() -> None:
     np.testing.assert_allclose(got, fused, rtol=0, atol=2e-4 * np.max(np.abs(fused)))
 
 
+@pytest.mark.parametrize("n,num_taps", [(1000, 64), (4097, 300), (70001, 1000), (65536, 1024),
+                                        (300000, 17)])
+def test_channelize_64_branch_shapes(n: int, num_taps: int) -> None:
+    # M = 64 with 1..16 taps per branch: the fixed-shape staging of the fused kernel (first
+    # workgroup reaching before x[0], last one past x[n-1], inputs shorter than a workgroup) and
+    # the matrix-core kernel over many frame blocks in XCD-major order.
+    x = noise(n, n)
+    h = (np.random.default_rng(num_taps).standard_normal(num_taps) / 10).astype(np.float32)
+    want = oracle.channelize(x, h, 64)
+    got = rf.channelize(x, h, 64).reshape(-1, 64)
+    np.testing.assert_allclose(got, want, rtol=CHANNELIZE_RTOL, atol=CHANNELIZE_ATOL)
+    got_mfma = rf.channelize_mfma(x, h, 64).reshape(-1, 64)
+    bound = 2.0 ** -15 * channelize_term_sum(x, h, 64)[:, None]
+    assert np.all(np.abs(got_mfma - want) <= bound + 1e-30)
+
+
 def test_channelize_mfma_limits() -> None:
     x, h = cf32("tone"), taps("lowpass_long")
     with pytest.raises(rf.RfError):
