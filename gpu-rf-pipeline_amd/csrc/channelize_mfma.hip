@@ -146,10 +146,19 @@ __global__ __launch_bounds__(kThreads) void channelize_mfma_kernel(
     }
     float2 xq[kP + 15];  // x[(q0 + u) M - r], q0 = t0 + 16 g - 15
     const long long q0 = t0 + 16 * g - (kP - 1);
+    if (q0 * kM - (kM - 1) >= 0 && (q0 + kP + 14) * kM < n) {
+        // Every sample of the window inside x: no per-load test.
+        const float2* src = x + q0 * kM - r;
 #pragma unroll
-    for (int u = 0; u < kP + 15; ++u) {
-        const long long m = (q0 + u) * kM - r;
-        xq[u] = (m >= 0 && m < n) ? x[m] : make_float2(0.0f, 0.0f);
+        for (int u = 0; u < kP + 15; ++u) {
+            xq[u] = gload(src, u * kM);
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < kP + 15; ++u) {
+            const long long m = (q0 + u) * kM - r;
+            xq[u] = (m >= 0 && m < n) ? x[m] : make_float2(0.0f, 0.0f);
+        }
     }
     float2 v[16];
     float vmax = 0.0f;
@@ -224,6 +233,16 @@ __global__ __launch_bounds__(kThreads) void channelize_mfma_kernel(
     // 4. scale back, write z[t][k].{re, im} = out[2 (t M + k) + e] = out[t 128 + col].
     const float scale = ldexpf(1.0f, -ex);
     const int col = 32 * w + i;
+    if (t0 + kFramesPerBlock <= frames) {
+        float* dst = out + t0 * (2 * kM) + col;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int row = (q & 3) + 8 * (q >> 2) + 4 * (l >> 5);
+            dst[row * (2 * kM)] = acc0[q] * scale;
+            dst[(32 + row) * (2 * kM)] = acc1[q] * scale;
+        }
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int row = (q & 3) + 8 * (q >> 2) + 4 * (l >> 5);

@@ -54,7 +54,10 @@ __device__ __forceinline__ void xcorr_row4096(const float2* __restrict__ wa,
                                               long long row, int phat, float scale, float2* buf,
                                               const float2* __restrict__ stages)
 {
-    const int t = threadIdx.x;
+    // The lane index through an empty asm: callers loop over rows, and without it the compiler
+    // hoists every per-lane address of the body out of their loops into held registers.
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
     float2 ua[16];
     float2 ub[16];
 #pragma unroll
@@ -62,19 +65,23 @@ __device__ __forceinline__ void xcorr_row4096(const float2* __restrict__ wa,
         ua[r] = gload(wa, t + r * k4096Lanes);
         ub[r] = gload(wb, t + r * k4096Lanes);
     }
-    fft4096<false>(buf, t, ua, stages);
+    fft4096_staged<false>(buf, t, ua, stages);
     __syncthreads();  // every lane has read a's last pass from buf
-    fft4096<false>(buf, t, ub, stages);
+    fft4096_staged<false>(buf, t, ub, stages);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         ub[r] = weigh(ua[r], ub[r], scale, phat);
     }
     __syncthreads();
-    fft4096<true>(buf, t, ub, stages);
+    fft4096_staged<true>(buf, t, ub, stages);
+    // w_L^-(j1 row) with j1 row < L <= 2^24: 32-bit index arithmetic.
+    const unsigned row32 = static_cast<unsigned>(row);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int j1 = t + r * k4096Lanes;
-        gstore(wb, j1, cmul(ub[r], four_step_twiddle<true>(fs, j1 * row)));
+        const unsigned m = static_cast<unsigned>(j1) * row32;
+        const float2 w = cmul(gload(fs.lo, m & (kTwiddleSize - 1)), gload(fs.hi, m >> kTwiddleLog2));
+        gstore(wb, j1, cmul(ub[r], make_float2(w.x, -w.y)));
     }
 }
 
