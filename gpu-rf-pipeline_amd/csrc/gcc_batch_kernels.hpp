@@ -68,12 +68,13 @@ __global__ __launch_bounds__(kItemThreads) void batch_column_fwd_kernel(
         const int2 m = map[begin + item - which * total];
         const PairDesc d = pairs[m.x];
         const FourStep fs = shape_of(d);
-        const long long L = 1LL << d.log2L;
+        // Indices below L <= 2^24 and lengths below 2^24: 32-bit arithmetic throughout.
+        const int L = 1 << d.log2L;
         const int log2n1 = d.log2n1;
         const long long c0 = static_cast<long long>(m.y) * column_tile(fs);
         float2* dst = d.work + (which ? L : 0);
         auto store = [=](long long j1, int k2, float2 v) {
-            gstore(dst, j1 + (static_cast<long long>(k2) << log2n1), v);
+            gstore(dst, static_cast<int>(j1) + (k2 << log2n1), v);
         };
         if (which == 0) {
             const float2* a = d.a;
@@ -81,7 +82,10 @@ __global__ __launch_bounds__(kItemThreads) void batch_column_fwd_kernel(
             column_pass_block<false>(
                 fs, c0,
                 [=](long long j1, int j2) {
-                    return rotated_a(a, na, L, j1 + (static_cast<long long>(j2) << log2n1));
+                    // element n of a rotated by na - 1 (src/xcorr.cu:278-282), zero padded
+                    int k = static_cast<int>(j1) + (j2 << log2n1) + na - 1;
+                    k = k >= L ? k - L : k;
+                    return k < na ? gload(a, k) : make_float2(0.0f, 0.0f);
                 },
                 store, lds, table);
         } else {
@@ -90,7 +94,8 @@ __global__ __launch_bounds__(kItemThreads) void batch_column_fwd_kernel(
             column_pass_block<false>(
                 fs, c0,
                 [=](long long j1, int j2) {
-                    return padded_b(b, nb, j1 + (static_cast<long long>(j2) << log2n1));
+                    const int k = static_cast<int>(j1) + (j2 << log2n1);
+                    return k < nb ? gload(b, k) : make_float2(0.0f, 0.0f);
                 },
                 store, lds, table);
         }
@@ -128,38 +133,41 @@ __global__ __launch_bounds__(kItemThreads) void batch_column_inv_peak_kernel(
         }
         const PairDesc d = pairs[p];
         const FourStep fs = shape_of(d);
-        const long long L = 1LL << d.log2L;
+        // Indices below L <= 2^24: 32-bit arithmetic throughout.
+        const int L = 1 << d.log2L;
         const int log2n1 = d.log2n1;
         const int n2 = 1 << d.log2n2;
         const int lc = column_tile_log2(fs);
         const int C = 1 << lc;
         const int slots = column_slots(n2);
-        const long long c0 = static_cast<long long>(within) * C;
+        const int c0 = within * C;
         const float2* src = d.work + L;
+        int tid = threadIdx.x;  // through an empty asm: no per-lane indices hoisted out of the walk
+        asm volatile("" : "+v"(tid));
         // C n2 = 4096 values, 8 per lane, all loads in flight before the LDS writes
         float2 v[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int e = threadIdx.x + i * kItemThreads;
-            v[i] = gload(src, (c0 + (e & (C - 1))) + (static_cast<long long>(e >> lc) << log2n1));
+            const int e = tid + i * kItemThreads;
+            v[i] = gload(src, (c0 + (e & (C - 1))) + ((e >> lc) << log2n1));
         }
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int e = threadIdx.x + i * kItemThreads;
+            const int e = tid + i * kItemThreads;
             lds[(e & (C - 1)) * slots + lds_pad(e >> lc)] = v[i];
         }
         __syncthreads();
-        const int f = threadIdx.x >> (d.log2n2 - 3);
-        const int t = threadIdx.x & ((n2 >> 3) - 1);
+        const int f = tid >> (d.log2n2 - 3);
+        const int t = tid & ((n2 >> 3) - 1);
         lds_fft<true>(lds + f * slots, d.log2n2, t, true, table);
         const long long lags = static_cast<long long>(d.num_a) + d.num_b - 1;
         unsigned long long best = 0ULL;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int e = threadIdx.x + i * kItemThreads;
+            const int e = tid + i * kItemThreads;
             const int j2 = e >> lc;
             const int col = e & (C - 1);
-            const long long n = (c0 + col) + (static_cast<long long>(j2) << log2n1);
+            const int n = (c0 + col) + (j2 << log2n1);
             if (n < lags) {
                 const float2 val = lds[col * slots + lds_pad(j2)];
                 const unsigned long long key = peak_key(hypotf(val.x, val.y), n);
