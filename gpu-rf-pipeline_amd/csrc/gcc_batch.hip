@@ -448,7 +448,7 @@ rf_status rf_gcc_batch_tune(rf_gcc_batch* h, rf_peak_result* results, rf_stream 
         hipEvent_t stop = nullptr;
         RF_HIP_CHECK(hipEventCreate(&start));
         RF_HIP_CHECK(hipEventCreate(&stop));
-        const int grids[] = {2, 4, 8};
+        const int grids[] = {2, 3, 4, 6, 8};
         const std::size_t budgets[] = {0, std::size_t(48) << 20, std::size_t(96) << 20,
                                        std::size_t(160) << 20};
         float best_ms = 0.0f;

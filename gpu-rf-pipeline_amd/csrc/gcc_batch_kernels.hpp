@@ -76,29 +76,19 @@ __global__ __launch_bounds__(kItemThreads) void batch_column_fwd_kernel(
         auto store = [=](long long j1, int k2, float2 v) {
             gstore(dst, static_cast<int>(j1) + (k2 << log2n1), v);
         };
-        if (which == 0) {
-            const float2* a = d.a;
-            const int na = d.num_a;
-            column_pass_block<false>(
-                fs, c0,
-                [=](long long j1, int j2) {
-                    // element n of a rotated by na - 1 (src/xcorr.cu:278-282), zero padded
-                    int k = static_cast<int>(j1) + (j2 << log2n1) + na - 1;
-                    k = k >= L ? k - L : k;
-                    return k < na ? gload(a, k) : make_float2(0.0f, 0.0f);
-                },
-                store, lds, table);
-        } else {
-            const float2* b = d.b;
-            const int nb = d.num_b;
-            column_pass_block<false>(
-                fs, c0,
-                [=](long long j1, int j2) {
-                    const int k = static_cast<int>(j1) + (j2 << log2n1);
-                    return k < nb ? gload(b, k) : make_float2(0.0f, 0.0f);
-                },
-                store, lds, table);
-        }
+        // One code path for both inputs: a is read rotated by na - 1 (src/xcorr.cu:278-282),
+        // b as it is; both zero past their length. The choices are uniform per item.
+        const float2* src = which ? d.b : d.a;
+        const int len = which ? d.num_b : d.num_a;
+        const int rot = which ? 0 : d.num_a - 1;
+        column_pass_block<false>(
+            fs, c0,
+            [=](long long j1, int j2) {
+                int k = static_cast<int>(j1) + (j2 << log2n1) + rot;
+                k = k >= L ? k - L : k;
+                return k < len ? gload(src, k) : make_float2(0.0f, 0.0f);
+            },
+            store, lds, table);
         __syncthreads();  // the next item reuses the LDS
     });
 }

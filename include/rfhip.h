@@ -279,7 +279,7 @@ rf_status rf_gcc_batch_launch_graph(rf_gcc_batch* handle, rf_peak_result* result
  * pairs: more groups add launch tails). */
 rf_status rf_gcc_batch_set_group_bytes(rf_gcc_batch* handle, size_t bytes);
 rf_status rf_gcc_batch_group_bytes(const rf_gcc_batch* handle, size_t* bytes, int* groups);
-/* Autotuner: times the bound batch for 2, 4 and 8 persistent workgroups per CU and group
+/* Autotuner: times the bound batch for 2, 3, 4, 6 and 8 persistent workgroups per CU and group
  * budgets of 0, 48, 96 and 160 MiB (3 runs each after a warm-up, device events on `stream`,
  * which it synchronises), keeps the fastest pair for later runs and reports the workgroups per
  * CU (rf_gcc_batch_group_bytes reports the budget). The results buffer is overwritten. */
