@@ -9,8 +9,9 @@ gain. One step is one rf_fir call over the whole capture with inputs already in 
   python bench.py [--steps K] [--warmup W] [--impl fft_ola] [--workload fir_long|fir_plain|
                    channelize] [--gpus 1]
 
-Input/output buffers rotate over three sets (768 MiB for the long FIR) so that the 256 MB
-Infinity Cache cannot hold a step's input from the previous visit. Timing: device events on
+Input/output buffers rotate over enough sets (at least 320 MiB: three for the long FIR, 20 for
+the 2^20-sample FIR) that the 256 MB Infinity Cache cannot hold a step's input from the previous
+visit. Timing: device events on
 the stream the kernels run on, bracketing all K steps, after a device synchronisation. The
 roofline figure uses the algorithmic bytes of one step (input + output + taps) over the time of
 the dominant kernel alone, measured the same way over K launches of a prepared filter
@@ -119,7 +120,10 @@ def time_steps(rf, step, steps: int, warmup: int, stream) -> float:
 def bench_fir(rf, args, long_filter: bool) -> dict:
     x, h = fir_long_input() if long_filter else fir_plain_input()
     n, t = x.size, h.size
-    sets = 3
+    # Enough rotating input/output sets that a step's buffers were evicted from the 256 MB
+    # Infinity Cache since their last visit: at least 320 MiB in all (3 sets at 2^24 samples,
+    # 20 at 2^20).
+    sets = max(3, -(-(320 << 20) // (16 * n)))
     stream = rf.Stream()
     d_h = rf.DeviceBuffer.from_array(h)
     d_x = [rf.DeviceBuffer.from_array(x) for _ in range(sets)]
@@ -146,16 +150,47 @@ def bench_fir(rf, args, long_filter: bool) -> dict:
         plan.run(d_x[i % sets].ptr, d_y[i % sets].ptr, n, stream.handle)
 
     kernel_ms = time_steps(rf, plan_step, args.steps, args.warmup, stream.handle) / args.steps
+    graph = None
+    if not long_filter:
+        # A short capture is a few microseconds of GPU work per call, as long as a launch from
+        # the host: the same K calls captured once into a HIP graph and replayed show the
+        # GPU-side rate without per-call host gaps.
+        g = rf.Graph(stream, lambda: [plan_step(i) for i in range(args.steps)])
+        g.launch(stream)
+        stream.synchronize()
+        gms = time_steps(rf, lambda i: g.launch(stream), 1, 0, stream.handle) / args.steps
+        g.destroy()
+        graph = {"ms_per_step": round(gms, 5), "msamples_per_s": round(n / gms / 1e3, 1),
+                 "hbm_frac": round(bytes_step / (gms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                 "calls_per_graph": args.steps}
     plan.destroy()
     achieved = bytes_step / (kernel_ms * 1e-3) / 1e9
+
+    # Arithmetic: the direct form costs 4 flop per tap and output (complex x real MAC); the
+    # overlap-save form two 4096-point FFTs (5 P log2 P each) and 4096 complex products (6 flop)
+    # per segment of 4097 - T outputs.
+    direct_flop = 4 * n * t
+    if args.impl == "fft_ola" and 65 < t <= 2049:
+        segments = -(-n // (4097 - t))
+        algo_flop = segments * (2 * 5 * 4096 * 12 + 6 * 4096)
+    else:
+        algo_flop = direct_flop
+    compute = {"algorithm_flop_per_step": algo_flop, "direct_form_flop_per_step": direct_flop,
+               "achieved_tflops": round(algo_flop / (kernel_ms * 1e-3) / 1e12, 2),
+               "direct_form_equivalent_tflops": round(direct_flop / (kernel_ms * 1e-3) / 1e12, 1),
+               "fp32_peak_tflops": FP32_PEAK_TFLOPS,
+               "frac": round(algo_flop / (kernel_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)}
 
     variants = {}
     for name in [v.name for v in rf.fir_impls()]:
         if args.no_variants or name == args.impl or name == "naive" and long_filter:
             continue
-        vms = time_steps(rf, step_for(rf.fir_impl_index(name)), max(2, args.steps // 5), 1,
-                         stream.handle) / max(2, args.steps // 5)
-        variants[name] = {"ms_per_step": round(vms, 4), "msamples_per_s": round(n / vms / 1e3, 1)}
+        reps = max(2, args.steps // 5)
+        vms = time_steps(rf, step_for(rf.fir_impl_index(name)), reps, 1, stream.handle) / reps
+        vy = d_y[(1 + reps - 1) % sets].download()
+        v_abs, v_rel, _ = sampled_fir_error(x, h, vy)
+        variants[name] = {"ms_per_step": round(vms, 4), "msamples_per_s": round(n / vms / 1e3, 1),
+                          "max_abs_err": v_abs, "max_rel_err": v_rel}
 
     cpu = None
     if not args.no_cpu:
@@ -180,11 +215,13 @@ def bench_fir(rf, args, long_filter: bool) -> dict:
                    "buffer_sets": sets, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles(f"fir_{args.impl}")},
+                     "traffic": traffic_from_profiles(f"fir_{args.impl}") if long_filter else None},
         "roofline_kernel": {"launches": "rf_fir_plan_run: the filtering kernel only "
                                         "(fir_ola_kernel for fft_ola)",
                             "ms_per_launch": round(kernel_ms, 5),
                             "msamples_per_s": round(n / kernel_ms / 1e3, 1)},
+        "compute": compute,
+        "graph_replay": graph,
         "accuracy": {"max_abs_err": max_abs, "max_rel_err": max_rel,
                      "within_rtol_atol_1e-5": excess <= 1e-5,
                      "checked_outputs": "first taps+8 and 2000 random outputs vs float64"},

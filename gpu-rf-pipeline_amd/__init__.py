@@ -109,6 +109,10 @@ class FirConfig(ctypes.Structure):
 _proto("rf_fir_tune", _st, _vp, _vp, _vp, _int, _int, ctypes.c_uint, _vp,
        ctypes.POINTER(FirConfig))
 _proto("rf_fir_run_config", _st, ctypes.POINTER(FirConfig), _vp, _vp, _vp, _int, _int, _vp)
+_proto("rf_graph_begin_capture", _st, _vp)
+_proto("rf_graph_end_capture", _st, _vp, ctypes.POINTER(_vp))
+_proto("rf_graph_launch", _st, _vp, _vp)
+_proto("rf_graph_destroy", _st, _vp)
 _proto("rf_fir_plan_create", _st, _int, _vp, _int, _int, _vp, ctypes.POINTER(_vp))
 _proto("rf_fir_plan_run", _st, _vp, _vp, _vp, _int, _vp)
 _proto("rf_fir_plan_destroy", _st, _vp)
@@ -762,6 +766,36 @@ def fir_tune(x: int, taps: int, y: int, n: int, num_taps: int, impls: Sequence[s
 def fir_run_config(config: FirConfig, x: int, taps: int, y: int, n: int, num_taps: int,
                    stream=None) -> None:
     check(_lib.rf_fir_run_config(ctypes.byref(config), x, taps, y, n, num_taps, _sp(stream)))
+
+
+class Graph:
+    """Work enqueued by `record()` on a created stream, captured once and replayed by launch()."""
+
+    def __init__(self, stream: "Stream", record):
+        check(_lib.rf_graph_begin_capture(stream.handle))
+        try:
+            record()
+        finally:
+            p = _vp()
+            status = _lib.rf_graph_end_capture(stream.handle, ctypes.byref(p))
+        check(status)
+        self.handle = p.value
+
+    def launch(self, stream=None) -> None:
+        check(_lib.rf_graph_launch(self.handle, _sp(stream)))
+
+    def destroy(self) -> None:
+        if self.handle:
+            check(_lib.rf_graph_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.rf_graph_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
 
 
 class FirPlan:

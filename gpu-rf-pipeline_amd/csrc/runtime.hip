@@ -5,6 +5,8 @@
 #include "rf_common.hpp"
 
 #include <cstring>
+#include <mutex>
+#include <string>
 
 namespace rf {
 
@@ -25,32 +27,55 @@ void fail_hip(hipError_t err, const char* call, const char* file, int line)
              + ":" + std::to_string(line));
 }
 
-int max_lds_per_block()
+namespace {
+
+// The device limits the launch paths consult, queried once per device: a FIR or channelizer
+// call over a small capture is a few microseconds of GPU work, and three attribute queries per
+// call would add host time of the same order.
+struct DeviceLimits {
+    int lds_per_block = 0;
+    int lds_per_cu = 0;
+    int cus = 0;
+};
+
+constexpr int kMaxLimitDevices = 64;
+std::once_flag g_limits_once[kMaxLimitDevices];
+DeviceLimits g_limits[kMaxLimitDevices];
+
+const DeviceLimits& device_limits()
 {
     int device = 0;
     RF_HIP_CHECK(hipGetDevice(&device));
-    int bytes = 0;
-    RF_HIP_CHECK(hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
-    return bytes;
+    if (device < 0 || device >= kMaxLimitDevices) {
+        fail(RF_LIMIT, "device index " + std::to_string(device) + " out of range");
+    }
+    std::call_once(g_limits_once[device], [device] {
+        DeviceLimits& l = g_limits[device];
+        RF_HIP_CHECK(hipDeviceGetAttribute(&l.lds_per_block,
+                                           hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+        RF_HIP_CHECK(hipDeviceGetAttribute(&l.lds_per_cu,
+                                           hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                           device));
+        RF_HIP_CHECK(hipDeviceGetAttribute(&l.cus, hipDeviceAttributeMultiprocessorCount, device));
+    });
+    return g_limits[device];
+}
+
+}  // namespace
+
+int max_lds_per_block()
+{
+    return device_limits().lds_per_block;
 }
 
 int lds_per_cu()
 {
-    int device = 0;
-    RF_HIP_CHECK(hipGetDevice(&device));
-    int bytes = 0;
-    RF_HIP_CHECK(hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
-                                       device));
-    return bytes;
+    return device_limits().lds_per_cu;
 }
 
 int cu_count()
 {
-    int device = 0;
-    RF_HIP_CHECK(hipGetDevice(&device));
-    int cus = 0;
-    RF_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-    return cus;
+    return device_limits().cus;
 }
 
 namespace {
@@ -303,6 +328,48 @@ rf_status rf_event_elapsed_ms(void* start, void* stop, float* ms)
     return guarded([&] {
         RF_HIP_CHECK(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start),
                                          static_cast<hipEvent_t>(stop)));
+    });
+}
+
+rf_status rf_graph_begin_capture(rf_stream stream)
+{
+    return guarded([&] {
+        if (stream == nullptr) rf::fail(RF_INVALID_ARG, "graph capture: needs a created stream");
+        RF_HIP_CHECK(hipStreamBeginCapture(rf::as_stream(stream), hipStreamCaptureModeThreadLocal));
+    });
+}
+
+rf_status rf_graph_end_capture(rf_stream stream, void** graph)
+{
+    return guarded([&] {
+        if (stream == nullptr || graph == nullptr) rf::fail(RF_INVALID_ARG, "graph capture: NULL");
+        *graph = nullptr;
+        hipGraph_t g = nullptr;
+        RF_HIP_CHECK(hipStreamEndCapture(rf::as_stream(stream), &g));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t err = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (err != hipSuccess) {
+            rf::fail_hip(err, "hipGraphInstantiate", __FILE__, __LINE__);
+        }
+        *graph = exec;
+    });
+}
+
+rf_status rf_graph_launch(void* graph, rf_stream stream)
+{
+    return guarded([&] {
+        if (graph == nullptr) rf::fail(RF_INVALID_ARG, "graph launch: NULL graph");
+        RF_HIP_CHECK(hipGraphLaunch(static_cast<hipGraphExec_t>(graph), rf::as_stream(stream)));
+    });
+}
+
+rf_status rf_graph_destroy(void* graph)
+{
+    return guarded([&] {
+        if (graph != nullptr) {
+            RF_HIP_CHECK(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph)));
+        }
     });
 }
 

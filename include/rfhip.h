@@ -82,6 +82,16 @@ rf_status rf_event_destroy(void* event);
 rf_status rf_event_record(void* event, rf_stream stream);
 rf_status rf_event_synchronize(void* event);
 rf_status rf_event_elapsed_ms(void* start, void* stop, float* ms);
+/* HIP graphs by stream capture: the work enqueued on `stream` (a created stream, not NULL)
+ * between begin and end is recorded instead of run; end instantiates it (*graph is a
+ * hipGraphExec_t). rf_graph_launch replays it on a stream, rf_graph_destroy frees it. rf_fir
+ * with a direct variant and rf_fir_plan_run with fft_ola or a direct variant only enqueue
+ * kernels and can be captured; calls that take stream-ordered workspace per call (rf_fir with
+ * fft_ola or an MFMA variant) should not be. */
+rf_status rf_graph_begin_capture(rf_stream stream);
+rf_status rf_graph_end_capture(rf_stream stream, void** graph);
+rf_status rf_graph_launch(void* graph, rf_stream stream);
+rf_status rf_graph_destroy(void* graph);
 
 /* ---------------------------------------------------------------------------------------
  * FIR:  y[n] = sum_{k=0}^{T-1} taps[k] * x[n-k],  x[m] = 0 for m < 0,  n < num_samples.

@@ -151,3 +151,12 @@ def test_fir_plan_arguments() -> None:
     assert p.value is None
     assert L.rf_fir_plan_run(None, 8, 8, 16, None) == rf.RF_INVALID_ARG
     assert L.rf_fir_plan_destroy(None) == rf.RF_OK
+
+
+def test_graph_arguments() -> None:
+    L = rf.lib()
+    g = ctypes.c_void_p()
+    assert L.rf_graph_begin_capture(None) == rf.RF_INVALID_ARG
+    assert L.rf_graph_end_capture(None, ctypes.byref(g)) == rf.RF_INVALID_ARG
+    assert L.rf_graph_launch(None, None) == rf.RF_INVALID_ARG
+    assert L.rf_graph_destroy(None) == rf.RF_OK

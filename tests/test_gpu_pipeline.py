@@ -257,6 +257,40 @@ def test_fir_plan_gives_the_bits_of_fir(impl: str) -> None:
     stream.destroy()
 
 
+def test_fir_calls_replayed_from_a_graph() -> None:
+    # A captured graph of direct-variant rf_fir calls and prepared overlap-save runs, replayed
+    # three times with eager work and synchronisations in between: every replay gives the eager
+    # bits.
+    x1, x2 = noise(31, 30011), noise(32, 5000)
+    h_short, h_long = taps("lowpass"), taps("lowpass_long")
+    stream = rf.Stream()
+    d_x1, d_x2 = rf.DeviceBuffer.from_array(x1), rf.DeviceBuffer.from_array(x2)
+    d_hs, d_hl = rf.DeviceBuffer.from_array(h_short), rf.DeviceBuffer.from_array(h_long)
+    outs = [rf.DeviceBuffer(x1.size), rf.DeviceBuffer(x2.size), rf.DeviceBuffer(x1.size)]
+    plan = rf.FirPlan("fft_ola", d_hl.ptr, h_long.size)
+    reg = rf.fir_impl_index("register")
+
+    def record():
+        rf.fir_device(reg, d_x1.ptr, d_hs.ptr, outs[0].ptr, x1.size, h_short.size, 0, stream.handle)
+        rf.fir_device(reg, d_x2.ptr, d_hs.ptr, outs[1].ptr, x2.size, h_short.size, 0, stream.handle)
+        plan.run(d_x1.ptr, outs[2].ptr, x1.size, stream.handle)
+
+    graph = rf.Graph(stream, record)
+    want = [rf.fir(x1, h_short, "register"), rf.fir(x2, h_short, "register"),
+            rf.fir(x1, h_long, "fft_ola")]
+    for rep in range(3):
+        for o in outs:
+            o.zero(stream)
+        graph.launch(stream)
+        stream.synchronize()
+        for o, w in zip(outs, want):
+            np.testing.assert_array_equal(o.download(stream=stream), w)
+        rf.fir(x2, h_long, "fft_ola")  # eager work between replays
+    graph.destroy()
+    plan.destroy()
+    stream.destroy()
+
+
 @pytest.mark.parametrize("num_taps", [66, 300, 777, 1500])
 def test_fir_ola_uneven_halo_and_lengths(num_taps: int) -> None:
     # Overlap-save segments of 4096 keep 4097 - T outputs; a halo T - 1 that is not a multiple
