@@ -108,18 +108,40 @@ __device__ __forceinline__ void load8(const View& v, int row, int rows, int k, i
     }
 }
 
-// C[M x N] = A[M x K] B[K x N], one wave per 32 x 32 tile of C, grid (N / 32, ceil(M / 32)).
-// Rows m >= M and depths k >= K read as zero; N is a multiple of 32. VEC_A / VEC_B: the
-// operand is contiguous along k (A: sc == 1, B viewed as rows n: sr == 1).
+// C[M x N] = A[M x K] B[K x N]: one 32 x 32 tile of C per workgroup of four waves, which split
+// the K/16 matrix-core steps between them (wave w takes steps w, w + 4, ...; two steps' loads
+// in flight per wave) and add their partial tiles through LDS in wave order, so the result does
+// not depend on timing. Grid (N / 32, ceil(M / 32)). Rows m >= M and depths k >= K read as
+// zero; N is a multiple of 32. VEC_A / VEC_B: the operand is contiguous along k (A: sc == 1,
+// B viewed as rows n: sr == 1).
 // MFMA 32x32x16 operand layout: lane l holds A(l % 32, 8 (l / 32) + q) and
 // B(8 (l / 32) + q, l % 32) for q < 8; accumulator v of lane l is C(row, l % 32) with
 // row = (v % 4) + 8 (v / 4) + 4 (l / 32).
+constexpr int kGemmWaves = 4;
+
 template <bool VEC_A, bool VEC_B>
-__global__ __launch_bounds__(64) void mlp_gemm_kernel(View a, View b, int M, int K, int N,
-                                                      int epilogue, const float* __restrict__ bias,
-                                                      float* __restrict__ c, float* __restrict__ h)
+__device__ __forceinline__ void gemm_step(const View& a, const View& bt, int am, int bn, int M,
+                                          int N, int k, int K, bf16x8& av, bf16x8& bv)
 {
-    const int l = threadIdx.x;
+    float x[8];
+    float y[8];
+    load8<VEC_A>(a, am, M, k, K, x);
+    load8<VEC_B>(bt, bn, N, k, K, y);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        av[q] = static_cast<__bf16>(x[q]);
+        bv[q] = static_cast<__bf16>(y[q]);
+    }
+}
+
+template <bool VEC_A, bool VEC_B>
+__global__ __launch_bounds__(64 * kGemmWaves) void mlp_gemm_kernel(
+    View a, View b, int M, int K, int N, int epilogue, const float* __restrict__ bias,
+    float* __restrict__ c, float* __restrict__ h)
+{
+    __shared__ float partial[kGemmWaves - 1][16][64];
+    const int l = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
     const int m0 = blockIdx.y * 32;
     const int n0 = blockIdx.x * 32;
     const int i = l & 31;
@@ -132,19 +154,36 @@ __global__ __launch_bounds__(64) void mlp_gemm_kernel(View a, View b, int M, int
     for (int v = 0; v < 16; ++v) {
         acc[v] = 0.0f;
     }
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        float x[8];
-        float y[8];
-        load8<VEC_A>(a, am, M, k0 + kq, K, x);
-        load8<VEC_B>(bt, n0 + i, N, k0 + kq, K, y);
-        bf16x8 av;
-        bf16x8 bv;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            av[q] = static_cast<__bf16>(x[q]);
-            bv[q] = static_cast<__bf16>(y[q]);
+    const int steps = (K + 15) >> 4;
+    for (int s = w; s < steps; s += 2 * kGemmWaves) {
+        bf16x8 av0, bv0, av1, bv1;
+        const bool second = s + kGemmWaves < steps;
+        gemm_step<VEC_A, VEC_B>(a, bt, am, n0 + i, M, N, 16 * s + kq, K, av0, bv0);
+        if (second) {
+            gemm_step<VEC_A, VEC_B>(a, bt, am, n0 + i, M, N, 16 * (s + kGemmWaves) + kq, K, av1,
+                                    bv1);
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av0, bv0, acc, 0, 0, 0);
+        if (second) {
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av1, bv1, acc, 0, 0, 0);
+        }
+    }
+    if (w > 0) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            partial[w - 1][v][l] = acc[v];
+        }
+    }
+    __syncthreads();
+    if (w > 0) {
+        return;
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+#pragma unroll
+        for (int u = 0; u < kGemmWaves - 1; ++u) {
+            acc[v] += partial[u][v][l];
+        }
     }
     const int col = n0 + i;
 #pragma unroll
@@ -169,17 +208,22 @@ void gemm(View a, View b, int M, int K, int N, int epilogue, const float* bias, 
           float* h, hipStream_t stream)
 {
     const dim3 grid(static_cast<unsigned>(N / 32), static_cast<unsigned>(ceil_div(M, 32)));
+    const unsigned threads = 64 * kGemmWaves;
     // 16-byte loads need k-contiguous operands at 16-byte aligned rows (strides multiple of 4)
     const bool va = a.sc == 1 && a.sr % 4 == 0;
     const bool vb = b.sr == 1 && b.sc % 4 == 0;
     if (va && vb) {
-        mlp_gemm_kernel<true, true><<<grid, 64, 0, stream>>>(a, b, M, K, N, epilogue, bias, c, h);
+        mlp_gemm_kernel<true, true><<<grid, threads, 0, stream>>>(a, b, M, K, N, epilogue, bias,
+                                                                 c, h);
     } else if (va) {
-        mlp_gemm_kernel<true, false><<<grid, 64, 0, stream>>>(a, b, M, K, N, epilogue, bias, c, h);
+        mlp_gemm_kernel<true, false><<<grid, threads, 0, stream>>>(a, b, M, K, N, epilogue, bias,
+                                                                  c, h);
     } else if (vb) {
-        mlp_gemm_kernel<false, true><<<grid, 64, 0, stream>>>(a, b, M, K, N, epilogue, bias, c, h);
+        mlp_gemm_kernel<false, true><<<grid, threads, 0, stream>>>(a, b, M, K, N, epilogue, bias,
+                                                                  c, h);
     } else {
-        mlp_gemm_kernel<false, false><<<grid, 64, 0, stream>>>(a, b, M, K, N, epilogue, bias, c, h);
+        mlp_gemm_kernel<false, false><<<grid, threads, 0, stream>>>(a, b, M, K, N, epilogue,
+                                                                   bias, c, h);
     }
     check_launch("mlp_gemm_kernel");
 }
