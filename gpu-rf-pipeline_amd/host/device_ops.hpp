@@ -49,6 +49,28 @@ inline void fir_load_const_taps(const float* /*taps*/, int num_taps, rf_stream /
     }
 }
 
+// A filter prepared once for many blocks (rf_fir_plan_*): construction copies the device taps
+// (and, for fft_ola, makes their spectrum) and synchronises; run() enqueues the filtering
+// kernel only, with the bits of fir_launch.
+class FirPlan {
+public:
+    FirPlan(FirImpl impl, const float* taps, int num_taps, rf_stream stream = nullptr)
+    {
+        RF_CHECK(rf_fir_plan_create(static_cast<int>(impl), taps, num_taps, 0, stream, &handle_));
+    }
+    ~FirPlan() { rf_fir_plan_destroy(handle_); }
+    FirPlan(const FirPlan&) = delete;
+    FirPlan& operator=(const FirPlan&) = delete;
+
+    void run(const cf32* x, cf32* y, int num_samples, rf_stream stream = nullptr) const
+    {
+        RF_CHECK(rf_fir_plan_run(handle_, as_rf(x), as_rf(y), num_samples, stream));
+    }
+
+private:
+    rf_fir_plan* handle_ = nullptr;
+};
+
 inline void fir_device(FirImpl impl, const cf32* x, const float* taps, cf32* y, int num_samples,
                        int num_taps)
 {

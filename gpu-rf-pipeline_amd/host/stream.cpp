@@ -126,6 +126,9 @@ std::vector<cf32> fir_stream(const std::vector<cf32>& x, const std::vector<float
     DeviceBuffer<float> d_taps(taps.size());
     d_taps.upload(taps.data(), taps.size());
     fir_load_const_taps(d_taps.get(), num_taps, nullptr);
+    // The filter is the same for every chunk: prepared once (for fft_ola its spectrum too), so
+    // each chunk launches only the filtering kernel.
+    const FirPlan plan(config.impl, d_taps.get(), num_taps);
     RF_CHECK(rf_device_synchronize());
 
     const auto retire = [&](Slot& slot) {
@@ -156,13 +159,11 @@ std::vector<cf32> fir_stream(const std::vector<cf32>& x, const std::vector<float
         slot.offset = start - in_start;
         const rf_stream s = slot.stream.get();
         if (config.memory == StreamMemory::ZeroCopy) {
-            fir_launch(config.impl, slot.in.device(), d_taps.get(), slot.out.device(),
-                       static_cast<int>(in_len), num_taps, s);
+            plan.run(slot.in.device(), slot.out.device(), static_cast<int>(in_len), s);
         } else {
             const cf32* source = whole ? x_pinned.host() + in_start : slot.in.host();
             RF_CHECK(rf_memcpy_h2d(slot.dev_in.get(), source, in_len * sizeof(cf32), s));
-            fir_launch(config.impl, slot.dev_in.get(), d_taps.get(), slot.dev_out.get(),
-                       static_cast<int>(in_len), num_taps, s);
+            plan.run(slot.dev_in.get(), slot.dev_out.get(), static_cast<int>(in_len), s);
             cf32* target = whole ? y_pinned.host() + start : slot.out.host() + slot.offset;
             RF_CHECK(rf_memcpy_d2h(target, slot.dev_out.get() + slot.offset,
                                    slot.count * sizeof(cf32), s));
