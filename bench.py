@@ -50,7 +50,7 @@ def load_rfhip():
 
 def load_oracle():
     sys.path.insert(0, str(ROOT / "oracle"))
-    import oracle  # the checker and the CPU baseline
+    import oracle  # the float64 oracle: result checks and the CPU baseline
     return oracle
 
 

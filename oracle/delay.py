@@ -1,7 +1,7 @@
 """float64 restatement of the learned delay search (librfhip's delay_synth.hip, delay_search.hip
 and delay_mlp.hip), which has no counterpart in the reference.
 
-TEST INFRASTRUCTURE ONLY: the checker the GPU path is compared with. Only tests/,
+TEST INFRASTRUCTURE ONLY: the oracle the GPU path is compared with. Only tests/,
 __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
 
 What pins each part (tests/test_oracle_delay.py):

@@ -1,8 +1,8 @@
 /*
  * oracle.c — float64 CPU restatement of gpu-rf-pipeline's hot-path definitions.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the GPU path (tests, smoke, bench's CPU baseline).
- * Nothing in the shipped library calls it.
+ * TEST INFRASTRUCTURE ONLY: the oracle the GPU path is compared with (tests, smoke, bench's
+ * CPU baseline). Nothing in the shipped library calls it.
  *
  * Every sum runs in the order of its definition, in float64. The FFT is an iterative radix-2
  * Cooley-Tukey transform with every twiddle computed directly by cos/sin of an exactly

@@ -1,7 +1,7 @@
 /*
  * oracle.h — float64 CPU restatement of gpu-rf-pipeline's hot-path definitions.
  *
- * TEST INFRASTRUCTURE ONLY. This is the checker the GPU results are compared with. Only the
+ * TEST INFRASTRUCTURE ONLY. This is the oracle the GPU results are compared with. Only the
  * tests, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the library
  * (librfhip.so) never links or calls it.
  *

@@ -1,6 +1,6 @@
 """numpy front end of the float64 C oracle (oracle.c).
 
-TEST INFRASTRUCTURE ONLY: the checker the GPU path is compared with. Only tests/,
+TEST INFRASTRUCTURE ONLY: the oracle the GPU path is compared with. Only tests/,
 __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
 
 Every function takes and returns numpy arrays (complex128 / float64) and restates one

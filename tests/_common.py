@@ -1,5 +1,5 @@
 """Shared helpers of the test suite: fixture I/O, and loaders for the library binding and the
-float64 oracle (the checker)."""
+float64 oracle."""
 from __future__ import annotations
 
 import importlib.util
@@ -16,7 +16,7 @@ PACKAGE_DIR = ROOT / "gpu-rf-pipeline_amd"
 LIB = PACKAGE_DIR / "librfhip.so"
 
 sys.path.insert(0, str(ROOT / "oracle"))
-import oracle  # noqa: E402  (the checker)
+import oracle  # noqa: E402  (the float64 oracle)
 
 
 def load_rfhip():
