@@ -25,8 +25,12 @@
 //   6 fallback: inverse row and column passes with the fused peak search of rf_gcc_batch.
 // Peaks and the sub-sample offset follow rf_gcc_batch (first largest |r|; f = m1 / (m0 + m1)
 // towards the larger neighbour).
+// For L = 2^15 (n <= 2^14) steps 1-2 run as one workgroup per pair instead (gcc_pair32k.hpp:
+// even and odd half-spectra of 16384 points), which stores the same W and the band sums of each
+// half, and the fallback runs rf_gcc_batch's single-workgroup GCC-PHAT of those pairs.
 
 #include "gcc_batch_kernels.hpp"
+#include "gcc_pair32k.hpp"
 #include "fir_internal.hpp"
 #include "twiddle.hpp"
 
@@ -65,6 +69,9 @@ struct rf_delay_search {
     float* probs = nullptr;     // count * 64
     int* active = nullptr;      // count: 1 = full inverse transform
     int* fallback = nullptr;    // count + 1: number of fallback pairs, then their indices
+    bool fused = false;         // L = 2^15: one workgroup per pair (gcc_pair32k.hpp)
+    float2* scratch = nullptr;  // fused: pair32k_scratch_values()
+    float2* derot32k = nullptr; // fused: kDerot32kSize de-rotation factors
     std::size_t bytes = 0;
 };
 
@@ -198,6 +205,99 @@ __global__ __launch_bounds__(256) void search_features_kernel(
         }
         const int index = static_cast<int>(0xFFFFFFFFu - static_cast<unsigned>(key & 0xFFFFFFFFULL));
         coarse[p] = index >> 4;  // (lag + 512) / 16
+    }
+}
+
+// Sum over each row of 16 lanes (DPP rows: lanes 16 i .. 16 i + 15), left in all 16 lanes.
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float x)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), CTRL,
+                                                              0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float row16_sum(float v)
+{
+    v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_move<0x124>(v);  // row_ror:4
+    v += dpp_move<0x128>(v);  // row_ror:8
+    return v;
+}
+
+// De-rotation factors exp(+2 pi i k (n - 1) / L) of bin k = 2 (t + 1024 j) + o of the L = 2^15
+// search: lane part exp(+2 pi i 2 t (n - 1) / L) (t < 1024), then row parts
+// exp(+2 pi i (2048 j + o) (n - 1) / L) for o = 0 (j < 16) and o = 1 (j < 16).
+constexpr int kDerot32kRow = kPair32kLanes;
+constexpr int kDerot32kSize = kDerot32kRow + 32;
+
+// One half of the L = 2^15 forward search step of pair d: the PHAT weights of the even (ODD
+// false) or odd bins of the rotated a (as the other lengths store them, rotated by the
+// de-rotation factors' conjugates) into W, in the four-step order of the other lengths (bin k
+// at (k mod 8) 4096 + k / 8; bin 2 k' + o, k' = t + 1024 j, at (2 (t mod 4) + o) 4096 + t / 4 +
+// 256 j), and the de-rotated sums of the 1024 bands of 32 bins, 16 of them per half, into
+// partial[half][band] (band (t + 1024 j) / 16: the 16 lanes of a DPP row, whose lane j keeps
+// the sum of step j for one store per lane).
+template <bool ODD>
+__device__ __forceinline__ void search_pair32k_half(const PairDesc& d, int t, float2* u,
+                                                    float2* v, float2* buf, const float2* tw,
+                                                    float2 fold_t, float2 rot_t,
+                                                    const float2* __restrict__ derot,
+                                                    float2* __restrict__ partial,
+                                                    const float2* __restrict__ table)
+{
+    constexpr int P = kPair32kLanes;
+    constexpr int V = kPair32kHalf / P;
+    pair32k_spectra<ODD, P>(d, t, u, v, buf, tw, fold_t, table);
+    const float scale = 1.0f / static_cast<float>(1 << kPair32kLog2L);
+    const __amdgpu_buffer_rsrc_t W = __builtin_amdgcn_make_buffer_rsrc(
+        d.work + (1 << kPair32kLog2L), static_cast<short>(0), -1, 0x00020000);
+    const unsigned slot = static_cast<unsigned>(((2 * (t & 3) + (ODD ? 1 : 0)) << k4096Log2)
+                                                + (t >> 2)) * 8u;
+    float2 mine = make_float2(0.0f, 0.0f);  // lane t keeps the sum of band t / 16 + 64 (t % 16)
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+        // a is not rotated here, so x is already the de-rotated spectrum (lag 0 at index 0)
+        // whose band sums the features take; the refine reads the spectrum of the rotated a,
+        // x exp(-2 pi i k (n - 1) / L)
+        float2 x = phat_weight(u[j], v[j], scale);
+        const float2 w = cmulconj(x, cmul(rot_t, derot[kDerot32kRow + (ODD ? 16 : 0) + j]));
+        using u32x2 = unsigned __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, w), W, slot,
+                                              static_cast<unsigned>(256 * j) * 8u, 0);
+        x.x = row16_sum(x.x);
+        x.y = row16_sum(x.y);
+        mine = (t & 15) == j ? x : mine;
+    }
+    gstore(partial, (ODD ? kBands : 0) + (t >> 4) + 64 * (t & 15), mine);
+}
+
+// Steps 1-2 for L = 2^15, one workgroup per pair.
+__global__ __launch_bounds__(kPair32kLanes) void search_pair32k_kernel(
+    const PairDesc* __restrict__ pairs, int count, float2* __restrict__ partial,
+    const float2* __restrict__ derot, const float2* __restrict__ table)
+{
+    constexpr int P = kPair32kLanes;
+    constexpr int V = kPair32kHalf / P;
+    __shared__ float2 buf[k16384Slots];
+    __shared__ float2 tw[kTwLds];
+    const int lane = threadIdx.x;
+    fft16384_load_tables(tw, table, lane, P);
+    const float2 fold_t = table[kFoldLane + lane];
+    const float2 rot_t = derot[lane];
+    float2 u[V];
+    float2 v[V];
+    __syncthreads();  // tw
+    for (int p = blockIdx.x; p < count; p += gridDim.x) {
+        // the lane index through an empty asm: no per-lane addresses hoisted out of the loop
+        int t = lane;
+        asm volatile("" : "+v"(t));
+        const PairDesc d = pairs[p];
+        float2* out = partial + static_cast<long long>(p) * 2 * kBands;
+        search_pair32k_half<false>(d, t, u, v, buf, tw, fold_t, rot_t, derot, out, table);
+        __syncthreads();
+        search_pair32k_half<true>(d, t, u, v, buf, tw, fold_t, rot_t, derot, out, table);
+        __syncthreads();  // the next pair reuses buf
     }
 }
 
@@ -394,6 +494,16 @@ void features_launch(rf_delay_search& h, const float2* a, const float2* b, hipSt
     search_bind_kernel<<<static_cast<unsigned>(ceil_div(h.count, 256)), 256, 0, stream>>>(
         h.dev, h.count, a, b, h.n);
     check_launch("search_bind_kernel");
+    if (h.fused) {
+        search_pair32k_kernel<<<static_cast<unsigned>(pair32k_grid(h.count)), kPair32kLanes, 0,
+                                stream>>>(h.dev, h.count, h.partial, h.derot32k,
+                                          fft16384_table());
+        check_launch("search_pair32k_kernel");
+        search_features_kernel<<<static_cast<unsigned>(h.count), 256, 0, stream>>>(
+            h.partial, 1, h.features, h.coarse, table);  // two rows: even and odd bins
+        check_launch("search_features_kernel");
+        return;
+    }
     batch_column_fwd_kernel<<<persistent_grid(2LL * h.col_total, 4), kItemThreads,
                               column_lds_bytes(), stream>>>(h.dev, h.col_map, 0, h.col_total,
                                                             table);
@@ -430,6 +540,12 @@ void search_launch(rf_delay_search& h, const float2* a, const float2* b, rf_dela
     search_refine_kernel<<<static_cast<unsigned>(h.count), kRefineThreads, 0, stream>>>(
         h.dev, results, h.active, h.fallback, table);
     check_launch("search_refine_kernel");
+    if (h.fused) {
+        // fallback pairs (their count stays on the device): the single-workgroup GCC-PHAT
+        launch_pair32k(h.dev, h.fallback + 1, h.fallback, h.count, h.count, results, h.scratch,
+                       stream);
+        return;
+    }
     // Fallback pairs only (their count stays on the device): grids sized for every pair.
     const int rows = h.count << h.fs.log2n2;
     search_inverse_row_kernel<<<persistent_grid(rows, 2), k4096Lanes, 0, stream>>>(
@@ -462,6 +578,8 @@ void destroy_search(rf_delay_search* h)
     (void)hipFree(h->probs);
     (void)hipFree(h->active);
     (void)hipFree(h->fallback);
+    (void)hipFree(h->scratch);
+    (void)hipFree(h->derot32k);
     delete h;
 }
 
@@ -511,6 +629,34 @@ rf_delay_search* create_search(int count, int n)
             RF_HIP_CHECK(hipMemcpy(h->derotate, host.data(), host.size() * sizeof(float2),
                                    hipMemcpyHostToDevice));
         }
+        h->fused = h->log2L == kPair32kLog2L;
+        if (h->fused) {
+            (void)fft16384_table();
+            const std::size_t scratch_bytes =
+                static_cast<std::size_t>(pair32k_scratch_values()) * sizeof(float2);
+            RF_HIP_CHECK(hipMalloc(&h->scratch, scratch_bytes));
+            std::vector<float2> host(kDerot32kSize);
+            const long long rot = n - 1;
+            const double two_pi = 6.283185307179586476925286766559;
+            auto factor = [&](long long k) {
+                const long long m = (k % L) * (rot % L) % L;
+                const double a = two_pi * static_cast<double>(m) / static_cast<double>(L);
+                return make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+            };
+            for (int t = 0; t < kPair32kLanes; ++t) {
+                host[static_cast<std::size_t>(t)] = factor(2LL * t);
+            }
+            for (int o = 0; o < 2; ++o) {
+                for (int j = 0; j < 16; ++j) {
+                    host[static_cast<std::size_t>(kDerot32kRow + 16 * o + j)] =
+                        factor(2048LL * j + o);
+                }
+            }
+            RF_HIP_CHECK(hipMalloc(&h->derot32k, host.size() * sizeof(float2)));
+            RF_HIP_CHECK(hipMemcpy(h->derot32k, host.data(), host.size() * sizeof(float2),
+                                   hipMemcpyHostToDevice));
+            h->bytes += scratch_bytes + host.size() * sizeof(float2);
+        }
         const long long partial = static_cast<long long>(count) * n2 * kBands;
         RF_HIP_CHECK(hipMalloc(&h->partial, partial * sizeof(float2)));
         RF_HIP_CHECK(hipMalloc(&h->features, static_cast<std::size_t>(count) * kFeatures * sizeof(float)));
@@ -534,6 +680,7 @@ rf_delay_search* create_search(int count, int n)
             d.work = h->work + 2LL * L * i;
             d.lo = h->tables;
             d.hi = h->tables + kTwiddleSize;
+            d.fused = h->fused ? 1 : 0;
             for (int c = 0; c < col_items; ++c) {
                 col_map.push_back(make_int2(i, c));
             }
@@ -545,7 +692,7 @@ rf_delay_search* create_search(int count, int n)
         RF_HIP_CHECK(hipMalloc(&h->col_map, col_map.size() * sizeof(int2)));
         RF_HIP_CHECK(hipMemcpy(h->col_map, col_map.data(), col_map.size() * sizeof(int2),
                                hipMemcpyHostToDevice));
-        h->bytes = (tsize + work + partial + L) * sizeof(float2)
+        h->bytes += (tsize + work + partial + L) * sizeof(float2)
                    + static_cast<std::size_t>(count) * (kFeatures + kBins + 2) * sizeof(float)
                    + h->host.size() * sizeof(PairDesc) + col_map.size() * sizeof(int2);
     } catch (...) {

@@ -17,11 +17,14 @@
 // Pairs need 4096 <= L <= 2^24 (L = next power of two >= Na + Nb - 1, so Na + Nb - 1 > 2048);
 // GCC-PHAT's weights depend on L, so shorter pairs are not padded up but refused (the
 // single-pair rf_xcorr_fft handles them in one workgroup). All items cost the same; a
-// persistent grid walks them in XCD-major order. Pairs whose rows are 4096 points (L >= 2^15)
+// persistent grid walks them in XCD-major order. Pairs whose rows are 4096 points (L >= 2^16)
 // run their row pass as one 4096-point row per item on 256 lanes (register-resident radix-16
-// transforms); shorter rows use the generic LDS transform, F rows per item.
+// transforms); shorter rows use the generic LDS transform, F rows per item. Pairs with
+// L = 2^15 skip the passes: one workgroup per pair does their whole GCC-PHAT and peak from
+// registers and LDS (gcc_pair32k.hpp), reading each input from HBM once per half-spectrum.
 
 #include "gcc_batch_kernels.hpp"
+#include "gcc_pair32k.hpp"
 #include "twiddle.hpp"
 
 #include <algorithm>
@@ -62,6 +65,10 @@ struct rf_gcc_batch {
     std::vector<Group> groups;
     std::size_t group_budget = 0;
     float2* work = nullptr;
+    // Pairs with L = 2^15: one workgroup each (gcc_pair32k.hpp).
+    int fused_count = 0;
+    int* fused_list = nullptr;  // their pair indices
+    float2* scratch = nullptr;  // pair32k_scratch_values()
     std::map<int, float2*> tables;
     std::size_t bytes = 0;
     int grid_mult = 2;          // persistent workgroups per CU
@@ -159,8 +166,9 @@ void make_groups(rf_gcc_batch& h, std::size_t budget)
         int p1 = p0;
         std::size_t bytes = 0;
         while (p1 < h.count) {
+            const PairDesc& d = h.host[static_cast<std::size_t>(p1)];
             const std::size_t pair_bytes =
-                (static_cast<std::size_t>(2) << h.host[static_cast<std::size_t>(p1)].log2L) * sizeof(float2);
+                d.fused != 0 ? 0 : (static_cast<std::size_t>(2) << d.log2L) * sizeof(float2);
             if (p1 > p0 && budget != 0 && bytes + pair_bytes > budget) {
                 break;
             }
@@ -190,6 +198,10 @@ void batch_run(rf_gcc_batch& h, rf_peak_result* results, hipStream_t stream)
     batch_init_kernel<<<static_cast<unsigned>(ceil_div(h.count, 256)), 256, 0, stream>>>(results,
                                                                                          h.count);
     check_launch("batch_init_kernel");
+    if (h.fused_count > 0) {
+        launch_pair32k(h.dev, h.fused_list, static_cast<const int*>(nullptr), h.fused_count,
+                       h.fused_count, results, h.scratch, stream);
+    }
     // Persistent grids: grid_mult workgroups per CU, never more than the items, rounded up to a
     // multiple of 8 for the XCD-major walk (surplus workgroups find no item).
     const int cap = h.grid_mult * cus;
@@ -197,6 +209,9 @@ void batch_run(rf_gcc_batch& h, rf_peak_result* results, hipStream_t stream)
         return static_cast<unsigned>(ceil_div(std::min(items, cap), 8) * 8);
     };
     for (const rf_gcc_batch::Group& g : h.groups) {
+        if (g.col_items == 0) {
+            continue;  // only pairs of the single-workgroup path
+        }
         batch_column_fwd_kernel<<<grid_for(2 * g.col_items), kItemThreads, column_lds_bytes(),
                                   stream>>>(h.dev, h.col_map, g.col_begin, g.col_items, table);
         check_launch("batch_column_fwd_kernel");
@@ -237,6 +252,8 @@ void destroy_batch(rf_gcc_batch* h)
         (void)hipFree(list->map);
     }
     (void)hipFree(h->work);
+    (void)hipFree(h->fused_list);
+    (void)hipFree(h->scratch);
     for (auto& kv : h->tables) {
         (void)hipFree(kv.second);
     }
@@ -266,6 +283,7 @@ rf_status rf_gcc_batch_create(int count, const int* num_a, const int* num_b,
             std::vector<int> colp(static_cast<std::size_t>(count) + 1, 0);
             std::vector<int> row_pairs[2];            // 4096-point rows, other rows
             std::vector<int> row_prefix[2] = {{0}, {0}};
+            std::vector<int> fused;
             long long work_total = 0;
             for (int i = 0; i < count; ++i) {
                 if (num_a[i] < 1 || num_b[i] < 1) {
@@ -296,6 +314,12 @@ rf_status rf_gcc_batch_create(int count, const int* num_a, const int* num_b,
                 d.log2L = log2L;
                 d.log2n1 = fs.log2n1;
                 d.log2n2 = fs.log2n2;
+                if (log2L == rf::kPair32kLog2L) {
+                    d.fused = 1;
+                    fused.push_back(i);
+                    colp[static_cast<std::size_t>(i) + 1] = colp[static_cast<std::size_t>(i)];
+                    continue;
+                }
                 d.col_items = (1 << fs.log2n1) / rf::column_tile(fs);
                 d.row_items = (1 << fs.log2n2) / rf::lds_batch_per_block(fs.log2n1);
                 colp[static_cast<std::size_t>(i) + 1] = colp[static_cast<std::size_t>(i)] + d.col_items;
@@ -313,9 +337,25 @@ rf_status rf_gcc_batch_create(int count, const int* num_a, const int* num_b,
             rf::make_groups(*h, rf::kDefaultGroupBytes);
             (void)rf::twiddle_table_4096();
             (void)rf::fft4096_stage_table();
-            RF_HIP_CHECK(hipMalloc(&h->work, static_cast<std::size_t>(work_total) * sizeof(float2)));
+            if (work_total > 0) {
+                RF_HIP_CHECK(hipMalloc(&h->work, static_cast<std::size_t>(work_total) * sizeof(float2)));
+            }
+            h->fused_count = static_cast<int>(fused.size());
+            if (!fused.empty()) {
+                (void)rf::fft16384_table();
+                RF_HIP_CHECK(hipMalloc(&h->fused_list, fused.size() * sizeof(int)));
+                RF_HIP_CHECK(hipMemcpy(h->fused_list, fused.data(), fused.size() * sizeof(int),
+                                       hipMemcpyHostToDevice));
+                const std::size_t scratch_bytes =
+                    static_cast<std::size_t>(rf::pair32k_scratch_values()) * sizeof(float2);
+                RF_HIP_CHECK(hipMalloc(&h->scratch, scratch_bytes));
+                h->bytes += scratch_bytes + fused.size() * sizeof(int);
+            }
             long long offset = 0;
             for (auto& d : h->host) {
+                if (d.fused != 0) {
+                    continue;
+                }
                 d.work = h->work + offset;
                 offset += 2LL << d.log2L;
                 auto it = h->tables.find(d.log2L);
@@ -332,6 +372,9 @@ rf_status rf_gcc_batch_create(int count, const int* num_a, const int* num_b,
             }
             RF_HIP_CHECK(hipMalloc(&h->dev, h->host.size() * sizeof(PairDesc)));
             auto upload_map = [&](const std::vector<int2>& map, int2** dev) {
+                if (map.empty()) {
+                    return;  // every pair takes the single-workgroup path
+                }
                 RF_HIP_CHECK(hipMalloc(dev, map.size() * sizeof(int2)));
                 RF_HIP_CHECK(hipMemcpy(*dev, map.data(), map.size() * sizeof(int2),
                                        hipMemcpyHostToDevice));
@@ -418,6 +461,7 @@ rf_status rf_gcc_batch_launch_graph(rf_gcc_batch* h, rf_peak_result* results, rf
             }
             (void)rf::twiddle_table_4096();
             (void)rf::fft4096_stage_table();
+            (void)rf::fft16384_table();
             hipGraph_t graph = nullptr;
             RF_HIP_CHECK(hipStreamBeginCapture(h->capture, hipStreamCaptureModeThreadLocal));
             try {

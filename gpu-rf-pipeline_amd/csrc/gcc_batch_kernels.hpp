@@ -27,7 +27,7 @@ struct PairDesc {
     int log2n2;
     int col_items;    // n1 / C
     int row_items;    // n2 / F
-    int pad;
+    int fused;        // 1: L = 2^15, one workgroup per pair (gcc_pair32k.hpp); no work buffer
 };
 
 namespace {
@@ -220,6 +220,9 @@ __global__ __launch_bounds__(64) void batch_finish_kernel(const PairDesc* __rest
         return;
     }
     const PairDesc d = pairs[blockIdx.x];
+    if (d.fused != 0) {
+        return;  // finished by its own kernel
+    }
     const FourStep fs = shape_of(d);
     R* res = &results[blockIdx.x];
     const unsigned long long key = *reinterpret_cast<const unsigned long long*>(res);

@@ -1,7 +1,7 @@
 // Twiddle tables (float64 on the host, stored as float32 on the device).
 
 #include "twiddle.hpp"
-#include "fft_core.hpp"
+#include "fft16384.hpp"
 
 #include <cmath>
 #include <mutex>
@@ -81,6 +81,42 @@ const float2* fft4096_stage_table()
         fail(RF_HIP_ERROR, "twiddle table: initialisation failed on an earlier call");
     }
     return g_stage[device];
+}
+
+namespace {
+std::once_flag g_16384_once[kMaxDevices];
+float2* g_16384[kMaxDevices] = {};
+}  // namespace
+
+const float2* fft16384_table()
+{
+    int device = 0;
+    RF_HIP_CHECK(hipGetDevice(&device));
+    if (device < 0 || device >= kMaxDevices) {
+        fail(RF_LIMIT, "twiddle table: device index out of range");
+    }
+    std::call_once(g_16384_once[device], [device] {
+        std::vector<float2> host(kTw16384Size, make_float2(0.0f, 0.0f));
+        for (int k = 0; k < 4; ++k) {
+            fill_twiddles(host.data() + kTw4 + 16 * k, 16, 64, k);      // exp(-2 pi i r k / 64)
+        }
+        for (int k = 0; k < 64; ++k) {
+            fill_twiddles(host.data() + kTw64 + 17 * k, 16, 1024, k);   // exp(-2 pi i r k / 1024)
+        }
+        fill_twiddles(host.data() + kTwHi, 128, 128, 1);
+        fill_twiddles(host.data() + kTwLo, 128, 16384, 1);
+        fill_twiddles(host.data() + kFoldLane, 1024, 32768, 1);
+        fill_twiddles(host.data() + kFoldRow, 32, 64, 1);
+        float2* dev = nullptr;
+        RF_HIP_CHECK(hipMalloc(&dev, host.size() * sizeof(float2)));
+        RF_HIP_CHECK(hipMemcpy(dev, host.data(), host.size() * sizeof(float2),
+                               hipMemcpyHostToDevice));
+        g_16384[device] = dev;
+    });
+    if (g_16384[device] == nullptr) {
+        fail(RF_HIP_ERROR, "twiddle table: initialisation failed on an earlier call");
+    }
+    return g_16384[device];
 }
 
 }  // namespace rf

@@ -14,6 +14,10 @@ const float2* twiddle_table_4096();
 // values).
 const float2* fft4096_stage_table();
 
+// Factors of the 16384-point transform on 1024 lanes and of the 2^15-point split into two
+// halves, laid out as the kTw* / kFold* offsets of fft16384.hpp (kTw16384Size values).
+const float2* fft16384_table();
+
 // Host-side float64 -> float32 table of exp(-2 pi i m / n) for m < count.
 void fill_twiddles(float2* host, long long count, long long n, long long step);
 
