@@ -74,6 +74,13 @@ struct rf_gcc_batch {
     int grid_mult = 2;          // persistent workgroups per CU
     hipStream_t capture = nullptr;
     hipGraphExec_t graph = nullptr;
+    // When the batch has pairs of both kinds, the single-workgroup pairs run on a side stream
+    // beside the passes (forked from and joined back into the caller's stream), so that their
+    // one-pair-per-CU kernel fills in where the passes leave room instead of adding its whole
+    // length after them.
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr;
+    hipEvent_t join = nullptr;
     rf_peak_result* graph_results = nullptr;
 };
 
@@ -198,9 +205,17 @@ void batch_run(rf_gcc_batch& h, rf_peak_result* results, hipStream_t stream)
     batch_init_kernel<<<static_cast<unsigned>(ceil_div(h.count, 256)), 256, 0, stream>>>(results,
                                                                                          h.count);
     check_launch("batch_init_kernel");
+    const bool forked = h.fused_count > 0 && h.side != nullptr;
+    if (forked) {
+        RF_HIP_CHECK(hipEventRecord(h.fork, stream));
+        RF_HIP_CHECK(hipStreamWaitEvent(h.side, h.fork, 0));
+    }
     if (h.fused_count > 0) {
         launch_pair32k(h.dev, h.fused_list, static_cast<const int*>(nullptr), h.fused_count,
-                       h.fused_count, results, h.scratch, stream);
+                       h.fused_count, results, h.scratch, forked ? h.side : stream);
+    }
+    if (forked) {
+        RF_HIP_CHECK(hipEventRecord(h.join, h.side));
     }
     // Persistent grids: grid_mult workgroups per CU, never more than the items, rounded up to a
     // multiple of 8 for the XCD-major walk (surplus workgroups find no item).
@@ -233,6 +248,9 @@ void batch_run(rf_gcc_batch& h, rf_peak_result* results, hipStream_t stream)
     batch_finish_kernel<<<static_cast<unsigned>(h.count), 64, 0, stream>>>(h.dev, results,
                                                                          nullptr);
     check_launch("batch_finish_kernel");
+    if (forked) {
+        RF_HIP_CHECK(hipStreamWaitEvent(stream, h.join, 0));
+    }
 }
 
 void destroy_batch(rf_gcc_batch* h)
@@ -245,6 +263,15 @@ void destroy_batch(rf_gcc_batch* h)
     }
     if (h->capture != nullptr) {
         (void)hipStreamDestroy(h->capture);
+    }
+    if (h->side != nullptr) {
+        (void)hipStreamDestroy(h->side);
+    }
+    if (h->fork != nullptr) {
+        (void)hipEventDestroy(h->fork);
+    }
+    if (h->join != nullptr) {
+        (void)hipEventDestroy(h->join);
     }
     (void)hipFree(h->dev);
     (void)hipFree(h->col_map);
@@ -350,6 +377,11 @@ rf_status rf_gcc_batch_create(int count, const int* num_a, const int* num_b,
                     static_cast<std::size_t>(rf::pair32k_scratch_values()) * sizeof(float2);
                 RF_HIP_CHECK(hipMalloc(&h->scratch, scratch_bytes));
                 h->bytes += scratch_bytes + fused.size() * sizeof(int);
+                if (work_total > 0) {
+                    RF_HIP_CHECK(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+                    RF_HIP_CHECK(hipEventCreateWithFlags(&h->fork, hipEventDisableTiming));
+                    RF_HIP_CHECK(hipEventCreateWithFlags(&h->join, hipEventDisableTiming));
+                }
             }
             long long offset = 0;
             for (auto& d : h->host) {

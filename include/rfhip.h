@@ -262,7 +262,8 @@ rf_status rf_peak(const rf_cf32* r, int n, rf_peak_result* result, rf_stream str
  * work of gcc_phat_cuda (src/xcorr.cu:379-399) plus the host peak loop of rfgpu doa
  * (src/main.cpp:390-401) for `count` pairs of uneven length in one pass sequence of
  * persistent kernels (column FFTs, row FFTs + PHAT weighting + inverse row FFTs, inverse
- * column FFTs + peak), optionally replayed as one hipGraph.
+ * column FFTs + peak), optionally replayed as one hipGraph. Pairs whose FFT length is 2^15
+ * run in one workgroup each instead (whole GCC-PHAT and peak in registers and LDS).
  * Each pair needs num_a + num_b - 1 > 2048 (FFT length 4096 .. 2^24); RF_LIMIT otherwise.
  * ------------------------------------------------------------------------------------- */
 typedef struct rf_gcc_batch rf_gcc_batch;
