@@ -63,16 +63,6 @@ __device__ __forceinline__ void fft4096_load_twiddles(int t, const float2* stage
     load_factors(stages + 256 + 16 * t, tw.w256);
 }
 
-template <bool INV>
-__device__ __forceinline__ void twiddled_dft16(float2* u, const float2* w)
-{
-#pragma unroll
-    for (int r = 1; r < 16; ++r) {
-        u[r] = INV ? cmulconj(u[r], w[r]) : cmul(u[r], w[r]);
-    }
-    dft16<INV>(u);
-}
-
 // Span-1 pass from registers (inputs t + 256 r) into LDS at 16 t + r.
 template <bool INV>
 __device__ __forceinline__ void fft4096_first(float2* buf, int t, float2* u)

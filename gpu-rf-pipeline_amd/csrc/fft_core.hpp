@@ -150,6 +150,57 @@ __device__ __forceinline__ float2 rot16(float2 u, int e)
     return make_float2(u.x * c - u.y * si, u.x * si + u.y * c);
 }
 
+// a <- a + w b, b <- a - w b in six instructions: the sum as two FMA chains per component and
+// the difference as 2 a - sum (one FMA per component), instead of a complex multiply (four) and
+// an add and a subtract (four). The difference carries the sum's rounding (about one ulp of
+// the larger of the two results), as a plain butterfly after a rounded product does.
+__device__ __forceinline__ void fused_bfly(float2& a, float2& b, float2 w)
+{
+    const float sx = fmaf(w.x, b.x, fmaf(-w.y, b.y, a.x));
+    const float sy = fmaf(w.x, b.y, fmaf(w.y, b.x, a.y));
+    b = make_float2(fmaf(2.0f, a.x, -sx), fmaf(2.0f, a.y, -sy));
+    a = make_float2(sx, sy);
+}
+
+// a <- a + rot_e(c), c <- a - rot_e(c), rot_e = exp(-+2 pi i e / 16) for a compile-time e
+// (after unrolling): quarter turns are exact swaps and a plain butterfly, the others
+// fused_bfly with the constant factor.
+template <bool INV>
+__device__ __forceinline__ void rot_bfly(float2& a, float2& c, int e)
+{
+    if ((e & 3) == 0) {
+        const float2 r = rot16<INV>(c, e);
+        const float2 t = a;
+        a = cadd(t, r);
+        c = csub(t, r);
+        return;
+    }
+    fused_bfly(a, c, rot16<INV>(make_float2(1.0f, 0.0f), e));
+}
+
+// Second stage of the 16-point DFT: y[4 n2 + k1] holds output k1 of the first-stage DFT4 of
+// group n2, not yet multiplied by w16^(n2 k1); DFT4 over n2 with those factors folded into its
+// butterflies, outputs X[k1 + 4 k2] into v in natural order.
+template <bool INV>
+__device__ __forceinline__ void dft16_second(const float2* y, float2* v)
+{
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        float2 a = y[k1];
+        float2 b = rot16<INV>(y[4 + k1], k1);
+        float2 c = y[8 + k1];
+        float2 d = y[12 + k1];
+        rot_bfly<INV>(a, c, 2 * k1);   // a + w^2k1 c, a - w^2k1 c
+        rot_bfly<INV>(b, d, 3 * k1);   // w^k1 b + w^3k1 d, w^k1 b - w^3k1 d
+        // forward: -i * d13 = (d13.y, -d13.x); inverse: +i * d13 = (-d13.y, d13.x)
+        const float2 m = INV ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);
+        v[k1] = cadd(a, b);
+        v[k1 + 8] = csub(a, b);
+        v[k1 + 4] = cadd(c, m);
+        v[k1 + 12] = csub(c, m);
+    }
+}
+
 // 16-point DFT as 4 x 4: DFT4 over n1 of x[4 n1 + n2], twiddle w16^(n2 k1), DFT4 over n2;
 // output X[k1 + 4 k2] back into v in natural order.
 template <bool INV>
@@ -164,22 +215,36 @@ __device__ __forceinline__ void dft16(float2* v)
         float2 d = v[12 + n2];
         dft4<INV>(a, b, c, d);
         y[4 * n2 + 0] = a;
-        y[4 * n2 + 1] = rot16<INV>(b, n2);
-        y[4 * n2 + 2] = rot16<INV>(c, 2 * n2);
-        y[4 * n2 + 3] = rot16<INV>(d, 3 * n2);
+        y[4 * n2 + 1] = b;
+        y[4 * n2 + 2] = c;
+        y[4 * n2 + 3] = d;
     }
+    dft16_second<INV>(y, v);
+}
+
+// The 16-point DFT of (u[0], w[1] u[1], ..., w[15] u[15]) (conj(w[r]) for the inverse): in
+// each first-stage DFT4 the factors of the third and fourth inputs are folded into its two
+// input butterflies (fused_bfly), the first two are applied as products.
+template <bool INV>
+__device__ __forceinline__ void twiddled_dft16(float2* u, const float2* w)
+{
+    auto f = [&](int r) { return INV ? make_float2(w[r].x, -w[r].y) : w[r]; };
+    float2 y[16];
 #pragma unroll
-    for (int k1 = 0; k1 < 4; ++k1) {
-        float2 a = y[k1];
-        float2 b = y[4 + k1];
-        float2 c = y[8 + k1];
-        float2 d = y[12 + k1];
-        dft4<INV>(a, b, c, d);
-        v[k1] = a;
-        v[k1 + 4] = b;
-        v[k1 + 8] = c;
-        v[k1 + 12] = d;
+    for (int n2 = 0; n2 < 4; ++n2) {
+        float2 p0 = n2 == 0 ? u[0] : cmul(u[n2], f(n2));
+        float2 p1 = cmul(u[4 + n2], f(4 + n2));
+        float2 p2 = u[8 + n2];
+        float2 p3 = u[12 + n2];
+        fused_bfly(p0, p2, f(8 + n2));    // s02, d02
+        fused_bfly(p1, p3, f(12 + n2));   // s13, d13
+        const float2 m = INV ? make_float2(-p3.y, p3.x) : make_float2(p3.y, -p3.x);
+        y[4 * n2 + 0] = cadd(p0, p1);
+        y[4 * n2 + 2] = csub(p0, p1);
+        y[4 * n2 + 1] = cadd(p2, m);
+        y[4 * n2 + 3] = csub(p2, m);
     }
+    dft16_second<INV>(y, u);
 }
 
 // One radix-2^lr butterfly on u[0 .. R-1] with its Stockham twiddles. `k` = i mod p and
