@@ -248,7 +248,7 @@ __device__ __forceinline__ void search_pair32k_half(const PairDesc& d, int t, fl
 {
     constexpr int P = kPair32kLanes;
     constexpr int V = kPair32kHalf / P;
-    pair32k_spectra<ODD, P>(d, t, u, v, buf, tw, fold_t, table);
+    pair32k_spectra<ODD, P, false>(d, t, u, v, buf, tw, fold_t, table);  // see Fft16384Layout
     const float scale = 1.0f / static_cast<float>(1 << kPair32kLog2L);
     const __amdgpu_buffer_rsrc_t W = __builtin_amdgcn_make_buffer_rsrc(
         d.work + (1 << kPair32kLog2L), static_cast<short>(0), -1, 0x00020000);

@@ -126,15 +126,15 @@ __device__ __forceinline__ void apply_row_tree(float2* u, const float2* row)
     }
 }
 
-// Pass inputs t + P (b + B r) (butterfly t + P b, input r) into u[b + B r].
-template <int P>
-__device__ __forceinline__ void fft16384_pass_in(const float2* buf, int t, float2* u)
+// Pass inputs t + P j' (butterfly t + P b, input r: j' = b + B r) into u[j'], through the
+// exchange's layout: slot(t + P j') = base(t) + STRIDE j'.
+template <int P, int STRIDE>
+__device__ __forceinline__ void fft16384_pass_in(const float2* src, float2* u)
 {
     constexpr int B = 1024 / P;
-    const float2* src = buf + pad16(t);
 #pragma unroll
     for (int j = 0; j < 16 * B; ++j) {
-        u[j] = src[(17 * P / 16) * j];
+        u[j] = src[STRIDE * j];
     }
 }
 
@@ -143,15 +143,41 @@ __device__ __forceinline__ void fft16384_pass_in(const float2* buf, int t, float
 // t + P b per lane (inputs u[b + B r]) and the radix-4 pass 4 B. `buf` holds k16384Slots values
 // and `tw` the kTwLds factors; the caller separates two uses of `buf` with a barrier once every
 // lane has read what the previous transform left there.
-template <bool INV, int P>
+// LDS layouts of the three exchanges (write pattern, then read pattern t + P j'), as
+// base(t) + compile-time offsets. SPLIT: one layout per exchange, chosen so that 32 consecutive
+// lanes of each write and read land on distinct 8-byte bank pairs (j + j/32; j + j/32 +
+// 2 (j/64); no padding); otherwise j + j/16 for all three (conflict-free for 16 lanes only),
+// which needs fewer live address registers: measured, SPLIT makes the exact GCC-PHAT kernel
+// 7 % faster and the search's forward kernel, which spills more with it, 8 % slower.
+template <bool SPLIT, int P>
+struct Fft16384Layout {
+    // radix-4 outputs 4 t + 4 P q + s
+    static __device__ __forceinline__ int w1(int t) { return SPLIT ? 4 * t + (t >> 3) : 4 * t + (t >> 2); }
+    static constexpr int kW1 = SPLIT ? 33 * P / 8 : 17 * P / 4;
+    // reads of the span-4, span-64 and span-1024 passes
+    static __device__ __forceinline__ int r1(int t) { return SPLIT ? t + (t >> 5) : pad16(t); }
+    static constexpr int kR1 = SPLIT ? 33 * P / 32 : 17 * P / 16;
+    static __device__ __forceinline__ int r2(int t) { return SPLIT ? t + (t >> 5) + 2 * (t >> 6) : pad16(t); }
+    static constexpr int kR2 = 17 * P / 16;
+    static __device__ __forceinline__ int r3(int t) { return SPLIT ? t : pad16(t); }
+    static constexpr int kR3 = SPLIT ? P : 17 * P / 16;
+    // span-4 outputs 16 (t - k) + k + 4 r + 16 P b (k = t % 4): base 17 (t - k) + k in both
+    static __device__ __forceinline__ constexpr int w2(int r, int b) { return 17 * P * b + 4 * r + (SPLIT ? (r >> 3) : (r >> 2)); }
+    // span-64 outputs 16 (t - k) + k + 64 r + 16 P b (k = t % 64)
+    static __device__ __forceinline__ int w3(int t, int k) { return SPLIT ? 16 * (t - k) + k : 17 * (t - k) + pad16(k); }
+    static __device__ __forceinline__ constexpr int w3(int r, int b, int) { return SPLIT ? 16 * P * b + 64 * r : 17 * P * b + 68 * r; }
+};
+
+template <bool INV, int P, bool SPLIT = true>
 __device__ __forceinline__ void fft16384(float2* buf, const float2* tw, int t, float2* u)
 {
     static_assert(P == 1024 || P == 512, "16384-point transform on 1024 or 512 lanes");
     constexpr int B = 1024 / P;
+    using Lay = Fft16384Layout<SPLIT, P>;
     {
         // radix 4, span 1: butterfly t + P q (q < 4 B) takes inputs t + P q + 4096 s =
         // u[q + 4 B s] and writes outputs 4 t + 4 P q + s
-        float2* dst = buf + 4 * t + (t >> 2);
+        float2* dst = buf + Lay::w1(t);
 #pragma unroll
         for (int q = 0; q < 4 * B; ++q) {
             float2 a = u[q];
@@ -159,17 +185,17 @@ __device__ __forceinline__ void fft16384(float2* buf, const float2* tw, int t, f
             float2 c = u[q + 8 * B];
             float2 d = u[q + 12 * B];
             dft4<INV>(a, b, c, d);
-            dst[(17 * P / 4) * q + 0] = a;
-            dst[(17 * P / 4) * q + 1] = b;
-            dst[(17 * P / 4) * q + 2] = c;
-            dst[(17 * P / 4) * q + 3] = d;
+            dst[Lay::kW1 * q + 0] = a;
+            dst[Lay::kW1 * q + 1] = b;
+            dst[Lay::kW1 * q + 2] = c;
+            dst[Lay::kW1 * q + 3] = d;
         }
     }
     __syncthreads();
     float2 x[16];
     {
         // span 4: k = t % 4 for both butterflies of a lane (P is a multiple of 4)
-        fft16384_pass_in<P>(buf, t, u);
+        fft16384_pass_in<P, Lay::kR1>(buf + Lay::r1(t), u);
         __syncthreads();
         const int k = t & 3;
         float2* dst = buf + 17 * (t - k) + k;
@@ -183,17 +209,17 @@ __device__ __forceinline__ void fft16384(float2* buf, const float2* tw, int t, f
             dft16<INV>(x);
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                dst[17 * P * b + 4 * r + (r >> 2)] = x[r];
+                dst[Lay::w2(r, b)] = x[r];
             }
         }
     }
     __syncthreads();
     {
         // span 64: k = t % 64 for both butterflies
-        fft16384_pass_in<P>(buf, t, u);
+        fft16384_pass_in<P, Lay::kR2>(buf + Lay::r2(t), u);
         __syncthreads();
         const int k = t & 63;
-        float2* dst = buf + 17 * (t - k) + pad16(k);
+        float2* dst = buf + Lay::w3(t, k);
 #pragma unroll
         for (int b = 0; b < B; ++b) {
 #pragma unroll
@@ -204,14 +230,14 @@ __device__ __forceinline__ void fft16384(float2* buf, const float2* tw, int t, f
             dft16<INV>(x);
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                dst[17 * P * b + 68 * r] = x[r];
+                dst[Lay::w3(r, b, 0)] = x[r];
             }
         }
     }
     __syncthreads();
     {
         // span 1024: butterfly t + P b, outputs t + P b + 1024 r stay in u[b + B r]
-        fft16384_pass_in<P>(buf, t, u);
+        fft16384_pass_in<P, Lay::kR3>(buf + Lay::r3(t), u);
 #pragma unroll
         for (int b = 0; b < B; ++b) {
 #pragma unroll

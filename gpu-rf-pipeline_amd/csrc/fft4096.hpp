@@ -5,11 +5,14 @@
 // A lane starts with the inputs t + 256 r (r < 16) in registers and ends with the outputs
 // t + 256 r in registers, so a caller can read its inputs straight from HBM, apply a pointwise
 // operation between two transforms without touching LDS, and write its outputs straight back;
-// each transform makes two LDS round trips. LDS slot of element j: j + j/16, which puts the 16
-// lanes of each stride-16 write and of each (t/16)*256 + t%16 write on 16 different banks.
-// Every LDS access of a pass is one per-lane base plus a compile-time offset (slot(t + 256 r)
-// = slot(t) + 272 r, slot(16 t + r) = 17 t + r for r < 16, slot(j + 16 r) = slot(j) + 17 r),
-// so a pass keeps one address register instead of sixteen.
+// each transform makes two LDS round trips. Each round trip has its own LDS layout, chosen so
+// that 32 consecutive lanes of its write and of its read land on distinct 8-byte bank pairs:
+// the first (writes 16 t + r, reads t + 256 r) puts element j at slot1(j) = j + j/32, the
+// second (writes 256 (t/16) + 16 r + t%16, reads t + 256 r) at slot2(j) = j + j/32 + 2 (j/64).
+// With the earlier single layout j + j/16 (conflict-free for 16 lanes, not for 32) the overlap-
+// save kernel waited 23 % longer on LDS instructions and ran 2-3 % slower. Every LDS access of
+// a pass is one per-lane base plus a compile-time offset, so a pass keeps one address register
+// instead of sixteen.
 // Twiddles: per-device table [span][k][r] = exp(-2 pi i r k / (16 span)), r < 16, k < span,
 // span 16 at offset 0 and span 256 at offset 256 (fft4096_stage_table()); a lane needs the 15
 // factors r = 1..15 of row k = t % 16 of the span-16 table and of row k = t of the span-256
@@ -68,7 +71,7 @@ template <bool INV>
 __device__ __forceinline__ void fft4096_first(float2* buf, int t, float2* u)
 {
     dft16<INV>(u);
-    float2* dst = buf + 17 * t;
+    float2* dst = buf + 16 * t + (t >> 1);   // slot1(16 t + r) = 16 t + t / 2 + r
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         dst[r] = u[r];
@@ -81,18 +84,18 @@ template <bool INV>
 __device__ __forceinline__ void fft4096_middle(float2* buf, int t, const float2* w16)
 {
     float2 u[16];
-    const float2* src = buf + pad16(t);
+    const float2* src = buf + t + (t >> 5);   // slot1(t + 256 r) = t + t / 32 + 264 r
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        u[r] = src[272 * r];
+        u[r] = src[264 * r];
     }
     __syncthreads();
     twiddled_dft16<INV>(u, w16);
     const int k = t & 15;
-    float2* dst = buf + pad16((t - k) * 16 + k);
+    float2* dst = buf + 17 * (t - k) + k;     // slot2(256 (t / 16) + 16 r + t % 16)
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        dst[17 * r] = u[r];
+        dst[16 * r + (r >> 1) + 2 * (r >> 2)] = u[r];
     }
     __syncthreads();
 }
@@ -102,7 +105,7 @@ template <bool INV>
 __device__ __forceinline__ void fft4096_last(const float2* buf, int t, float2* u,
                                              const float2* w256)
 {
-    const float2* src = buf + pad16(t);
+    const float2* src = buf + t + (t >> 5) + 2 * (t >> 6);   // slot2(t + 256 r) = ... + 272 r
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         u[r] = src[272 * r];

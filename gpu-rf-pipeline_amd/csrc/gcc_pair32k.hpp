@@ -123,18 +123,18 @@ __device__ __forceinline__ float2 phat_weight(float2 fa, float2 fb, float scale)
 // never written by the kernels, so the compiler would otherwise move the loads across the
 // barriers (ahead of a transform) or keep a first load's registers for the second half. The
 // caller puts a barrier before the call once every lane has read `buf`.
-template <bool ODD, int P>
+template <bool ODD, int P, bool SPLIT = true>
 __device__ __forceinline__ void pair32k_spectra(const PairDesc& d, int t, float2* u, float2* v,
                                                 float2* buf, const float2* tw, float2 fold_t,
                                                 const float2* __restrict__ table)
 {
     pair32k_load_low<P>(opaque(d.a), d.num_a, t, u);
     pair32k_fold<ODD, P>(opaque(d.a), d.num_a, t, u, fold_t, table);
-    fft16384<false, P>(buf, tw, t, u);
+    fft16384<false, P, SPLIT>(buf, tw, t, u);
     pair32k_load_low<P>(opaque(d.b), d.num_b, t, v);
     pair32k_fold<ODD, P>(opaque(d.b), d.num_b, t, v, fold_t, table);
     __syncthreads();
-    fft16384<false, P>(buf, tw, t, v);
+    fft16384<false, P, SPLIT>(buf, tw, t, v);
 }
 
 // One workgroup of P lanes per pair, pairs blockIdx.x, blockIdx.x + gridDim.x, ... of the list
