@@ -105,14 +105,30 @@ __device__ __forceinline__ void column_pass_block(const FourStep& fs, long long 
     const int f = tid >> (fs.log2n2 - 3);  // n2 / 8 lanes per column transform
     const int t = tid & ((n2 >> 3) - 1);
     lds_fft<INV>(lds + (f < C ? f : 0) * slots, fs.log2n2, t, f < C, table);
+    // The lane's 8 elements share the column j1 and step k2 by n2 / 8: their four-step factors
+    // W^(j1 k2) are b s^i with b = W^(j1 k2_0) and s = W^(j1 n2 / 8), two table lookups
+    // instead of eight (at most four roundings more).
+    const unsigned j1 = static_cast<unsigned>(c0) + static_cast<unsigned>(tid & (C - 1));
+    const unsigned k20 = static_cast<unsigned>(tid >> lc);
+    const float2 b = four_step_twiddle32<INV>(fs, j1 * k20);
+    const float2 s1 = four_step_twiddle32<INV>(fs, j1 * static_cast<unsigned>(n2 >> 3));
+    float2 sp[8];
+    sp[0] = b;
+    sp[1] = cmul(b, s1);
+    const float2 s2 = cmul(s1, s1);
+    const float2 s4 = cmul(s2, s2);
+    sp[2] = cmul(b, s2);
+    sp[3] = cmul(sp[1], s2);
+    sp[4] = cmul(b, s4);
+    sp[5] = cmul(sp[1], s4);
+    sp[6] = cmul(sp[2], s4);
+    sp[7] = cmul(sp[3], s4);
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         const int e = tid + i * lanes;
         const int k2 = e >> lc;
         const int col = e & (C - 1);
-        const unsigned j1 = static_cast<unsigned>(c0) + static_cast<unsigned>(col);
-        v[i] = cmul(lds[col * slots + lds_pad(k2)],
-                    four_step_twiddle32<INV>(fs, j1 * static_cast<unsigned>(k2)));
+        v[i] = cmul(lds[col * slots + lds_pad(k2)], sp[i]);
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) {

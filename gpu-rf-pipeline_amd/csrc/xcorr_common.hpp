@@ -74,14 +74,35 @@ __device__ __forceinline__ void xcorr_row4096(const float2* __restrict__ wa,
     }
     __syncthreads();
     fft4096_staged<true>(buf, t, ub, stages);
-    // w_L^-(j1 row) with j1 row < L <= 2^24: 32-bit index arithmetic.
+    // w_L^-(j1 row), j1 = t + 256 r, as b s^r with b = w_L^(t row) and s = w_L^(256 row) (the
+    // same for every lane): four table lookups instead of 32, s^2, s^4, s^8 by squaring and
+    // each factor one product from an earlier one (at most five roundings deep). j1 row < L <=
+    // 2^24: 32-bit index arithmetic.
     const unsigned row32 = static_cast<unsigned>(row);
+    auto lookup = [&](unsigned m) {
+        return cmul(gload(fs.lo, m & (kTwiddleSize - 1)), gload(fs.hi, m >> kTwiddleLog2));
+    };
+    float2 w[16];
+    w[0] = lookup(static_cast<unsigned>(t) * row32);
+    const float2 s1 = lookup(static_cast<unsigned>(k4096Lanes) * row32);
+    const float2 s2 = cmul(s1, s1);
+    const float2 s4 = cmul(s2, s2);
+    const float2 s8 = cmul(s4, s4);
+    w[1] = cmul(w[0], s1);
+    w[2] = cmul(w[0], s2);
+    w[3] = cmul(w[1], s2);
+#pragma unroll
+    for (int r = 4; r < 8; ++r) {
+        w[r] = cmul(w[r - 4], s4);
+    }
+#pragma unroll
+    for (int r = 8; r < 16; ++r) {
+        w[r] = cmul(w[r - 8], s8);
+    }
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int j1 = t + r * k4096Lanes;
-        const unsigned m = static_cast<unsigned>(j1) * row32;
-        const float2 w = cmul(gload(fs.lo, m & (kTwiddleSize - 1)), gload(fs.hi, m >> kTwiddleLog2));
-        gstore(wb, j1, cmul(ub[r], make_float2(w.x, -w.y)));
+        gstore(wb, j1, cmulconj(ub[r], w[r]));
     }
 }
 
