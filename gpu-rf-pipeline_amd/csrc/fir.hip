@@ -330,7 +330,9 @@ __global__ __launch_bounds__(k4096Lanes) void fir_ola_taps_kernel(
     fft4096<false>(buf, t, u, stages);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        spec[t + r * k4096Lanes] = u[r];
+        // pairs (bins t + 256 r and t + 256 (r + 1), r even) at spec + 2 ((r / 2) 256 + t): the
+        // overlap-save kernel reads its two bins with one 16-byte load, contiguous over lanes
+        spec[2 * ((r >> 1) * k4096Lanes + t) + (r & 1)] = u[r];
     }
 }
 
@@ -389,10 +391,15 @@ __device__ __forceinline__ void ola_segment(const float2* __restrict__ x,
     Fft4096Twiddles tw;
     fft4096_load_twiddles(t, stages, tw);
     fft4096<false>(buf, t, u, tw);
+    // the tap spectrum in bin pairs (fir_ola_taps_kernel): 8 loads of 16 bytes per lane
     const __amdgpu_buffer_rsrc_t h = unchecked_buffer(spec);
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        u[r] = cmul(u[r], buffer_load2(h, lane_bytes, r * kBlockBytes));
+    for (int q = 0; q < 8; ++q) {
+        typedef float f32x4 __attribute__((ext_vector_type(4)));
+        const f32x4 hv = __builtin_bit_cast(
+            f32x4, __builtin_amdgcn_raw_buffer_load_b128(h, 2u * lane_bytes, q * 2u * kBlockBytes, 0));
+        u[2 * q] = cmul(u[2 * q], make_float2(hv.x, hv.y));
+        u[2 * q + 1] = cmul(u[2 * q + 1], make_float2(hv.z, hv.w));
     }
     __syncthreads();  // every lane has read the forward result before it is overwritten
     fft4096<true>(buf, t, u, tw);
