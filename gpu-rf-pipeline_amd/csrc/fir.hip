@@ -607,22 +607,6 @@ bool mfma_applies(int num_taps)
     return num_taps > kOlaMinTaps && num_taps <= kOlaMaxTaps;
 }
 
-// Dynamic LDS that holds the overlap-save kernels at three resident workgroups per CU. The
-// kernel needs 35 KiB of LDS and 126 registers, so four would fit; measured on MI355X
-// (2^24 samples, 1025 taps) three per CU run the whole launch about 5 % faster than four,
-// which contend for the same CU's VALU and memory pipes without hiding more latency. The
-// reserve is unused memory: each workgroup asks for a quarter of the CU's LDS plus 4 KiB.
-std::size_t ola_lds_reserve()
-{
-    const long long per_cu = lds_per_cu();
-    const long long used = static_cast<long long>(k4096Slots) * sizeof(float2);
-    const long long total = per_cu / 4 + 4096;
-    if (total <= used || 3 * total > per_cu || total > max_lds_per_block()) {
-        return 0;
-    }
-    return static_cast<std::size_t>(total - used);
-}
-
 void launch_ola_taps(const float* taps, int num_taps, float2* spec, hipStream_t stream)
 {
     fir_ola_taps_kernel<<<1, k4096Lanes, 0, stream>>>(taps, num_taps, spec,
@@ -637,7 +621,7 @@ void launch_ola(const float2* x, const float2* spec, float2* y, int n, int num_t
     const long long q = kOlaP - (num_taps - 1);
     const long long segments = ceil_div(n, q);
     const long long per_xcd = ceil_div(segments, 8);
-    fir_ola_kernel<<<static_cast<unsigned>(8 * per_xcd), k4096Lanes, ola_lds_reserve(), stream>>>(
+    fir_ola_kernel<<<static_cast<unsigned>(8 * per_xcd), k4096Lanes, 0, stream>>>(
         x, spec, y, n, num_taps, segments, per_xcd, fft4096_stage_table());
     check_launch("fir_ola_kernel");
 }
@@ -738,7 +722,7 @@ void fir_batched_launch(int impl, const FirBatch& batch, int count, int longest,
             launch_ola_taps(taps, num_taps, static_cast<float2*>(spec.ptr), stream);
             const long long q = kOlaP - (num_taps - 1);
             const dim3 grid(static_cast<unsigned>(ceil_div(longest, q)), count);
-            fir_ola_batched_kernel<<<grid, k4096Lanes, ola_lds_reserve(), stream>>>(
+            fir_ola_batched_kernel<<<grid, k4096Lanes, 0, stream>>>(
                 batch, static_cast<const float2*>(spec.ptr), num_taps, fft4096_stage_table());
             check_launch("fir_ola_batched_kernel");
             return;
