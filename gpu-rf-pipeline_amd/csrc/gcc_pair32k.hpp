@@ -38,9 +38,8 @@ namespace {
 constexpr int kPair32kLog2L = 15;
 constexpr int kPair32kHalf = 1 << (kPair32kLog2L - 1);  // 16384
 
-
 // The same pointer, opaque to the compiler, so that a load through it is issued where it is
-// written (see the kernel).
+// written (pair32k_spectra, pair32k_fold).
 template <class T>
 __device__ __forceinline__ T* opaque(T* p)
 {
