@@ -166,3 +166,32 @@ def test_group_schedules_give_identical_results(batch) -> None:
         assert [(p.index, p.magnitude, p.frac) for p in got] == want
         for r, o in zip(d_r, outs):
             np.testing.assert_array_equal(r.download(), o)
+
+
+def test_one_sample_channels_of_fft_length_2e15() -> None:
+    """A channel of one sample beside one of 16385..20000 (FFT length 2^15, the single-workgroup
+    path): every output against the oracle and the oracle's peak (same tie rule as above)."""
+    rng = np.random.default_rng(15)
+    shapes = [(1, 20000), (17000, 1), (16385, 1), (1, 16385)]
+    pairs = []
+    for na, nb in shapes:
+        a = (rng.standard_normal(na) + 1j * rng.standard_normal(na)).astype(np.complex64)
+        b = (rng.standard_normal(nb) + 1j * rng.standard_normal(nb)).astype(np.complex64)
+        pairs.append((a, b))
+    d_a = [rf.DeviceBuffer.from_array(a) for a, _ in pairs]
+    d_b = [rf.DeviceBuffer.from_array(b) for _, b in pairs]
+    d_r = [rf.DeviceBuffer(a.size + b.size - 1) for a, b in pairs]
+    h = rf.GccBatch([a.size for a, _ in pairs], [b.size for _, b in pairs])
+    h.bind([x.ptr for x in d_a], [x.ptr for x in d_b], [x.ptr for x in d_r])
+    res = rf.DeviceBuffer(16 * len(pairs), np.uint8)
+    h.run(res.ptr)
+    for (a, b), pk, r in zip(pairs, rf.read_peaks(res, len(pairs)), d_r):
+        want = oracle.gcc_phat(a, b)
+        got = r.download()
+        np.testing.assert_allclose(got, want, rtol=0, atol=GCC_ATOL)
+        ref_idx, _ = oracle.peak(want)
+        if pk.index != ref_idx:
+            ref = np.abs(want)
+            assert abs(ref[pk.index] - ref[ref_idx]) < 2 * GCC_ATOL
+        assert pk.magnitude == pytest.approx(float(np.abs(got[pk.index])), rel=1e-6)
+    h.destroy()
