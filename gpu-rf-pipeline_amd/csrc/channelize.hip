@@ -136,26 +136,58 @@ __global__ __launch_bounds__(kChanThreads) void channelize_fused_kernel(
         // costs one HBM round trip instead of one per loop iteration.
         constexpr int kLen = ((kChanOutputs >> LOG2M) + PB - 1) << LOG2M;
         constexpr int kIters = (kLen + kChanThreads - 1) / kChanThreads;
-        float2 v[kIters];
-        if (base >= 0 && base + kLen <= n) {
-            const float2* src = x + base;
+        // Interior groups whose samples base - 1 .. base + kLen start 16-byte aligned load
+        // them as pairs (one 16-byte load per two samples, half the load instructions; the
+        // first and last sample of the pairs fall outside the group's range and are dropped).
+        constexpr int kPairs = kLen / 2 + 1;
+        constexpr int kPairIters = (kPairs + kChanThreads - 1) / kChanThreads;
+        const unsigned long long pair0 =
+            reinterpret_cast<unsigned long long>(x) + static_cast<unsigned long long>(base - 1) * 8ULL;
+        if (base >= 1 && base + kLen + 1 <= n && (pair0 & 15ULL) == 0) {
+            typedef float f32x4 __attribute__((ext_vector_type(4)));
+            typedef __attribute__((address_space(1))) const f32x4 global_f32x4;
+            const global_f32x4* src = reinterpret_cast<const global_f32x4*>(pair0);
+            f32x4 w[kPairIters];
 #pragma unroll
-            for (int k = 0; k < kIters; ++k) {
-                const int i = threadIdx.x + k * kChanThreads;
-                v[k] = i < kLen ? gload(src, i) : make_float2(0.0f, 0.0f);
+            for (int k = 0; k < kPairIters; ++k) {
+                const int q = threadIdx.x + k * kChanThreads;
+                w[k] = q < kPairs ? src[q] : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            }
+#pragma unroll
+            for (int k = 0; k < kPairIters; ++k) {
+                const int q = threadIdx.x + k * kChanThreads;
+                const int i = 2 * q - 1;  // LDS index of the pair's first sample
+                if (q < kPairs) {
+                    if (i >= 0) {
+                        lds[i] = make_float2(w[k].x, w[k].y);
+                    }
+                    if (i + 1 < kLen) {
+                        lds[i + 1] = make_float2(w[k].z, w[k].w);
+                    }
+                }
             }
         } else {
+            float2 v[kIters];
+            if (base >= 0 && base + kLen <= n) {
+                const float2* src = x + base;
+#pragma unroll
+                for (int k = 0; k < kIters; ++k) {
+                    const int i = threadIdx.x + k * kChanThreads;
+                    v[k] = i < kLen ? gload(src, i) : make_float2(0.0f, 0.0f);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < kIters; ++k) {
+                    const int i = threadIdx.x + k * kChanThreads;
+                    v[k] = i < kLen ? load_sample(x, base + i, n) : make_float2(0.0f, 0.0f);
+                }
+            }
 #pragma unroll
             for (int k = 0; k < kIters; ++k) {
                 const int i = threadIdx.x + k * kChanThreads;
-                v[k] = i < kLen ? load_sample(x, base + i, n) : make_float2(0.0f, 0.0f);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < kIters; ++k) {
-            const int i = threadIdx.x + k * kChanThreads;
-            if (i < kLen) {
-                lds[i] = v[k];
+                if (i < kLen) {
+                    lds[i] = v[k];
+                }
             }
         }
     } else {
