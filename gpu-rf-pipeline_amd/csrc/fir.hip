@@ -376,7 +376,29 @@ __device__ __forceinline__ void ola_segment(const float2* __restrict__ x,
     const unsigned lane_bytes = static_cast<unsigned>(t) * 8u;
     constexpr unsigned kBlockBytes = k4096Lanes * 8u;
     float2 u[16];
-    if (in0 >= 0 && in0 + kOlaP <= n) {
+    if (in0 >= 0 && in0 + kOlaP <= n
+        && (reinterpret_cast<unsigned long long>(x + in0) & 15ULL) == 0) {
+        // Blocks 2 q and 2 q + 1 with one 16-byte load per lane: an even lane t reads samples
+        // t, t + 1 of block 2 q, an odd lane samples t - 1, t of block 2 q + 1; one DPP swap
+        // between the lane pair gives each lane sample t of both blocks.
+        const __amdgpu_buffer_rsrc_t src = unchecked_buffer(x + in0);
+        const bool odd = (t & 1) != 0;
+        const unsigned pair_bytes = odd ? (lane_bytes - 8u + kBlockBytes) : lane_bytes;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            typedef float f32x4 __attribute__((ext_vector_type(4)));
+            const f32x4 v = __builtin_bit_cast(
+                f32x4, __builtin_amdgcn_raw_buffer_load_b128(src, pair_bytes, q * 2u * kBlockBytes, 0));
+            const float sx = odd ? v.x : v.z;   // even lane sends sample t + 1 of block 2 q
+            const float sy = odd ? v.y : v.w;   // odd lane sends sample t - 1 of block 2 q + 1
+            const float rx = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(
+                                                           __builtin_bit_cast(int, sx), 0xB1, 0xF, 0xF, false));
+            const float ry = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(
+                                                           __builtin_bit_cast(int, sy), 0xB1, 0xF, 0xF, false));
+            u[2 * q] = odd ? make_float2(rx, ry) : make_float2(v.x, v.y);
+            u[2 * q + 1] = odd ? make_float2(v.z, v.w) : make_float2(rx, ry);
+        }
+    } else if (in0 >= 0 && in0 + kOlaP <= n) {
         const __amdgpu_buffer_rsrc_t src = unchecked_buffer(x + in0);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
