@@ -291,6 +291,38 @@ def test_fir_calls_replayed_from_a_graph() -> None:
     stream.destroy()
 
 
+def test_fir_ola_one_launch_calls_on_two_streams() -> None:
+    # A one-shot overlap-save call makes its tap spectrum inside the filtering launch
+    # (workgroup 0 writes it and a per-call ready flag; the others wait for the flag). Calls
+    # queued on two streams with different filters, with no synchronisation between them, each
+    # equal the prepared-filter result bit for bit.
+    shapes = [(70000, 1025), (4096, 66), (123457, 2049), (9000, 513), (300000, 1500), (5000, 200)]
+    rng = np.random.default_rng(7)
+    streams = [rf.Stream(), rf.Stream()]
+    ola = rf.fir_impl_index("fft_ola")
+    cases = []
+    for i, (n, num_taps) in enumerate(shapes):
+        x = noise(100 + i, n)
+        h = (rng.standard_normal(num_taps) / 20).astype(np.float32)
+        cases.append((x, h, rf.DeviceBuffer.from_array(x), rf.DeviceBuffer.from_array(h),
+                      rf.DeviceBuffer(n)))
+    for i, (x, h, d_x, d_h, d_y) in enumerate(cases):
+        rf.fir_device(ola, d_x.ptr, d_h.ptr, d_y.ptr, x.size, h.size, 0, streams[i % 2].handle)
+    for s in streams:
+        s.synchronize()
+    s = streams[0]
+    for x, h, d_x, d_h, d_y in cases:
+        print(f"one-shot fft_ola n={x.size} taps={h.size}", flush=True)
+        plan = rf.FirPlan("fft_ola", d_h.ptr, h.size)
+        want = rf.DeviceBuffer(x.size)
+        plan.run(d_x.ptr, want.ptr, x.size, s.handle)
+        s.synchronize()
+        np.testing.assert_array_equal(d_y.download(stream=s), want.download(stream=s))
+        plan.destroy()
+    for st in streams:
+        st.destroy()
+
+
 @pytest.mark.parametrize("num_taps", [66, 300, 777, 1500])
 def test_fir_ola_uneven_halo_and_lengths(num_taps: int) -> None:
     # Overlap-save segments of 4096 keep 4097 - T outputs; a halo T - 1 that is not a multiple
