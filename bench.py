@@ -103,6 +103,32 @@ def sampled_fir_error(x: np.ndarray, h: np.ndarray, y: np.ndarray, count: int = 
         float((err - 1e-5 * np.abs(want)).max())
 
 
+def release(rf, *objs) -> None:
+    """Frees device buffers and destroys plans, graphs and streams explicitly, after the device
+    has finished with them, so that nothing is left for finalizers at interpreter exit."""
+    rf.synchronize()
+    for o in objs:
+        for item in (o if isinstance(o, (list, tuple)) else [o]):
+            if item is None:
+                continue
+            for method in ("free", "destroy"):
+                fn = getattr(item, method, None)
+                if fn is not None:
+                    fn()
+                    break
+
+
+def fir_errors(y: np.ndarray, want: np.ndarray) -> dict:
+    """Error of every output against the float64 oracle output, with the reference's FIR
+    tolerance (rtol = atol = 1e-5, tests/test_pipeline.py:113-117)."""
+    err = np.abs(y.astype(np.complex128) - want)
+    mag = np.abs(want)
+    return {"max_abs_err": float(err.max()),
+            "max_rel_err": float((err / np.maximum(mag, 1e-30)).max()),
+            "within_rtol_atol_1e-5": bool(np.all(err <= 1e-5 + 1e-5 * mag)),
+            "checked_outputs": f"all {y.size} outputs vs the float64 oracle"}
+
+
 def time_steps(rf, step, steps: int, warmup: int, stream) -> float:
     for i in range(warmup):
         step(i)
@@ -140,6 +166,9 @@ def bench_fir(rf, args, long_filter: bool) -> dict:
     ms_step = ms / args.steps
     y = d_y[(args.warmup + args.steps - 1) % sets].download()
     max_abs, max_rel, excess = sampled_fir_error(x, h, y)
+    accuracy = {"max_abs_err": max_abs, "max_rel_err": max_rel,
+                "within_rtol_atol_1e-5": excess <= 1e-5,
+                "checked_outputs": "first taps+8 and 2000 random outputs vs float64"}
     bytes_step = 16 * n + 4 * t
 
     # The filtering kernel alone, timed the same way: a prepared filter (taps copied, their
@@ -181,6 +210,20 @@ def bench_fir(rf, args, long_filter: bool) -> dict:
                "fp32_peak_tflops": FP32_PEAK_TFLOPS,
                "frac": round(algo_flop / (kernel_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)}
 
+    want = None
+    cpu = None
+    if not args.no_cpu:
+        oracle = load_oracle()
+        sample = n  # the whole capture: about 7 s of CPU work at 2^24 x 1025 taps
+        t0 = time.perf_counter()
+        want = oracle.fir(x[:sample], h)
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(sample / dt / 1e6, 4), "unit": "Msamples/s", "cores": 1,
+               "kind": "port",
+               "sample": f"float64 oracle (oracle/oracle.c orc_fir), one thread, first {sample} "
+                         f"of the {n} samples x {t} taps, {dt:.2f} s"}
+        accuracy = fir_errors(y, want)
+
     variants = {}
     for name in [v.name for v in rf.fir_impls()]:
         if args.no_variants or name == args.impl or name == "naive" and long_filter:
@@ -188,21 +231,15 @@ def bench_fir(rf, args, long_filter: bool) -> dict:
         reps = max(2, args.steps // 5)
         vms = time_steps(rf, step_for(rf.fir_impl_index(name)), reps, 1, stream.handle) / reps
         vy = d_y[(1 + reps - 1) % sets].download()
-        v_abs, v_rel, _ = sampled_fir_error(x, h, vy)
-        variants[name] = {"ms_per_step": round(vms, 4), "msamples_per_s": round(n / vms / 1e3, 1),
+        if want is not None:
+            verr = fir_errors(vy, want)
+            v_abs, v_rel = verr["max_abs_err"], verr["max_rel_err"]
+        else:
+            v_abs, v_rel, _ = sampled_fir_error(x, h, vy)
+        variants[name] = {"kernel": rf.fir_kernel(name, t),
+                          "ms_per_step": round(vms, 4), "msamples_per_s": round(n / vms / 1e3, 1),
                           "max_abs_err": v_abs, "max_rel_err": v_rel}
-
-    cpu = None
-    if not args.no_cpu:
-        oracle = load_oracle()
-        sample = n  # the whole capture: about 7 s of CPU work at 2^24 x 1025 taps
-        t0 = time.perf_counter()
-        oracle.fir(x[:sample], h)
-        dt = time.perf_counter() - t0
-        cpu = {"value": round(sample / dt / 1e6, 4), "unit": "Msamples/s", "cores": 1,
-               "kind": "port",
-               "sample": f"float64 oracle (oracle/oracle.c orc_fir), one thread, first {sample} "
-                         f"of the {n} samples x {t} taps, {dt:.2f} s"}
+    release(rf, d_h, d_x, d_y, stream)
     workload = (f"FIR {n} complex64 samples x {t}-tap "
                 + ("Kaiser(beta=8) lowpass 0.1 fs" if long_filter else "Hamming lowpass 0.2 fs"))
     return {
@@ -211,20 +248,23 @@ def bench_fir(rf, args, long_filter: bool) -> dict:
         "unit": "Msamples/s",
         "ms_per_step": round(ms_step, 5),
         "dtype": "f32",
-        "config": {"workload": workload, "impl": args.impl, "samples": n, "taps": t,
+        "config": {"workload": workload, "impl": args.impl, "kernel": rf.fir_kernel(args.impl, t),
+                   "samples": n, "taps": t,
                    "buffer_sets": sets, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profiles(f"fir_{args.impl}") if long_filter else None},
-        "roofline_kernel": {"launches": "rf_fir_plan_run: the filtering kernel only "
-                                        "(fir_ola_kernel for fft_ola)",
+        "roofline_kernel": {"kernel": rf.fir_kernel(args.impl, t),
+                            "launches": "rf_fir_plan_run: the filtering kernel only, the kernel "
+                                        "the timed rf_fir step runs after its tap transform",
                             "ms_per_launch": round(kernel_ms, 5),
                             "msamples_per_s": round(n / kernel_ms / 1e3, 1)},
+        "step_roofline": {"kernels": "whole rf_fir call (tap transform + filtering kernel)",
+                          "achieved": round(bytes_step / (ms_step * 1e-3) / 1e9, 1),
+                          "frac": round(bytes_step / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "compute": compute,
         "graph_replay": graph,
-        "accuracy": {"max_abs_err": max_abs, "max_rel_err": max_rel,
-                     "within_rtol_atol_1e-5": excess <= 1e-5,
-                     "checked_outputs": "first taps+8 and 2000 random outputs vs float64"},
+        "accuracy": accuracy,
         "variants": variants,
         "cpu_baseline": cpu,
     }
@@ -296,6 +336,7 @@ def bench_channelize(rf, args) -> dict:
     ms_mfma = time_steps(rf, step_mfma, args.steps, args.warmup, stream.handle) / args.steps
     z_mfma = d_z[(args.warmup + args.steps - 1) % sets].download().reshape(-1, m)
     err_mfma = channelize_errors(z_mfma, x, h, m, check_frames)
+    release(rf, d_h, d_x, d_z, stream)
     bytes_step = 16 * n + 4 * t
     achieved = bytes_step / (ms_step * 1e-3) / 1e9
     cpu = None
@@ -396,6 +437,7 @@ def bench_gcc_batch(rf, args) -> dict:
 
     ms_plain = time_steps(rf, plain, max(2, args.steps // 4), 1, stream.handle) / max(2, args.steps // 4)
     plain_peaks = rf.read_peaks(pres, count)
+    release(rf, h, singles, d_r, pres, res, d_a, d_b, stream)
     same = int(sum(p.index == q.index for p, q in zip(peaks, plain_peaks)))
     lag_err = [float(abs(p.index - (n - 1) + p.frac - d)) for p, n, d in zip(peaks, na, delays)]
     within = int(sum(e <= 0.1 for e in lag_err))
@@ -484,15 +526,6 @@ def bench_delay(rf, args) -> dict:
                           args.steps, args.warmup, stream.handle) / args.steps
     exact = rf.read_peaks(pres, count)
 
-    def quality(results):
-        agree = sum(r.index == e.index for r, e in zip(results, exact))
-        fallback = sum(r.path == 1 for r in results)
-        err = [abs(r.index - (n - 1) + r.frac - dd) for r, dd in zip(results, delays)]
-        return {"lag_equal_to_exact": int(agree), "agreement": round(agree / count, 5),
-                "fallback_rate": round(fallback / count, 5),
-                "within_0.1_sample": int(sum(e <= 0.1 for e in err)),
-                "max_abs_lag_error": round(float(max(err)), 4)}
-
     labels = [int(min(max((int(np.rint(x)) + 512) >> 4, 0), 63)) for x in delays]
     proposals_right = sum(r.proposal == lab for r, lab in zip(r_mlp, labels))
     cpu = None
@@ -509,6 +542,17 @@ def bench_delay(rf, args) -> dict:
         dt = time.perf_counter() - t0
         cpu = {"value": round(12 / dt, 3), "unit": "pairs/s", "cores": 1, "kind": "port",
                "sample": "float64 oracle features + exact GCC-PHAT + peak, one thread, 12 pairs"}
+    release(rf, batch, search, mlp, pres, res, a, b, d, stream)
+
+    def quality(results):
+        agree = sum(r.index == e.index for r, e in zip(results, exact))
+        fallback = sum(r.path == 1 for r in results)
+        err = [abs(r.index - (n - 1) + r.frac - dd) for r, dd in zip(results, delays)]
+        return {"lag_equal_to_exact": int(agree), "agreement": round(agree / count, 5),
+                "fallback_rate": round(fallback / count, 5),
+                "within_0.1_sample": int(sum(e <= 0.1 for e in err)),
+                "max_abs_lag_error": round(float(max(err)), 4)}
+
     L = 1 << 15
     flop_features = 2 * 5 * L * 15 + 12 * L          # two forward FFTs + weighting
     flop_inverse = 5 * L * 15
@@ -561,6 +605,7 @@ def bench_copy(rf, args) -> dict:
 
     ms = time_steps(rf, step, args.steps, args.warmup, stream.handle)
     ms_step = ms / args.steps
+    release(rf, src, dst, stream)
     gbs = 2 * nbytes / (ms_step * 1e-3) / 1e9
     return {
         "metric": "copy GB/s (read + write)", "value": round(gbs, 1), "unit": "GB/s",

@@ -18,6 +18,7 @@ Build it with `make -C gpu-rf-pipeline_amd` (or __graft_entry__.build()).
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import math
 import os
@@ -61,6 +62,34 @@ class PeakResult(ctypes.Structure):
     _fields_ = [("index", ctypes.c_int32), ("magnitude", ctypes.c_float), ("frac", ctypes.c_double)]
 
 
+# Set by an atexit hook. Once the interpreter is shutting down, finalizers leave device objects
+# to the end of the process instead of calling into the HIP runtime, whose own teardown (and a
+# profiler's, under rocprofv3) may already be under way; the driver reclaims everything when
+# the process ends. Code that wants its objects released calls free()/destroy() itself.
+_exiting = False
+
+
+def _at_exit() -> None:
+    global _exiting
+    _exiting = True
+
+
+atexit.register(_at_exit)
+
+
+def _finalize(obj, attr: str, destroy) -> None:
+    """Body of every __del__: release obj.<attr> with destroy(), unless the process is exiting
+    or it was released already."""
+    handle = getattr(obj, attr, None)
+    if not handle or _exiting:
+        return
+    try:
+        destroy(handle)
+    except Exception:
+        pass
+    setattr(obj, attr, None)
+
+
 def _proto(name: str, restype, *argtypes) -> None:
     fn = getattr(_lib, name)
     fn.restype = restype
@@ -94,6 +123,7 @@ _proto("rf_fir_impl_count", _int)
 _proto("rf_fir_impl_name", ctypes.c_char_p, _int)
 _proto("rf_fir_impl_summary", ctypes.c_char_p, _int)
 _proto("rf_fir_parse_impl", _st, ctypes.c_char_p, ctypes.POINTER(_int))
+_proto("rf_fir_kernel_name", ctypes.c_char_p, _int, _int)
 _proto("rf_fir", _st, _int, _vp, _vp, _vp, _int, _int, _int, _vp)
 _proto("rf_fir_batched", _st, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
        ctypes.POINTER(_int), _int, _vp, _int, _vp)
@@ -252,12 +282,7 @@ class DeviceBuffer:
             self._ptr = 0
 
     def __del__(self):
-        try:
-            if self._ptr:
-                _lib.rf_free(self._ptr)
-                self._ptr = 0
-        except Exception:
-            pass
+        _finalize(self, "_ptr", _lib.rf_free)
 
     def __enter__(self):
         return self
@@ -279,6 +304,9 @@ class Stream:
         if self.handle:
             check(_lib.rf_stream_destroy(self.handle))
             self.handle = None
+
+    def __del__(self):
+        _finalize(self, "handle", _lib.rf_stream_destroy)
 
 
 def _sp(stream) -> int | None:
@@ -305,12 +333,7 @@ class Event:
         return float(ms.value)
 
     def __del__(self):
-        try:
-            if self.handle:
-                _lib.rf_event_destroy(self.handle)
-                self.handle = None
-        except Exception:
-            pass
+        _finalize(self, "handle", _lib.rf_event_destroy)
 
 
 # --- timing harness (src/bench.hpp:1-46, src/bench.cpp:65-93) ------------------------------
@@ -426,12 +449,7 @@ class FftPlan:
             self.handle = None
 
     def __del__(self):
-        try:
-            if self.handle:
-                _lib.rf_fft_plan_destroy(self.handle)
-                self.handle = None
-        except Exception:
-            pass
+        _finalize(self, "handle", _lib.rf_fft_plan_destroy)
 
 
 def spectrum_device(plan: FftPlan, src: int, dst: int, stream=None) -> None:
@@ -494,12 +512,7 @@ class XcorrFft:
             self.handle = None
 
     def __del__(self):
-        try:
-            if self.handle:
-                _lib.rf_xcorr_fft_destroy(self.handle)
-                self.handle = None
-        except Exception:
-            pass
+        _finalize(self, "handle", _lib.rf_xcorr_fft_destroy)
 
 
 # --- host-vector operations (src/ops.hpp:16-52) --------------------------------------------
@@ -549,6 +562,15 @@ def fir_impl_index(name: str) -> int:
     idx = _int(0)
     check(_lib.rf_fir_parse_impl(name.encode(), ctypes.byref(idx)))
     return idx.value
+
+
+def fir_kernel(impl: str, num_taps: int) -> str:
+    """Name of the kernel rf_fir runs for this variant and tap count (fft_ola and the
+    matrix-core variants use the register kernel outside 66..2049 taps)."""
+    name = _lib.rf_fir_kernel_name(fir_impl_index(impl), num_taps)
+    if name is None:
+        raise RfError(RF_INVALID_ARG, f"no FIR kernel for {impl} with {num_taps} taps")
+    return name.decode()
 
 
 def fir(x, taps, impl: str = "naive", block: int = DEFAULT_FIR_BLOCK,
@@ -744,12 +766,7 @@ class GccBatch:
             self.handle = None
 
     def __del__(self):
-        try:
-            if self.handle:
-                _lib.rf_gcc_batch_destroy(self.handle)
-                self.handle = None
-        except Exception:
-            pass
+        _finalize(self, "handle", _lib.rf_gcc_batch_destroy)
 
 
 def fir_tune(x: int, taps: int, y: int, n: int, num_taps: int, impls: Sequence[str],
@@ -790,12 +807,7 @@ class Graph:
             self.handle = None
 
     def __del__(self):
-        try:
-            if self.handle:
-                _lib.rf_graph_destroy(self.handle)
-                self.handle = None
-        except Exception:
-            pass
+        _finalize(self, "handle", _lib.rf_graph_destroy)
 
 
 class FirPlan:
@@ -818,12 +830,7 @@ class FirPlan:
             self.handle = None
 
     def __del__(self):
-        try:
-            if self.handle:
-                _lib.rf_fir_plan_destroy(self.handle)
-                self.handle = None
-        except Exception:
-            pass
+        _finalize(self, "handle", _lib.rf_fir_plan_destroy)
 
 
 DELAY_FEATURES = 256
@@ -901,12 +908,7 @@ class DelayMlp:
             self.handle = None
 
     def __del__(self):
-        try:
-            if self.handle:
-                _lib.rf_delay_mlp_destroy(self.handle)
-                self.handle = None
-        except Exception:
-            pass
+        _finalize(self, "handle", _lib.rf_delay_mlp_destroy)
 
 
 class DelaySearch:
@@ -939,12 +941,7 @@ class DelaySearch:
             self.handle = None
 
     def __del__(self):
-        try:
-            if self.handle:
-                _lib.rf_delay_search_destroy(self.handle)
-                self.handle = None
-        except Exception:
-            pass
+        _finalize(self, "handle", _lib.rf_delay_search_destroy)
 
 
 def read_delay_results(d_results: "DeviceBuffer", count: int) -> list[DelayResult]:

@@ -15,12 +15,13 @@
 //     a wave reads on 16 different 16-byte bank groups);
 //   3 wave w computes output columns 32w .. 32w + 31 for both 32-frame row tiles with
 //     v_mfma_f32_32x32x16_f16, three products per step (Vhi Dhi + Vhi Dlo + Vlo Dhi; Vlo Dlo,
-//     2^-22 of a product, is dropped); its D fragments (hi and lo, built in float64 on the
-//     host once per device) stay in registers;
+//     2^-22 of a product, is dropped); its D fragments (hi and lo, computed in float64 at
+//     build time, tables.hpp) stay in registers;
 //   4 the FP32 result is scaled back exactly and written as 32 consecutive floats per row.
 
 #include "fir_internal.hpp"
 #include "rf_common.hpp"
+#include "twiddle.hpp"
 
 #include <cmath>
 #include <mutex>
@@ -33,75 +34,13 @@ namespace {
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int kM = 64;                 // channels
+constexpr int kM = kDftChannels;       // channels
 constexpr int kP = 16;                 // taps per branch (T <= kP * kM)
 constexpr int kFramesPerBlock = 64;
 constexpr int kThreads = 256;
 constexpr int kRowHalves = 2 * kM + 8; // 136: one V row in LDS
-constexpr int kSteps = 2 * kM / 16;    // 8 k-steps of 16
-constexpr int kTableHalves = 4 * kSteps * 64 * 8;  // per hi or lo: [w][s][lane][q]
-
-// D fragment of wave w, step s, lane l, element q: row kk = 16 s + 8 (l / 32) + q (= 2r + d),
-// column n = 32 w + l % 32 (= 2k + e). exp(+i theta), theta = 2 pi (k r mod M) / M:
-// Re z gets Re v cos - Im v sin, Im z gets Re v sin + Im v cos.
-void build_dft_table(std::vector<_Float16>& hi, std::vector<_Float16>& lo)
-{
-    hi.assign(kTableHalves, static_cast<_Float16>(0.0f));
-    lo.assign(kTableHalves, static_cast<_Float16>(0.0f));
-    const double two_pi = 6.283185307179586476925286766559;
-    for (int w = 0; w < 4; ++w) {
-        for (int s = 0; s < kSteps; ++s) {
-            for (int l = 0; l < 64; ++l) {
-                for (int q = 0; q < 8; ++q) {
-                    const int kk = 16 * s + 8 * (l / 32) + q;
-                    const int n = 32 * w + l % 32;
-                    const int r = kk >> 1;
-                    const int d = kk & 1;
-                    const int k = n >> 1;
-                    const int e = n & 1;
-                    const double theta = two_pi * static_cast<double>((k * r) % kM) / kM;
-                    const double c = std::cos(theta);
-                    const double sn = std::sin(theta);
-                    const double v = e == 0 ? (d == 0 ? c : -sn) : (d == 0 ? sn : c);
-                    const int idx = ((w * kSteps + s) * 64 + l) * 8 + q;
-                    const _Float16 h = static_cast<_Float16>(static_cast<float>(v));
-                    hi[idx] = h;
-                    lo[idx] = static_cast<_Float16>(static_cast<float>(v - static_cast<double>(h)));
-                }
-            }
-        }
-    }
-}
-
-constexpr int kMaxDevices = 64;
-std::once_flag g_once[kMaxDevices];
-_Float16* g_table[kMaxDevices] = {};
-
-// hi table then lo table, per device, built once.
-const _Float16* dft_table()
-{
-    int device = 0;
-    RF_HIP_CHECK(hipGetDevice(&device));
-    if (device < 0 || device >= kMaxDevices) {
-        fail(RF_LIMIT, "channelize mfma: device index out of range");
-    }
-    std::call_once(g_once[device], [device] {
-        std::vector<_Float16> hi;
-        std::vector<_Float16> lo;
-        build_dft_table(hi, lo);
-        _Float16* dev = nullptr;
-        RF_HIP_CHECK(hipMalloc(&dev, 2 * kTableHalves * sizeof(_Float16)));
-        RF_HIP_CHECK(hipMemcpy(dev, hi.data(), kTableHalves * sizeof(_Float16),
-                               hipMemcpyHostToDevice));
-        RF_HIP_CHECK(hipMemcpy(dev + kTableHalves, lo.data(), kTableHalves * sizeof(_Float16),
-                               hipMemcpyHostToDevice));
-        g_table[device] = dev;
-    });
-    if (g_table[device] == nullptr) {
-        fail(RF_HIP_ERROR, "channelize mfma: table initialisation failed on an earlier call");
-    }
-    return g_table[device];
-}
+constexpr int kSteps = kDftSteps;      // 8 k-steps of 16
+constexpr int kTableHalves = kDftHalves;  // per hi or lo: [w][s][lane][q] (tables.hpp)
 
 // Power-of-two exponent e with max * 2^e in [2^13, 2^14) (0 for max == 0).
 __device__ __forceinline__ int scale_exponent(float max_abs)
@@ -281,7 +220,7 @@ extern "C" rf_status rf_channelize_mfma(const rf_cf32* x, const float* taps, rf_
         if (x == nullptr || taps == nullptr || out == nullptr) {
             rf::fail(RF_INVALID_ARG, "channelize mfma: NULL buffer");
         }
-        const _Float16* table = rf::dft_table();
+        const _Float16* table = rf::dft64_table();
         const unsigned grid =
             static_cast<unsigned>(8 * rf::ceil_div(rf::ceil_div(frames, rf::kFramesPerBlock), 8));
         rf::channelize_mfma_kernel<<<grid, rf::kThreads, 0, rf::as_stream(stream)>>>(

@@ -35,17 +35,7 @@ constexpr int k16384 = 1 << k16384Log2;
 constexpr int k16384Lanes = 1024;
 constexpr int k16384Slots = k16384 + k16384 / 16;
 
-// Layout of the factor tables, in float2 units, both in the device table and in LDS.
-constexpr int kTw4 = 0;                   // span 4: 4 rows x 16
-constexpr int kTw64 = kTw4 + 4 * 16;      // span 64: 64 rows x 17 (16 used)
-constexpr int kTwHi = kTw64 + 64 * 17;    // 128: exp(-2 pi i h / 128)
-constexpr int kTwLo = kTwHi + 128;        // 128: exp(-2 pi i l / 16384)
-constexpr int kTwLds = kTwLo + 128;       // 1408 values copied to LDS
-// Only in the device table: the fold factors of a 2^15-point transform split into two halves,
-// exp(-2 pi i n / 32768) for n = t + P j = (per lane) * (per j).
-constexpr int kFoldLane = kTwLds;         // 1024: exp(-2 pi i t / 32768)
-constexpr int kFoldRow = kFoldLane + 1024;  // 32: exp(-2 pi i j / 64), the factor of n = 512 j
-constexpr int kTw16384Size = kFoldRow + 32;
+// The factor tables are laid out as the kTw* / kFold* offsets of tables.hpp.
 
 // Copies the LDS part of the table with `lanes` lanes; the caller puts a barrier before the
 // first transform.

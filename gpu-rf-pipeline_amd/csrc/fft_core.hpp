@@ -18,11 +18,10 @@
 #pragma once
 
 #include "rf_common.hpp"
+#include "tables.hpp"
 
 namespace rf {
 
-constexpr int kTwiddleLog2 = 12;
-constexpr int kTwiddleSize = 1 << kTwiddleLog2;  // 4096
 constexpr int kMaxLdsFft = 4096;                 // largest single-workgroup transform
 
 __host__ __device__ constexpr int lds_pad(int j)

@@ -22,7 +22,6 @@
 #include "rf_common.hpp"
 #include "twiddle.hpp"
 
-#include <atomic>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -351,13 +350,6 @@ __global__ __launch_bounds__(k4096Lanes) void fir_ola_taps_kernel(
     ola_taps_spectrum(taps, num_taps, spec, buf, stages);
 }
 
-// Cache-policy operand of a buffer access on gfx950: bit 4 (sc1) gives device scope.
-constexpr int kCachePolicyDevice = 16;
-
-// Polls of the ready flag before a segment gives up waiting for the tap spectrum (about a
-// second): the wait always ends, and a spectrum that never came shows as a wrong output.
-constexpr unsigned kOlaReadyPolls = 1u << 20;
-
 // Unchecked buffer accesses for blocks known to lie inside their arrays: address = base +
 // soffset + voffset, with the workgroup-uniform parts (base, block offset) in scalar registers
 // and one per-lane offset register shared by all 16 blocks of a segment, instead of a 64-bit
@@ -387,26 +379,16 @@ __device__ __forceinline__ void buffer_store2(float2 v, __amdgpu_buffer_rsrc_t r
 // spectrum, outputs y[out0 + i - (T-1)] for T-1 <= i < 4096. Lane t holds positions t + 256 r.
 // The twiddle factors are loaded after the samples, so they arrive while the first pass runs,
 // and serve both transforms.
-// With WAIT, the tap spectrum is written by another workgroup of the same launch, which stores
-// `tag` to *ready once it is complete; the segment waits for that after issuing its sample loads,
-// so the loads' latency overlaps the wait.
-template <bool WAIT = false>
 __device__ __forceinline__ void ola_segment(const float2* __restrict__ x,
                                             const float2* __restrict__ spec,
                                             float2* __restrict__ y, int n, int halo,
                                             long long out0, float2* buf,
-                                            const float2* __restrict__ stages,
-                                            const unsigned long long* ready = nullptr,
-                                            unsigned long long tag = 0)
+                                            const float2* __restrict__ stages)
 {
     const int t = threadIdx.x;
     const long long in0 = out0 - halo;
     const unsigned lane_bytes = static_cast<unsigned>(t) * 8u;
     constexpr unsigned kBlockBytes = k4096Lanes * 8u;
-    unsigned long long seen = 0;
-    if (WAIT && t == 0) {
-        seen = __hip_atomic_load(ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
     float2 u[16];
     if (in0 >= 0 && in0 + kOlaP <= n
         && (reinterpret_cast<unsigned long long>(x + in0) & 15ULL) == 0) {
@@ -442,24 +424,6 @@ __device__ __forceinline__ void ola_segment(const float2* __restrict__ x,
             u[r] = load_sample(x, in0 + t + r * k4096Lanes, n);
         }
     }
-    if constexpr (WAIT) {
-        // One lane polls the flag (device scope: it may be written from another XCD); its first
-        // read was issued before the sample loads. No acquire fence follows: on gfx950 that is
-        // an L2 invalidate, and one in each of thousands of workgroups costs more than the
-        // launch it saves (238 against 70 us measured). The spectrum loads below are
-        // device-scope instead (cache policy sc1), coherent with the stores the producer wrote
-        // back before its flag, and this kernel reads the spectrum only after the flag. The
-        // wait sits before the transforms, not between them, so that everything from here to
-        // the stores is one block of straight-line code, as in the kernels given a ready
-        // spectrum: the same multiply-adds are fused and the outputs are the same bits.
-        if (t == 0) {
-            for (unsigned i = 0; i < kOlaReadyPolls && seen != tag; ++i) {
-                __builtin_amdgcn_s_sleep(2);
-                seen = __hip_atomic_load(ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
-        __syncthreads();
-    }
     Fft4096Twiddles tw;
     fft4096_load_twiddles(t, stages, tw);
     fft4096<false>(buf, t, u, tw);
@@ -469,8 +433,7 @@ __device__ __forceinline__ void ola_segment(const float2* __restrict__ x,
     for (int q = 0; q < 8; ++q) {
         typedef float f32x4 __attribute__((ext_vector_type(4)));
         const f32x4 hv = __builtin_bit_cast(
-            f32x4, __builtin_amdgcn_raw_buffer_load_b128(h, 2u * lane_bytes, q * 2u * kBlockBytes,
-                                                            WAIT ? kCachePolicyDevice : 0));
+            f32x4, __builtin_amdgcn_raw_buffer_load_b128(h, 2u * lane_bytes, q * 2u * kBlockBytes, 0));
         u[2 * q] = cmul(u[2 * q], make_float2(hv.x, hv.y));
         u[2 * q + 1] = cmul(u[2 * q + 1], make_float2(hv.z, hv.w));
     }
@@ -525,35 +488,6 @@ __global__ __launch_bounds__(k4096Lanes) void fir_ola_kernel(const float2* __res
     ola_segment(x, spec, y, n, halo, seg * (kOlaP - halo), buf, stages);
 }
 
-// One-shot overlap-save: the tap spectrum and the filtering in one launch. Workgroup 0
-// transforms the taps into spec, releases the stores at agent scope (written back from its
-// XCD's L2) and then stores tag to *ready; workgroups 1.. filter segments, waiting for the tag
-// while their sample loads are in flight. Workgroups are dispatched in index order, so workgroup 0 is
-// resident before any workgroup that waits on it; the wait is bounded (kOlaReadyPolls). tag is
-// unique to the call, so a flag left from an earlier call never reads as ready.
-__global__ __launch_bounds__(k4096Lanes) void fir_ola_fused_kernel(
-    const float2* __restrict__ x, const float* __restrict__ taps, float2* __restrict__ spec,
-    unsigned long long* ready, unsigned long long tag, float2* __restrict__ y, int n,
-    int num_taps, long long segments, long long per_xcd, const float2* __restrict__ stages)
-{
-    __shared__ float2 buf[k4096Slots];
-    if (blockIdx.x == 0) {
-        ola_taps_spectrum(taps, num_taps, spec, buf, stages);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            __hip_atomic_store(ready, tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        return;
-    }
-    const long long seg = xcd_major(blockIdx.x - 1, per_xcd);
-    if (seg >= segments) {
-        return;
-    }
-    const int halo = num_taps - 1;
-    ola_segment<true>(x, spec, y, n, halo, seg * (kOlaP - halo), buf, stages, ready, tag);
-}
-
 __global__ __launch_bounds__(k4096Lanes) void fir_ola_batched_kernel(
     FirBatch batch, const float2* __restrict__ spec, int num_taps,
     const float2* __restrict__ stages)
@@ -580,9 +514,12 @@ const ImplInfo kFirImpls[RF_FIR_IMPL_COUNT] = {
     {"tiled", "samples staged in an LDS tile with a halo"},
     {"tiled_const", "tiled, with the taps read through the scalar (constant) cache"},
     {"register", "4, 8 or 16 outputs per lane from a register window over a padded LDS tile"},
-    {"fft_ola", "overlap-save with a 4096-point LDS FFT (register kernel for <= 65 taps)"},
-    {"mfma_f16x3", "Toeplitz GEMM on FP16 matrix cores, hi/lo split, FP32 accumulate"},
-    {"mfma_bf16x3", "Toeplitz GEMM on BF16 matrix cores, hi/lo split, FP32 accumulate"},
+    {"fft_ola", "overlap-save with a 4096-point LDS FFT for 66..2049 taps; "
+                "the register kernel outside that range"},
+    {"mfma_f16x3", "Toeplitz GEMM on FP16 matrix cores, hi/lo split, FP32 accumulate, for "
+                   "66..2049 taps; the register kernel outside that range"},
+    {"mfma_bf16x3", "Toeplitz GEMM on BF16 matrix cores, hi/lo split, FP32 accumulate, for "
+                    "66..2049 taps; the register kernel outside that range"},
 };
 
 void check_fir_args(int impl, const void* x, const float* taps, const void* y, int n,
@@ -728,37 +665,10 @@ void launch_ola(const float2* x, const float2* spec, float2* y, int n, int num_t
     check_launch("fir_ola_kernel");
 }
 
-// Tap spectrum and filtering in one launch (fir_ola_fused_kernel), for a call that owns spec
-// and ready. The tag mixes a process-wide call counter with a fixed odd constant, so neither an
-// earlier call's flag nor ordinary data left in reused memory equals it.
-void launch_ola_fused(const float2* x, const float* taps, float2* spec,
-                      unsigned long long* ready, float2* y, int n, int num_taps,
-                      hipStream_t stream)
-{
-    static std::atomic<unsigned long long> calls{0};
-    const unsigned long long tag = 0x9E3779B97F4A7C15ULL ^ (calls.fetch_add(1) + 1);
-    const long long q = kOlaP - (num_taps - 1);
-    const long long segments = ceil_div(n, q);
-    const long long per_xcd = ceil_div(segments, 8);
-    fir_ola_fused_kernel<<<static_cast<unsigned>(1 + 8 * per_xcd), k4096Lanes, 0, stream>>>(
-        x, taps, spec, ready, tag, y, n, num_taps, segments, per_xcd, fft4096_stage_table());
-    check_launch("fir_ola_fused_kernel");
-}
-
-// True while the stream records a graph (or when the query itself fails, e.g. the null stream
-// during another stream's global capture): such calls keep launches that replay correctly.
-bool stream_capturing(hipStream_t stream)
-{
-    hipStreamCaptureStatus status = hipStreamCaptureStatusNone;
-    if (hipStreamIsCapturing(stream, &status) != hipSuccess) {
-        (void)hipGetLastError();
-        return true;
-    }
-    return status != hipStreamCaptureStatusNone;
-}
-
+// `workspace`: at least kStreamWorkspaceBytes of device memory for the tap spectrum / split
+// taps of fft_ola and the matrix-core variants, or nullptr for the stream's own block.
 void fir_launch(int impl, const float2* x, const float* taps, float2* y, int n, int num_taps,
-                int block, hipStream_t stream)
+                int block, hipStream_t stream, void* workspace = nullptr)
 {
     if (n == 0) {
         return;
@@ -793,17 +703,10 @@ void fir_launch(int impl, const float2* x, const float* taps, float2* y, int n, 
             launch_register(x, taps, y, n, num_taps, block, stream);
             return;
         }
-        // the spectrum, then (16-byte aligned) the ready flag of the one-launch form
-        StreamScratch spec((kOlaP + 2) * sizeof(float2), stream);
-        float2* h = static_cast<float2*>(spec.ptr);
-        if (stream_capturing(stream)) {
-            // a replayed graph would reuse the tag: keep the spectrum in a launch of its own
-            launch_ola_taps(taps, num_taps, h, stream);
-            launch_ola(x, h, y, n, num_taps, stream);
-        } else {
-            launch_ola_fused(x, taps, h, reinterpret_cast<unsigned long long*>(h + kOlaP), y, n,
-                             num_taps, stream);
-        }
+        float2* h = static_cast<float2*>(workspace != nullptr ? workspace
+                                                              : stream_workspace(stream));
+        launch_ola_taps(taps, num_taps, h, stream);
+        launch_ola(x, h, y, n, num_taps, stream);
         return;
     }
     case RF_FIR_MFMA_F16X3:
@@ -812,7 +715,8 @@ void fir_launch(int impl, const float2* x, const float* taps, float2* y, int n, 
             launch_register(x, taps, y, n, num_taps, block, stream);
             return;
         }
-        fir_mfma_launch(impl == RF_FIR_MFMA_BF16X3, x, taps, y, n, num_taps, stream);
+        fir_mfma_launch(impl == RF_FIR_MFMA_BF16X3, x, taps, y, n, num_taps, stream,
+                        workspace != nullptr ? workspace : stream_workspace(stream));
         return;
     }
 }
@@ -823,8 +727,10 @@ void fir_batched_launch(int impl, const FirBatch& batch, int count, int longest,
     const int block = RF_DEFAULT_FIR_BLOCK;
     if (impl == RF_FIR_MFMA_F16X3 || impl == RF_FIR_MFMA_BF16X3) {
         // One launch per channel: the matrix-core kernel's tile shape is per capture.
+        void* ws = stream_workspace(stream);
         for (int c = 0; c < count; ++c) {
-            fir_launch(impl, batch.x[c], taps, batch.y[c], batch.n[c], num_taps, block, stream);
+            fir_launch(impl, batch.x[c], taps, batch.y[c], batch.n[c], num_taps, block, stream,
+                       ws);
         }
         return;
     }
@@ -857,12 +763,12 @@ void fir_batched_launch(int impl, const FirBatch& batch, int count, int longest,
     case RF_FIR_REGISTER:
     case RF_FIR_FFT_OLA: {
         if (impl == RF_FIR_FFT_OLA && ola_applies(num_taps)) {
-            StreamScratch spec(kOlaP * sizeof(float2), stream);
-            launch_ola_taps(taps, num_taps, static_cast<float2*>(spec.ptr), stream);
+            float2* spec = static_cast<float2*>(stream_workspace(stream));
+            launch_ola_taps(taps, num_taps, spec, stream);
             const long long q = kOlaP - (num_taps - 1);
             const dim3 grid(static_cast<unsigned>(ceil_div(longest, q)), count);
             fir_ola_batched_kernel<<<grid, k4096Lanes, 0, stream>>>(
-                batch, static_cast<const float2*>(spec.ptr), num_taps, fft4096_stage_table());
+                batch, spec, num_taps, fft4096_stage_table());
             check_launch("fir_ola_batched_kernel");
             return;
         }
@@ -925,6 +831,27 @@ rf_status rf_fir_parse_impl(const char* name, int* impl)
         }
         rf::fail(RF_INVALID_ARG, std::string("unknown fir implementation '") + name + "'");
     });
+}
+
+const char* rf_fir_kernel_name(int impl, int num_taps)
+{
+    if (impl < 0 || impl >= RF_FIR_IMPL_COUNT || num_taps < 1) {
+        return nullptr;
+    }
+    switch (impl) {
+    case RF_FIR_NAIVE:
+        return "fir_naive_kernel";
+    case RF_FIR_TILED:
+    case RF_FIR_TILED_CONST:
+        return "fir_tiled_kernel";
+    case RF_FIR_FFT_OLA:
+        return rf::ola_applies(num_taps) ? "fir_ola_kernel" : "fir_register_kernel";
+    case RF_FIR_MFMA_F16X3:
+    case RF_FIR_MFMA_BF16X3:
+        return rf::mfma_applies(num_taps) ? "fir_mfma_kernel" : "fir_register_kernel";
+    default:
+        return "fir_register_kernel";
+    }
 }
 
 rf_status rf_fir(int impl, const rf_cf32* x, const float* taps, rf_cf32* y, int num_samples,

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "rf_common.hpp"
+#include "workspace.hpp"
 
 namespace rf {
 
@@ -35,8 +36,9 @@ inline void allow_large_lds(const void* kernel, std::size_t bytes)
     }
 }
 
-// Split-precision Toeplitz-GEMM FIR on the matrix cores (fir_mfma.hip).
+// Split-precision Toeplitz-GEMM FIR on the matrix cores (fir_mfma.hip). `workspace` holds the
+// split taps: kStreamWorkspaceBytes of device memory, used in stream order.
 void fir_mfma_launch(bool bf16, const float2* x, const float* taps, float2* y, int n,
-                     int num_taps, hipStream_t stream);
+                     int num_taps, hipStream_t stream, void* workspace);
 
 }  // namespace rf

@@ -283,7 +283,7 @@ std::size_t mfma_lds_bytes(int num_taps)
 
 template <bool BF16>
 void launch_mfma(const float2* x, const float* taps, float2* y, int n, int num_taps,
-                 hipStream_t stream)
+                 hipStream_t stream, void* workspace)
 {
     if (num_taps > kMfmaMaxTaps) {
         fail(RF_LIMIT, "fir mfma: at most " + std::to_string(kMfmaMaxTaps) + " taps");
@@ -295,10 +295,13 @@ void launch_mfma(const float2* x, const float* taps, float2* y, int n, int num_t
     }
     const std::size_t ws_bytes =
         16 * static_cast<std::size_t>(mfma_copy_len(num_taps)) * 2 + 16;
-    StreamScratch ws(ws_bytes, stream);
+    if (ws_bytes > kStreamWorkspaceBytes) {
+        fail(RF_LIMIT, "fir mfma: " + std::to_string(num_taps) + " taps need "
+                           + std::to_string(ws_bytes) + " bytes of workspace");
+    }
     typedef typename Split<BF16>::T H;
-    H* tabs = static_cast<H*>(ws.ptr);
-    float* descale = reinterpret_cast<float*>(static_cast<char*>(ws.ptr) + ws_bytes - 16);
+    H* tabs = static_cast<H*>(workspace);
+    float* descale = reinterpret_cast<float*>(static_cast<char*>(workspace) + ws_bytes - 16);
     fir_mfma_taps_kernel<BF16><<<1, kMfmaThreads, 0, stream>>>(taps, num_taps, tabs, descale);
     check_launch("fir_mfma_taps_kernel");
     allow_large_lds(reinterpret_cast<const void*>(fir_mfma_kernel<BF16>), bytes);
@@ -311,12 +314,12 @@ void launch_mfma(const float2* x, const float* taps, float2* y, int n, int num_t
 }  // namespace
 
 void fir_mfma_launch(bool bf16, const float2* x, const float* taps, float2* y, int n,
-                     int num_taps, hipStream_t stream)
+                     int num_taps, hipStream_t stream, void* workspace)
 {
     if (bf16) {
-        launch_mfma<true>(x, taps, y, n, num_taps, stream);
+        launch_mfma<true>(x, taps, y, n, num_taps, stream, workspace);
     } else {
-        launch_mfma<false>(x, taps, y, n, num_taps, stream);
+        launch_mfma<false>(x, taps, y, n, num_taps, stream, workspace);
     }
 }
 

@@ -3,6 +3,7 @@
 // the same jobs directly (src/device_buffer.hpp:11-76, src/bench.cpp:10-40).
 
 #include "rf_common.hpp"
+#include "workspace.hpp"
 
 #include <cstring>
 #include <mutex>
@@ -278,13 +279,26 @@ rf_status rf_stream_create(rf_stream* stream)
     return guarded([&] {
         hipStream_t s = nullptr;
         RF_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        try {
+            // the stream's workspace block now, so no later call on it allocates
+            (void)rf::stream_workspace(s);
+        } catch (...) {
+            (void)hipStreamDestroy(s);
+            throw;
+        }
         *stream = s;
     });
 }
 
 rf_status rf_stream_destroy(rf_stream stream)
 {
-    return guarded([&] { RF_HIP_CHECK(hipStreamDestroy(rf::as_stream(stream))); });
+    return guarded([&] {
+        hipStream_t s = rf::as_stream(stream);
+        // the stream's workspace block (if any) is freed once the stream's work has finished
+        RF_HIP_CHECK(hipStreamSynchronize(s));
+        rf::release_stream_workspace(s);
+        RF_HIP_CHECK(hipStreamDestroy(s));
+    });
 }
 
 rf_status rf_stream_synchronize(rf_stream stream)

@@ -7,8 +7,12 @@
  *
  * Conventions carried over from the reference's operator surface (src/device_ops.hpp:1-106,
  * src/ops.hpp:1-52):
- *   - compute entry points only enqueue work on the given stream; they never synchronise and
- *     never copy to or from the host;
+ *   - compute entry points only enqueue work on the given stream; they never synchronise,
+ *     never copy to or from the host and, once a stream has been used, never allocate: the
+ *     constant tables are compiled into the library, and the one-shot FIR forms that derive a
+ *     table from the taps (fft_ola's 4096-point tap spectrum, the MFMA variants' split taps)
+ *     keep it in a 72 KiB workspace block per stream, allocated by the first such call on that
+ *     stream (or by rf_stream_create) and freed by rf_stream_destroy;
  *   - the caller owns every input and output buffer; plan handles own their workspaces;
  *   - taps are read from the caller's pointer on every launch (no program-wide tap bank, unlike
  *     the reference's __constant__ c_taps, src/fir.cu:208-209);
@@ -84,10 +88,11 @@ rf_status rf_event_synchronize(void* event);
 rf_status rf_event_elapsed_ms(void* start, void* stop, float* ms);
 /* HIP graphs by stream capture: the work enqueued on `stream` (a created stream, not NULL)
  * between begin and end is recorded instead of run; end instantiates it (*graph is a
- * hipGraphExec_t). rf_graph_launch replays it on a stream, rf_graph_destroy frees it. rf_fir
- * with a direct variant and rf_fir_plan_run with fft_ola or a direct variant only enqueue
- * kernels and can be captured; calls that take stream-ordered workspace per call (rf_fir with
- * fft_ola or an MFMA variant) should not be. */
+ * hipGraphExec_t). rf_graph_launch replays it on a stream, rf_graph_destroy frees it. Every
+ * rf_fir / rf_fir_batched / rf_fir_plan_run call only enqueues kernels and can be captured;
+ * rf_fir and rf_fir_batched with fft_ola or an MFMA variant keep their tap spectrum / split
+ * taps in the capture stream's workspace block, so a graph holding such a call is replayed on
+ * its capture stream (or on another stream while no eager call on the capture stream runs). */
 rf_status rf_graph_begin_capture(rf_stream stream);
 rf_status rf_graph_end_capture(rf_stream stream, void** graph);
 rf_status rf_graph_launch(void* graph, rf_stream stream);

@@ -191,8 +191,11 @@ def test_direct_fir_variants_agree_bit_for_bit() -> None:
 
 
 @pytest.mark.parametrize("impl", FIR_IMPLS)
-def test_fir_batched_equals_single_channel(impl: str) -> None:
-    h = taps("lowpass_long")
+@pytest.mark.parametrize("taps_name", ["lowpass", "lowpass_long"])
+def test_fir_batched_matches_oracle(impl: str, taps_name: str) -> None:
+    # Every channel of one batched launch against the float64 oracle, and bit-equal to the
+    # single-channel call of the same variant (same kernels, same arithmetic).
+    h = taps(taps_name)
     lens = [65536, 10007, 7, 4096, 1, 3000, 70001, 12]
     xs = [noise(100 + i, n) for i, n in enumerate(lens)]
     d_x = [rf.DeviceBuffer.from_array(x) for x in xs]
@@ -201,7 +204,65 @@ def test_fir_batched_equals_single_channel(impl: str) -> None:
     idx = rf.fir_impl_index(impl)
     rf.fir_batched_device(idx, [b.ptr for b in d_x], [b.ptr for b in d_y], lens, d_h.ptr, h.size)
     for x, y in zip(xs, d_y):
-        np.testing.assert_array_equal(y.download(), rf.fir(x, h, impl))
+        got = y.download()
+        assert_fir_close(impl, got, x, h)
+        np.testing.assert_array_equal(got, rf.fir(x, h, impl))
+
+
+# Which kernel each variant runs: overlap-save and the matrix-core GEMMs cover 66..2049 taps,
+# the register kernel the rest (a single tap is then exact, and very short filters are cheaper
+# direct).
+@pytest.mark.parametrize("num_taps,ola,mfma", [
+    (1, "fir_register_kernel", "fir_register_kernel"),
+    (63, "fir_register_kernel", "fir_register_kernel"),
+    (65, "fir_register_kernel", "fir_register_kernel"),
+    (66, "fir_ola_kernel", "fir_mfma_kernel"),
+    (1025, "fir_ola_kernel", "fir_mfma_kernel"),
+    (2049, "fir_ola_kernel", "fir_mfma_kernel"),
+    (2050, "fir_register_kernel", "fir_register_kernel"),
+])
+def test_fir_variant_kernel_by_tap_count(num_taps: int, ola: str, mfma: str) -> None:
+    assert rf.fir_kernel("fft_ola", num_taps) == ola
+    assert rf.fir_kernel("mfma_f16x3", num_taps) == mfma
+    assert rf.fir_kernel("mfma_bf16x3", num_taps) == mfma
+    assert rf.fir_kernel("register", num_taps) == "fir_register_kernel"
+    assert rf.fir_kernel("naive", num_taps) == "fir_naive_kernel"
+
+
+TRANSFORM_FIR = ["fft_ola", "mfma_f16x3", "mfma_bf16x3"]
+
+
+@pytest.mark.parametrize("impl", TRANSFORM_FIR)
+@pytest.mark.parametrize("num_taps", [66, 1025, 2049])
+def test_fir_transform_variants_at_their_tap_counts(impl: str, num_taps: int) -> None:
+    # The variants whose own kernels start at 66 taps, at both ends of their range and at the
+    # long-FIR size: against the oracle, and different in the last bits from the direct
+    # register kernel (so the variant's own arithmetic ran, not the direct fallback).
+    h = kaiser_lowpass(num_taps, 0.1, 8.0)
+    x = noise(40 + num_taps, 30011)
+    got = rf.fir(x, h, impl)
+    assert_fir_close(impl, got, x, h)
+    assert not np.array_equal(got, rf.fir(x, h, "register"))
+    # zeros in, exactly zero out, and no NaN from the transforms or the split scaling
+    z = rf.fir(np.zeros(20000, np.complex64), h, impl)
+    assert not np.any(z)
+
+
+@pytest.mark.parametrize("impl", TRANSFORM_FIR)
+@pytest.mark.parametrize("num_taps,delay", [(129, 100), (1025, 777), (2049, 2048)])
+def test_fir_transform_variants_delayed_impulse(impl: str, num_taps: int, delay: int) -> None:
+    # h = unit impulse at `delay`: y[n] = x[n - delay] (zero for n < delay). The transform and
+    # GEMM forms round (FFT: ~1e-7 of the segment's energy; FP16 split: 2^-20 relative; BF16
+    # split: 2^-15), so the check is to those bounds, sample by sample; a shifted or mirrored
+    # index would miss by O(1).
+    h = np.zeros(num_taps, np.float32)
+    h[delay] = 1.0
+    x = noise(50 + delay, 20011)
+    got = rf.fir(x, h, impl).astype(np.complex128)
+    want = np.concatenate([np.zeros(delay), x[:-delay].astype(np.complex128)])
+    tol = {"fft_ola": 2e-6, "mfma_f16x3": 2.0 ** -19, "mfma_bf16x3": 2.0 ** -14}[impl]
+    err = np.abs(got - want)
+    assert np.all(err <= tol * (np.abs(want) + 1.0)), float(err.max())
 
 
 @pytest.mark.parametrize("impl", ["tiled_const", "register"])
@@ -291,11 +352,11 @@ def test_fir_calls_replayed_from_a_graph() -> None:
     stream.destroy()
 
 
-def test_fir_ola_one_launch_calls_on_two_streams() -> None:
-    # A one-shot overlap-save call makes its tap spectrum inside the filtering launch
-    # (workgroup 0 writes it and a per-call ready flag; the others wait for the flag). Calls
-    # queued on two streams with different filters, with no synchronisation between them, each
-    # equal the prepared-filter result bit for bit.
+def test_fir_ola_one_shot_calls_on_two_streams() -> None:
+    # A one-shot overlap-save call makes its tap spectrum in its stream's workspace block, then
+    # filters. Calls queued on two streams with different filters, with no synchronisation
+    # between them, each equal the prepared-filter result bit for bit (each stream has its own
+    # block, and calls on one stream run in order).
     shapes = [(70000, 1025), (4096, 66), (123457, 2049), (9000, 513), (300000, 1500), (5000, 200)]
     rng = np.random.default_rng(7)
     streams = [rf.Stream(), rf.Stream()]
