@@ -125,6 +125,7 @@ _proto("rf_fir_impl_summary", ctypes.c_char_p, _int)
 _proto("rf_fir_parse_impl", _st, ctypes.c_char_p, ctypes.POINTER(_int))
 _proto("rf_fir_kernel_name", ctypes.c_char_p, _int, _int)
 _proto("rf_fir", _st, _int, _vp, _vp, _vp, _int, _int, _int, _vp)
+_proto("rf_fir_noalloc", _st, _int, _vp, _vp, _vp, _int, _int, _int, _vp)
 _proto("rf_fir_batched", _st, _int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
        ctypes.POINTER(_int), _int, _vp, _int, _vp)
 _proto("rf_magnitude", _st, _vp, _vp, _int, _vp)

@@ -23,6 +23,7 @@
 #include "twiddle.hpp"
 
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -795,6 +796,56 @@ void fir_batched_launch(int impl, const FirBatch& batch, int count, int longest,
     }
 }
 
+// Device memory a one-shot call of `impl` with num_taps taps needs beside its inputs and
+// outputs: the tap spectrum (fft_ola) or the split taps (MFMA variants); 0 for the direct
+// kernels.
+std::size_t fir_workspace_bytes(int impl, int num_taps)
+{
+    if (impl == RF_FIR_FFT_OLA && ola_applies(num_taps)) {
+        return kOlaP * sizeof(float2);
+    }
+    if ((impl == RF_FIR_MFMA_F16X3 || impl == RF_FIR_MFMA_BF16X3) && mfma_applies(num_taps)) {
+        return fir_mfma_workspace_bytes(num_taps);
+    }
+    return 0;
+}
+
+}  // namespace
+}  // namespace rf
+
+// Scratch of rf_fir_noalloc: the device memory rf_fir takes from a stream's workspace block,
+// here a fixed array of the library's code object, so that no call allocates. Each GPU holds
+// its own copy for the life of the process; every rf_fir_noalloc call on a GPU shares it.
+constexpr std::size_t kNoAllocScratchBytes = rf::kStreamWorkspaceBytes;  // 72 KiB
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace rf {
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::once_flag g_scratch_once[kScratchDevices];
+void* g_scratch[kScratchDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device (the name
+// itself is only a host-side stand-in). The lookup allocates nothing.
+void* noalloc_scratch()
+{
+    int device = 0;
+    RF_HIP_CHECK(hipGetDevice(&device));
+    if (device < 0 || device >= kScratchDevices) {
+        fail(RF_LIMIT, "fir noalloc: device index out of range");
+    }
+    std::call_once(g_scratch_once[device], [device] {
+        void* p = nullptr;
+        RF_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[device] = p;
+    });
+    if (g_scratch[device] == nullptr) {
+        fail(RF_HIP_ERROR, "fir noalloc: scratch lookup failed on an earlier call");
+    }
+    return g_scratch[device];
+}
+
 }  // namespace
 }  // namespace rf
 
@@ -863,6 +914,27 @@ rf_status rf_fir(int impl, const rf_cf32* x, const float* taps, rf_cf32* y, int 
         rf::fir_launch(impl, reinterpret_cast<const float2*>(x), taps,
                        reinterpret_cast<float2*>(y), num_samples, num_taps, b,
                        rf::as_stream(stream));
+    });
+}
+
+rf_status rf_fir_noalloc(int impl, const rf_cf32* x, const float* taps, rf_cf32* y,
+                         int num_samples, int num_taps, int block, rf_stream stream)
+{
+    return guarded([&] {
+        const int b = block == 0 ? RF_DEFAULT_FIR_BLOCK : block;
+        rf::check_fir_args(impl, x, taps, y, num_samples, num_taps, b);
+        const std::size_t need = rf::fir_workspace_bytes(impl, num_taps);
+        if (need > kNoAllocScratchBytes) {
+            rf::fail(RF_LIMIT, "fir noalloc: " + std::to_string(num_taps) + " taps need "
+                                   + std::to_string(need) + " bytes of scratch, more than the "
+                                   + std::to_string(kNoAllocScratchBytes) + " it has");
+        }
+        if (num_samples == 0) {
+            return;
+        }
+        rf::fir_launch(impl, reinterpret_cast<const float2*>(x), taps,
+                       reinterpret_cast<float2*>(y), num_samples, num_taps, b,
+                       rf::as_stream(stream), need > 0 ? rf::noalloc_scratch() : nullptr);
     });
 }
 

@@ -40,5 +40,7 @@ inline void allow_large_lds(const void* kernel, std::size_t bytes)
 // split taps: kStreamWorkspaceBytes of device memory, used in stream order.
 void fir_mfma_launch(bool bf16, const float2* x, const float* taps, float2* y, int n,
                      int num_taps, hipStream_t stream, void* workspace);
+// Bytes of `workspace` fir_mfma_launch uses for a filter of num_taps taps.
+std::size_t fir_mfma_workspace_bytes(int num_taps);
 
 }  // namespace rf

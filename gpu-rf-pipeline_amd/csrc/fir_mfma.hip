@@ -293,8 +293,7 @@ void launch_mfma(const float2* x, const float* taps, float2* y, int n, int num_t
         fail(RF_LIMIT, "fir mfma: " + std::to_string(num_taps) + " taps need "
                            + std::to_string(bytes) + " bytes of LDS per workgroup");
     }
-    const std::size_t ws_bytes =
-        16 * static_cast<std::size_t>(mfma_copy_len(num_taps)) * 2 + 16;
+    const std::size_t ws_bytes = fir_mfma_workspace_bytes(num_taps);
     if (ws_bytes > kStreamWorkspaceBytes) {
         fail(RF_LIMIT, "fir mfma: " + std::to_string(num_taps) + " taps need "
                            + std::to_string(ws_bytes) + " bytes of workspace");
@@ -312,6 +311,12 @@ void launch_mfma(const float2* x, const float* taps, float2* y, int n, int num_t
 }
 
 }  // namespace
+
+std::size_t fir_mfma_workspace_bytes(int num_taps)
+{
+    // 16 split tap copies (hi, lo) of mfma_copy_len halves each, then 16 bytes of descale
+    return 16 * static_cast<std::size_t>(mfma_copy_len(num_taps)) * 2 + 16;
+}
 
 void fir_mfma_launch(bool bf16, const float2* x, const float* taps, float2* y, int n,
                      int num_taps, hipStream_t stream, void* workspace)

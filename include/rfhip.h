@@ -131,6 +131,16 @@ rf_status rf_fir_parse_impl(const char* name, int* impl);
 rf_status rf_fir(int impl, const rf_cf32* x, const float* taps, rf_cf32* y, int num_samples,
                  int num_taps, int block, rf_stream stream);
 
+/* rf_fir that allocates nothing, for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No counterpart in
+ * the reference. Same arguments, errors and bits as rf_fir; fft_ola's tap spectrum and the MFMA
+ * variants' split taps go to a fixed 72 KiB scratch array compiled into the library (one per
+ * GPU, kept until the process ends) instead of a stream's workspace block; RF_LIMIT if a filter
+ * needed more. Every rf_fir_noalloc call on a GPU shares that array, so calls that overlap on
+ * one GPU (different streams, or a stream and a graph replay) race: serialise them. */
+rf_status rf_fir_noalloc(int impl, const rf_cf32* x, const float* taps, rf_cf32* y,
+                         int num_samples, int num_taps, int block, rf_stream stream);
+
 /* fir_batched_device (src/fused.cu:92-109): `count` (1..RF_MAX_FIR_BATCH) channels sharing one
  * set of taps in a single launch. x, y and num_samples are HOST arrays of `count` entries
  * holding device pointers and lengths. Results equal rf_fir with the same impl per channel. */

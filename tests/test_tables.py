@@ -99,9 +99,8 @@ def test_dft64_split() -> None:
     assert np.max(np.abs(hi - want)) < 2.0 ** -11
 
 
-@pytest.mark.skipif(shutil.which(str(LLVM / "llvm-readelf")) is None, reason="no ROCm llvm tools")
-def test_tables_are_in_the_gfx950_code_object(tmp_path: Path) -> None:
-    obj = BUILD / "twiddle.o"
+def device_symbols(obj: Path, tmp_path: Path) -> str:
+    """llvm-readelf --syms of the gfx950 code object bundled into one host object file."""
     if not obj.exists():
         pytest.skip("library not built in-tree")
     fatbin = tmp_path / "fatbin.bin"
@@ -111,11 +110,28 @@ def test_tables_are_in_the_gfx950_code_object(tmp_path: Path) -> None:
     subprocess.run([str(LLVM / "clang-offload-bundler"), "--unbundle", "--type=o",
                     f"--input={fatbin}", "--targets=hipv4-amdgcn-amd-amdhsa--gfx950",
                     f"--output={dev}"], check=True)
-    syms = subprocess.run([str(LLVM / "llvm-readelf"), "--syms", str(dev)], check=True,
+    return subprocess.run([str(LLVM / "llvm-readelf"), "--syms", str(dev)], check=True,
                           capture_output=True, text=True).stdout
+
+
+TOOLS = pytest.mark.skipif(shutil.which(str(LLVM / "llvm-readelf")) is None,
+                           reason="no ROCm llvm tools")
+
+
+@TOOLS
+def test_tables_are_in_the_gfx950_code_object(tmp_path: Path) -> None:
+    syms = device_symbols(BUILD / "twiddle.o", tmp_path)
     sizes = {"rf_table_tw4096": 4096 * 8, "rf_table_stage4096": 4352 * 8,
              "rf_table_16384": 2464 * 8, "rf_table_dft64": 2 * 16384 * 2}
     for name, size in sizes.items():
         m = re.search(rf"\s(\d+) OBJECT\s+GLOBAL\s+\w+\s+\d+ {name}\n", syms)
         assert m is not None, f"{name} missing from the device code object"
         assert int(m.group(1)) == size
+
+
+@TOOLS
+def test_noalloc_scratch_is_in_the_gfx950_code_object(tmp_path: Path) -> None:
+    # rf_fir_noalloc's file-scope scratch array (72 KiB), which the runtime registers by name
+    syms = device_symbols(BUILD / "fir.o", tmp_path)
+    m = re.search(r"\s(\d+) OBJECT\s+GLOBAL\s+\w+\s+\d+ _ZL7scratch\n", syms)
+    assert m is not None and int(m.group(1)) == 72 * 1024
