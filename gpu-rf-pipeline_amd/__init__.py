@@ -118,6 +118,7 @@ _proto("rf_event_create", _st, ctypes.POINTER(_vp))
 _proto("rf_event_destroy", _st, _vp)
 _proto("rf_event_record", _st, _vp, _vp)
 _proto("rf_event_synchronize", _st, _vp)
+_proto("rf_stream_wait_event", _st, _vp, _vp)
 _proto("rf_event_elapsed_ms", _st, _vp, _vp, ctypes.POINTER(ctypes.c_float))
 _proto("rf_fir_impl_count", _int)
 _proto("rf_fir_impl_name", ctypes.c_char_p, _int)
@@ -978,14 +979,39 @@ def tdoa(a, b) -> int:
     return int(res.index) - (a.size - 1)
 
 
+def tdoa_batch(channels: Sequence[np.ndarray]) -> list[int]:
+    """Lag of every channel 1.. behind channel 0 from one batched GCC-PHAT with the peak search
+    fused (rf_gcc_batch); pairs too short for the batch (at most 2048 output lags) use tdoa."""
+    ref = _cf32(channels[0])
+    chans = [_cf32(c) for c in channels[1:]]
+    lags = [0] * len(chans)
+    batched = [i for i, c in enumerate(chans) if ref.size + c.size - 1 > 2048]
+    for i, c in enumerate(chans):
+        if i not in batched:
+            lags[i] = tdoa(ref, c)
+    if batched:
+        d_ref = DeviceBuffer.from_array(ref)
+        d_ch = [DeviceBuffer.from_array(chans[i]) for i in batched]
+        h = GccBatch([ref.size] * len(batched), [chans[i].size for i in batched])
+        h.bind([d_ref.ptr] * len(batched), [d.ptr for d in d_ch])
+        res = DeviceBuffer(16 * len(batched), np.uint8)
+        h.run(res.ptr)
+        for i, p in zip(batched, read_peaks(res, len(batched))):
+            lags[i] = int(p.index) - (ref.size - 1)
+        h.destroy()
+        for b in [d_ref, res, *d_ch]:
+            b.free()
+    return lags
+
+
 def doa(channels: Sequence[np.ndarray], spacing: float, speed: float = 299792458.0,
         rate: float = 1.0) -> dict:
-    """Direction of arrival for a uniform linear array (src/main.cpp:352-416): GCC-PHAT of each
-    channel against channel 0 with the peak searched on the GPU, then the least-squares
-    angle on the host."""
+    """Direction of arrival for a uniform linear array (src/main.cpp:352-416): one batched
+    GCC-PHAT of every channel against channel 0 with the peak searched on the GPU, then the
+    least-squares angle on the host."""
     if len(channels) < 2:
         raise RfError(RF_INVALID_ARG, "doa needs at least two channels")
-    lags = [tdoa(channels[0], ch) for ch in channels[1:]]
+    lags = tdoa_batch(channels)
     sum_it = sum(i * lag / rate for i, lag in enumerate(lags, start=1))
     sum_ii = sum(float(i * i) for i in range(1, len(channels)))
     sine = max(-1.0, min(1.0, speed * (sum_it / sum_ii) / spacing))

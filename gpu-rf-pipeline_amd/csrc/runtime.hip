@@ -332,6 +332,14 @@ rf_status rf_event_record(void* event, rf_stream stream)
     });
 }
 
+rf_status rf_stream_wait_event(rf_stream stream, void* event)
+{
+    return guarded([&] {
+        if (event == nullptr) rf::fail(RF_INVALID_ARG, "stream wait: NULL event");
+        RF_HIP_CHECK(hipStreamWaitEvent(rf::as_stream(stream), static_cast<hipEvent_t>(event), 0));
+    });
+}
+
 rf_status rf_event_synchronize(void* event)
 {
     return guarded([&] { RF_HIP_CHECK(hipEventSynchronize(static_cast<hipEvent_t>(event))); });

@@ -1,16 +1,10 @@
-// rfgpu: command-line front end of the MI355X hot path, with the reference's subcommands,
-// flags, JSON reports and exit codes (src/main.cpp:1-549): 0 success, 1 runtime error (file,
-// HIP), 2 usage error.
+// rfgpu: command-line front end of the MI355X hot path. Its interface is the reference's
+// (src/main.cpp): the subcommands, their flags, the JSON reports and the exit codes 0 (done),
+// 1 (runtime error: files, device) and 2 (usage error).
 //
-//   rfgpu spectrum  --in x.cf32 --out spec.cf32 [--nfft 1024]
-//   rfgpu fir       --in x.cf32 --taps h.f32 --out y.cf32 [--impl naive] [--block 256]
-//   rfgpu xcorr     --a a.cf32 --b b.cf32 --out r.cf32 [--impl naive]
-//   rfgpu magnitude --in x.cf32 --out mag.f32
-//   rfgpu channelize --in x.cf32 --taps h.f32 --channels M --out z.cf32
-//   rfgpu gcc_phat  --a a.cf32 --b b.cf32 --out r.cf32
-//   rfgpu doa       --in ch0.cf32,ch1.cf32,... --spacing METERS [--speed M_PER_S] [--rate HZ]
-//   rfgpu stream    --in x.cf32 --taps h.f32 --out y.cf32 [--chunk N] [--depth D] [--memory M]
-//   rfgpu info
+// Every subcommand is a row of kCommands: its flags (name, type, lower bound), a synopsis for
+// the usage text, and a handler that receives the parsed, type-checked options. Reports are
+// built with a small JSON writer.
 
 #include "bench.hpp"
 #include "check.hpp"
@@ -19,14 +13,15 @@
 #include "stream.hpp"
 
 #include <algorithm>
+#include <cerrno>
 #include <climits>
 #include <cmath>
 #include <cstdio>
-#include <exception>
+#include <cstdlib>
 #include <map>
-#include <set>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -35,19 +30,540 @@ struct UsageError : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
 
+// --- JSON reports --------------------------------------------------------------------------
+
+// A JSON object built field by field, in insertion order.
+class Json {
+public:
+    Json& add(const std::string& key, const std::string& text)
+    {
+        return raw(key, quote(text));
+    }
+    Json& add(const std::string& key, const char* text) { return add(key, std::string(text)); }
+    Json& add(const std::string& key, long long v) { return raw(key, std::to_string(v)); }
+    Json& add(const std::string& key, int v) { return add(key, static_cast<long long>(v)); }
+    Json& add(const std::string& key, std::size_t v)
+    {
+        return add(key, static_cast<long long>(v));
+    }
+    Json& add(const std::string& key, double v, int decimals)
+    {
+        char buf[64];
+        std::snprintf(buf, sizeof buf, "%.*f", decimals, std::isfinite(v) ? v : 0.0);
+        return raw(key, buf);
+    }
+    Json& add(const std::string& key, const Json& object) { return raw(key, object.str()); }
+    Json& add(const std::string& key, const std::vector<std::string>& list)
+    {
+        std::string out;
+        for (const std::string& s : list) {
+            out += (out.empty() ? "" : ", ") + quote(s);
+        }
+        return raw(key, "[" + out + "]");
+    }
+    Json& add(const std::string& key, const std::vector<int>& list)
+    {
+        std::string out;
+        for (int v : list) {
+            out += (out.empty() ? "" : ", ") + std::to_string(v);
+        }
+        return raw(key, "[" + out + "]");
+    }
+    std::string str() const { return "{" + body_ + "}"; }
+    void print() const { std::printf("%s\n", str().c_str()); }
+
+private:
+    static std::string quote(const std::string& s)
+    {
+        std::string out = "\"";
+        for (char c : s) {
+            if (c == '"' || c == '\\') {
+                out += '\\';
+            }
+            out += c;
+        }
+        return out + "\"";
+    }
+    Json& raw(const std::string& key, const std::string& value)
+    {
+        body_ += (body_.empty() ? "" : ", ") + quote(key) + ": " + value;
+        return *this;
+    }
+    std::string body_;
+};
+
+Json phase(const PhaseTime& t)
+{
+    return Json().add("mean", t.mean_ms, 6).add("std", t.std_ms, 6);
+}
+
+// --- flags ---------------------------------------------------------------------------------
+
+enum class Type { Text, Count, Real };
+
+struct Flag {
+    const char* name;
+    Type type;
+    int min;  // Count: smallest accepted value
+};
+
+// The flags of one invocation, checked against the command's table as they are read.
+class Options {
+public:
+    Options(const std::string& command, const std::vector<Flag>& table, int argc, char** argv)
+    {
+        for (int i = 2; i < argc; i += 2) {
+            const std::string name = argv[i];
+            const Flag* flag = nullptr;
+            for (const Flag& f : table) {
+                if (name == f.name) {
+                    flag = &f;
+                }
+            }
+            if (flag == nullptr) {
+                throw UsageError("unknown option '" + name + "' for " + command);
+            }
+            if (i + 1 >= argc) {
+                throw UsageError("option " + name + " needs a value");
+            }
+            const std::string value = argv[i + 1];
+            if (flag->type == Type::Count) {
+                check_count(*flag, value);
+            } else if (flag->type == Type::Real) {
+                check_real(*flag, value);
+            }
+            if (!values_.emplace(name, value).second) {
+                throw UsageError("option " + name + " given twice");
+            }
+        }
+    }
+
+    bool has(const std::string& name) const { return values_.count(name) != 0; }
+
+    const std::string& need(const std::string& name) const
+    {
+        const auto it = values_.find(name);
+        if (it == values_.end()) {
+            throw UsageError("missing required option " + name);
+        }
+        return it->second;
+    }
+    std::string text(const std::string& name, const std::string& fallback) const
+    {
+        return has(name) ? need(name) : fallback;
+    }
+    int count(const std::string& name, int fallback) const
+    {
+        return has(name) ? std::atoi(need(name).c_str()) : fallback;
+    }
+    double real(const std::string& name, double fallback) const
+    {
+        return has(name) ? std::strtod(need(name).c_str(), nullptr) : fallback;
+    }
+
+private:
+    static void check_count(const Flag& f, const std::string& v)
+    {
+        char* end = nullptr;
+        errno = 0;
+        const long n = v.empty() ? 0 : std::strtol(v.c_str(), &end, 10);
+        if (v.empty() || *end != '\0' || errno != 0 || n < f.min || n > INT_MAX) {
+            throw UsageError(std::string("option ") + f.name + " needs an integer >= "
+                             + std::to_string(f.min) + ", got '" + v + "'");
+        }
+    }
+    static void check_real(const Flag& f, const std::string& v)
+    {
+        char* end = nullptr;
+        (void)std::strtod(v.c_str(), &end);
+        if (v.empty() || *end != '\0') {
+            throw UsageError(std::string("option ") + f.name + " needs a number, got '" + v
+                             + "'");
+        }
+    }
+    std::map<std::string, std::string> values_;
+};
+
+// --- shared pieces of the handlers --------------------------------------------------------
+
+const std::vector<Flag> kBenchFlags = {{"--bench", Type::Count, 1}, {"--warmup", Type::Count, 0}};
+
+BenchConfig bench_config(const Options& o)
+{
+    if (o.has("--warmup") && !o.has("--bench")) {
+        throw UsageError("--warmup needs --bench");
+    }
+    BenchConfig c;
+    if (o.has("--bench")) {
+        c.runs = o.count("--bench", 1);
+        c.warmup = o.count("--warmup", 3);
+        if (c.runs < 10) {
+            std::fprintf(stderr, "warning: --bench %d is below 10 runs\n", c.runs);
+        }
+    }
+    return c;
+}
+
+// Sizes go to the C ABI as int.
+void fits_int(std::size_t count, const std::string& what)
+{
+    if (count > static_cast<std::size_t>(INT_MAX)) {
+        throw std::runtime_error(what + " has " + std::to_string(count)
+                                 + " elements, more than the supported "
+                                 + std::to_string(INT_MAX));
+    }
+}
+
+std::vector<float> load_taps(const Options& o)
+{
+    std::vector<float> taps = read_f32(o.need("--taps"));
+    if (taps.empty()) {
+        throw std::runtime_error(o.need("--taps") + ": no taps");
+    }
+    fits_int(taps.size(), "tap array");
+    return taps;
+}
+
+std::string gpu_name()
+{
+    char name[256] = {};
+    RF_CHECK(rf_device_name(name, sizeof name));
+    return name;
+}
+
 template <class Fn>
-std::string names_of(const std::vector<Impl<Fn>>& impls, const char* sep)
+std::vector<std::string> impl_names(const std::vector<Impl<Fn>>& impls)
+{
+    std::vector<std::string> names;
+    for (const auto& i : impls) {
+        names.push_back(i.name);
+    }
+    return names;
+}
+
+std::string joined(const std::vector<std::string>& names)
 {
     std::string out;
-    for (const auto& i : impls) {
-        out += (out.empty() ? "" : sep);
-        out += i.name;
+    for (const std::string& n : names) {
+        out += (out.empty() ? "" : ", ") + n;
     }
     return out;
 }
 
 template <class Fn>
-void list_impls(const char* op, const std::vector<Impl<Fn>>& impls)
+const Impl<Fn>& choose(const std::vector<Impl<Fn>>& impls, const Options& o, const char* op)
+{
+    const std::string want = o.text("--impl", impls.front().name);
+    const auto it = std::find_if(impls.begin(), impls.end(),
+                                 [&](const Impl<Fn>& i) { return want == i.name; });
+    if (it == impls.end()) {
+        throw UsageError("unknown --impl '" + want + "' for " + op + " (available: "
+                         + joined(impl_names(impls)) + ")");
+    }
+    return *it;
+}
+
+// Writes the result and reports: a one-line summary, or the bench JSON with --bench.
+template <class T>
+int finish(const Options& o, const char* op, const char* impl, std::size_t num_in,
+           const std::vector<T>& out, const Bench& bench)
+{
+    const std::string& path = o.need("--out");
+    if constexpr (std::is_same_v<T, cf32>) {
+        write_cf32(path, out);
+    } else {
+        write_f32(path, out);
+    }
+    if (!o.has("--bench")) {
+        std::printf("%s[%s]: %zu samples in, %zu out -> %s\n", op, impl, num_in, out.size(),
+                    path.c_str());
+        return 0;
+    }
+    const BenchResult& r = bench.result();
+    Json()
+        .add("op", op)
+        .add("impl", impl)
+        .add("gpu", gpu_name())
+        .add("num_in", num_in)
+        .add("num_out", out.size())
+        .add("runs", bench.config().runs)
+        .add("warmup", bench.config().warmup)
+        .add("h2d_ms", phase(r.h2d))
+        .add("kernel_ms", phase(r.kernel))
+        .add("d2h_ms", phase(r.d2h))
+        .print();
+    return 0;
+}
+
+// --- subcommands ----------------------------------------------------------------------------
+
+int cmd_spectrum(const Options& o)
+{
+    o.need("--out");
+    const int nfft = o.count("--nfft", 1024);
+    Bench bench(bench_config(o));
+    const std::vector<cf32> x = read_cf32(o.need("--in"));
+    fits_int(x.size() + static_cast<std::size_t>(nfft), "padded spectrum input");
+    return finish(o, "spectrum", "fft", x.size(), spectrum_fft(x, nfft, bench), bench);
+}
+
+int cmd_magnitude(const Options& o)
+{
+    o.need("--out");
+    Bench bench(bench_config(o));
+    const std::vector<cf32> x = read_cf32(o.need("--in"));
+    fits_int(x.size(), "magnitude input");
+    const std::vector<float> mag = x.empty() ? std::vector<float>{} : magnitude_hip(x, bench);
+    return finish(o, "magnitude", "elementwise", x.size(), mag, bench);
+}
+
+int cmd_fir(const Options& o)
+{
+    o.need("--out");
+    const Impl<FirFn>& impl = choose(fir_impls(), o, "fir");
+    const int block = o.count("--block", kDefaultFirBlock);
+    if (block % 32 != 0 || block > 1024) {
+        throw UsageError("--block needs a multiple of 32 up to 1024, got "
+                         + std::to_string(block));
+    }
+    Bench bench(bench_config(o));
+    const std::vector<cf32> x = read_cf32(o.need("--in"));
+    const std::vector<float> taps = load_taps(o);
+    fits_int(x.size(), "fir input");
+    const std::vector<cf32> y = x.empty() ? std::vector<cf32>{} : impl.fn(x, taps, block, bench);
+    return finish(o, "fir", impl.name, x.size(), y, bench);
+}
+
+// xcorr and gcc_phat read two captures; both need a sample in each.
+std::pair<std::vector<cf32>, std::vector<cf32>> read_pair(const Options& o, const char* op)
+{
+    std::pair<std::vector<cf32>, std::vector<cf32>> ab{read_cf32(o.need("--a")),
+                                                       read_cf32(o.need("--b"))};
+    if (ab.first.empty() || ab.second.empty()) {
+        throw std::runtime_error(std::string(op) + " needs at least one sample in each input");
+    }
+    fits_int(ab.first.size() + ab.second.size(), std::string(op) + " output");
+    return ab;
+}
+
+int cmd_xcorr(const Options& o)
+{
+    o.need("--out");
+    const Impl<XcorrFn>& impl = choose(xcorr_impls(), o, "xcorr");
+    Bench bench(bench_config(o));
+    const auto [a, b] = read_pair(o, "xcorr");
+    return finish(o, "xcorr", impl.name, a.size() + b.size(), impl.fn(a, b, bench), bench);
+}
+
+int cmd_gcc_phat(const Options& o)
+{
+    o.need("--out");
+    Bench bench(bench_config(o));
+    const auto [a, b] = read_pair(o, "gcc_phat");
+    return finish(o, "gcc_phat", "fft", a.size() + b.size(), gcc_phat_hip(a, b, bench), bench);
+}
+
+int cmd_channelize(const Options& o)
+{
+    o.need("--out");
+    const int channels = std::atoi(o.need("--channels").c_str());
+    Bench bench(bench_config(o));
+    const std::vector<cf32> x = read_cf32(o.need("--in"));
+    const std::vector<float> taps = load_taps(o);
+    fits_int(x.size() + static_cast<std::size_t>(channels), "padded channelizer input");
+    return finish(o, "channelize", "polyphase", x.size(),
+                  channelize_hip(x, taps, channels, bench), bench);
+}
+
+std::vector<std::string> split_list(const std::string& list)
+{
+    std::vector<std::string> items;
+    std::size_t from = 0;
+    for (;;) {
+        const std::size_t comma = list.find(',', from);
+        items.push_back(list.substr(from, comma == std::string::npos ? std::string::npos
+                                                                    : comma - from));
+        if (comma == std::string::npos) {
+            return items;
+        }
+        from = comma + 1;
+    }
+}
+
+// Direction of arrival for a uniform linear array (interface: src/main.cpp:352-416): the lag of
+// every channel against channel 0 from one batched GCC-PHAT with the peak search on the device,
+// then the least-squares slope of lag time against element index, sin(angle) = c slope / d.
+int cmd_doa(const Options& o)
+{
+    const std::vector<std::string> paths = split_list(o.need("--in"));
+    if (paths.size() < 2) {
+        throw UsageError("--in needs at least two comma-separated channel files");
+    }
+    o.need("--spacing");
+    const double spacing = o.real("--spacing", 0.0);
+    const double speed = o.real("--speed", 299792458.0);
+    const double rate = o.has("--rate") ? o.real("--rate", 0.0) : sample_rate(paths[0]);
+    if (!(spacing > 0.0) || !(speed > 0.0) || !(rate > 0.0)) {
+        throw UsageError("--spacing, --speed, and the sample rate (--rate, or the first "
+                         "channel's metadata) must be positive");
+    }
+    std::vector<std::vector<cf32>> chans;
+    for (const std::string& p : paths) {
+        chans.push_back(read_cf32(p));
+        if (chans.back().empty()) {
+            throw std::runtime_error("doa: every channel needs at least one sample");
+        }
+        fits_int(chans.front().size() + chans.back().size(), "doa pair output");
+    }
+    const std::vector<int> lags = tdoa_batch_hip(chans);
+    double num = 0.0;
+    double den = 0.0;
+    for (std::size_t i = 1; i < chans.size(); ++i) {
+        num += static_cast<double>(i) * (lags[i - 1] / rate);
+        den += static_cast<double>(i) * static_cast<double>(i);
+    }
+    const double s = std::max(-1.0, std::min(1.0, speed * (num / den) / spacing));
+    const double degrees = std::asin(s) * (180.0 / 3.14159265358979323846);
+    Json()
+        .add("op", "doa")
+        .add("channels", chans.size())
+        .add("sample_rate", rate, 1)
+        .add("spacing_m", spacing, 6)
+        .add("speed_m_s", speed, 1)
+        .add("tdoa_samples", lags)
+        .add("angle_deg", degrees, 9)
+        .print();
+    return 0;
+}
+
+int cmd_stream(const Options& o)
+{
+    o.need("--out");
+    StreamConfig config;
+    const std::string impl = o.text("--impl", "tiled");
+    if (!parse_fir_impl(impl.c_str(), config.impl)) {
+        throw UsageError("unknown --impl '" + impl + "' for stream (available: "
+                         + joined(impl_names(fir_impls())) + ")");
+    }
+    const std::string memory = o.text("--memory", "copy");
+    if (!parse_stream_memory(memory, config.memory)) {
+        throw UsageError("unknown --memory '" + memory
+                         + "' (available: copy, registered, zero-copy)");
+    }
+    config.chunk = o.count("--chunk", config.chunk);
+    config.depth = o.count("--depth", config.depth);
+    const BenchConfig bench = bench_config(o);
+    const std::vector<cf32> x = read_cf32(o.need("--in"));
+    const std::vector<float> taps = load_taps(o);
+    fits_int(x.size(), "stream input");
+    fits_int(static_cast<std::size_t>(config.chunk) + taps.size(), "chunk with history");
+    const double rate = o.has("--rate") ? o.real("--rate", 0.0) : sample_rate(o.need("--in"));
+
+    std::vector<cf32> y;
+    StreamTiming timing;
+    double mean = 0.0;
+    double m2 = 0.0;
+    for (int i = -bench.warmup; i < bench.runs; ++i) {
+        y = fir_stream(x, taps, config, timing);
+        if (i >= 0) {
+            const double d = timing.total_ms - mean;
+            mean += d / (i + 1);
+            m2 += d * (timing.total_ms - mean);
+        }
+    }
+    write_cf32(o.need("--out"), y);
+    if (!o.has("--bench")) {
+        std::printf("stream[%s]: %zu samples in %d chunks of %d, depth %d -> %s\n", impl.c_str(),
+                    x.size(), timing.chunks, config.chunk, config.depth, o.need("--out").c_str());
+        return 0;
+    }
+    const double sd = bench.runs > 1 ? std::sqrt(m2 / (bench.runs - 1)) : 0.0;
+    const double msps = static_cast<double>(x.size()) / (mean * 1e3);
+    Json()
+        .add("op", "stream")
+        .add("impl", impl)
+        .add("memory", memory)
+        .add("gpu", gpu_name())
+        .add("num_in", x.size())
+        .add("chunk", config.chunk)
+        .add("depth", config.depth)
+        .add("chunks", timing.chunks)
+        .add("runs", bench.runs)
+        .add("warmup", bench.warmup)
+        .add("total_ms", Json().add("mean", mean, 6).add("std", sd, 6))
+        .add("msps", msps, 3)
+        .add("sample_rate", rate, 1)
+        .add("realtime_factor", rate > 0.0 ? msps * 1e6 / rate : 0.0, 3)
+        .print();
+    return 0;
+}
+
+int cmd_info(const Options&)
+{
+    std::size_t memory = 0;
+    RF_CHECK(rf_device_memory(&memory));
+    Json()
+        .add("gpu", gpu_name())
+        .add("memory_mb", memory / (1024 * 1024))
+        .add("abi", rf_abi_version())
+        .add("impls", Json()
+                          .add("spectrum", std::vector<std::string>{"fft"})
+                          .add("magnitude", std::vector<std::string>{"elementwise"})
+                          .add("fir", impl_names(fir_impls()))
+                          .add("xcorr", impl_names(xcorr_impls())))
+        .print();
+    return 0;
+}
+
+struct Command {
+    const char* name;
+    const char* synopsis;
+    std::vector<Flag> flags;
+    bool benchable;
+    int (*run)(const Options&);
+};
+
+const std::vector<Command>& commands()
+{
+    static const std::vector<Command> kCommands = {
+        {"spectrum", "--in X.cf32 --out OUT.cf32 [--nfft N]",
+         {{"--in", Type::Text, 0}, {"--out", Type::Text, 0}, {"--nfft", Type::Count, 1}},
+         true, cmd_spectrum},
+        {"fir", "--in X.cf32 --taps H.f32 --out OUT.cf32 [--impl NAME] [--block B]",
+         {{"--in", Type::Text, 0}, {"--taps", Type::Text, 0}, {"--out", Type::Text, 0},
+          {"--impl", Type::Text, 0}, {"--block", Type::Count, 32}},
+         true, cmd_fir},
+        {"xcorr", "--a A.cf32 --b B.cf32 --out OUT.cf32 [--impl NAME]",
+         {{"--a", Type::Text, 0}, {"--b", Type::Text, 0}, {"--out", Type::Text, 0},
+          {"--impl", Type::Text, 0}},
+         true, cmd_xcorr},
+        {"magnitude", "--in X.cf32 --out OUT.f32",
+         {{"--in", Type::Text, 0}, {"--out", Type::Text, 0}}, true, cmd_magnitude},
+        {"channelize", "--in X.cf32 --taps H.f32 --channels M --out OUT.cf32",
+         {{"--in", Type::Text, 0}, {"--taps", Type::Text, 0}, {"--channels", Type::Count, 1},
+          {"--out", Type::Text, 0}},
+         true, cmd_channelize},
+        {"gcc_phat", "--a A.cf32 --b B.cf32 --out OUT.cf32",
+         {{"--a", Type::Text, 0}, {"--b", Type::Text, 0}, {"--out", Type::Text, 0}}, true,
+         cmd_gcc_phat},
+        {"doa", "--in CH0,CH1,... --spacing M [--speed M/S] [--rate HZ]",
+         {{"--in", Type::Text, 0}, {"--spacing", Type::Real, 0}, {"--speed", Type::Real, 0},
+          {"--rate", Type::Real, 0}},
+         false, cmd_doa},
+        {"stream",
+         "--in X.cf32 --taps H.f32 --out OUT.cf32 [--impl NAME] [--chunk N] [--depth D]\n"
+         "                  [--memory copy|registered|zero-copy] [--rate HZ]",
+         {{"--in", Type::Text, 0}, {"--taps", Type::Text, 0}, {"--out", Type::Text, 0},
+          {"--impl", Type::Text, 0}, {"--chunk", Type::Count, 1}, {"--depth", Type::Count, 1},
+          {"--memory", Type::Text, 0}, {"--rate", Type::Real, 0}},
+         true, cmd_stream},
+        {"info", "", {}, false, cmd_info},
+    };
+    return kCommands;
+}
+
+template <class Fn>
+void print_impls(const char* op, const std::vector<Impl<Fn>>& impls)
 {
     for (std::size_t i = 0; i < impls.size(); ++i) {
         std::fprintf(stderr, "  %-6s --impl %-12s %s%s\n", op, impls[i].name, impls[i].summary,
@@ -57,388 +573,39 @@ void list_impls(const char* op, const std::vector<Impl<Fn>>& impls)
 
 void usage()
 {
+    std::fprintf(stderr, "usage:\n");
+    for (const Command& c : commands()) {
+        std::fprintf(stderr, "  rfgpu %-11s%s\n", c.name, c.synopsis);
+    }
     std::fprintf(stderr,
-        "usage:\n"
-        "  rfgpu spectrum  --in X.cf32 --out OUT.cf32 [--nfft N]\n"
-        "  rfgpu fir       --in X.cf32 --taps H.f32 --out OUT.cf32 [--impl NAME] [--block B]\n"
-        "  rfgpu xcorr     --a A.cf32 --b B.cf32 --out OUT.cf32 [--impl NAME]\n"
-        "  rfgpu magnitude --in X.cf32 --out OUT.f32\n"
-        "  rfgpu channelize --in X.cf32 --taps H.f32 --channels M --out OUT.cf32\n"
-        "  rfgpu gcc_phat  --a A.cf32 --b B.cf32 --out OUT.cf32\n"
-        "  rfgpu doa       --in CH0,CH1,... --spacing M [--speed M/S] [--rate HZ]\n"
-        "  rfgpu stream    --in X.cf32 --taps H.f32 --out OUT.cf32 [--impl NAME] [--chunk N]\n"
-        "                  [--depth D] [--memory copy|registered|zero-copy] [--rate HZ]\n"
-        "  rfgpu info\n"
-        "\n"
-        "Inputs may be SigMF recordings (NAME.sigmf-meta / .sigmf-data): cf32_le, ci16_le, ci8.\n"
-        "benchmark options (spectrum, magnitude, fir, xcorr, channelize, gcc_phat):\n"
-        "  --bench N      time N runs with device events and print JSON\n"
-        "  --warmup W     untimed runs first (default 3, needs --bench)\n"
-        "fir options:\n"
-        "  --block B      threads per workgroup, a multiple of 32 up to 1024 (default 256)\n"
-        "\n"
-        "implementations:\n");
-    list_impls("fir", fir_impls());
-    list_impls("xcorr", xcorr_impls());
+                 "\nInputs may be SigMF recordings (NAME.sigmf-meta / .sigmf-data): cf32_le, "
+                 "ci16_le, ci8.\n"
+                 "--bench N       time N runs with device events and print JSON "
+                 "(all but doa and info)\n"
+                 "--warmup W      untimed runs first (default 3, needs --bench)\n"
+                 "--block B       fir: threads per workgroup, a multiple of 32 up to 1024 "
+                 "(default 256)\n\nimplementations:\n");
+    print_impls("fir", fir_impls());
+    print_impls("xcorr", xcorr_impls());
 }
 
-class Args {
-public:
-    Args(int argc, char** argv, const std::set<std::string>& allowed)
-    {
-        for (int i = 2; i < argc; i += 2) {
-            const std::string flag = argv[i];
-            if (allowed.count(flag) == 0) {
-                throw UsageError("unknown option '" + flag + "' for " + argv[1]);
-            }
-            if (i + 1 >= argc) {
-                throw UsageError("option " + flag + " needs a value");
-            }
-            if (!values_.emplace(flag, argv[i + 1]).second) {
-                throw UsageError("option " + flag + " given twice");
-            }
-        }
-    }
-    bool has(const std::string& f) const { return values_.count(f) != 0; }
-    const std::string& required(const std::string& f) const
-    {
-        const auto it = values_.find(f);
-        if (it == values_.end()) {
-            throw UsageError("missing required option " + f);
-        }
-        return it->second;
-    }
-    std::string optional(const std::string& f, const std::string& fallback) const
-    {
-        const auto it = values_.find(f);
-        return it == values_.end() ? fallback : it->second;
-    }
-    int integer(const std::string& f, int fallback, int min) const
-    {
-        const auto it = values_.find(f);
-        if (it == values_.end()) {
-            return fallback;
-        }
-        std::size_t used = 0;
-        int v = 0;
-        try {
-            v = std::stoi(it->second, &used);
-        } catch (const std::exception&) {
-            used = 0;
-        }
-        if (used == 0 || used != it->second.size() || v < min) {
-            throw UsageError("option " + f + " needs an integer >= " + std::to_string(min)
-                             + ", got '" + it->second + "'");
-        }
-        return v;
-    }
-
-private:
-    std::map<std::string, std::string> values_;
-};
-
-template <class Fn>
-const Impl<Fn>& pick(const std::vector<Impl<Fn>>& impls, const Args& args, const char* op)
+int dispatch(int argc, char** argv)
 {
-    const std::string name = args.optional("--impl", impls.front().name);
-    for (const auto& i : impls) {
-        if (name == i.name) {
-            return i;
-        }
-    }
-    throw UsageError("unknown --impl '" + name + "' for " + op + " (available: "
-                     + names_of(impls, ", ") + ")");
-}
-
-BenchConfig bench_config(const Args& args)
-{
-    BenchConfig c;
-    if (args.has("--bench")) {
-        c.runs = args.integer("--bench", 1, 1);
-        c.warmup = args.integer("--warmup", 3, 0);
-        if (c.runs < 10) {
-            std::fprintf(stderr, "warning: --bench %d is below 10 runs\n", c.runs);
-        }
-    } else if (args.has("--warmup")) {
-        throw UsageError("--warmup needs --bench");
-    }
-    return c;
-}
-
-void require_int_range(std::size_t count, const std::string& what)
-{
-    if (count > static_cast<std::size_t>(INT_MAX)) {
-        throw std::runtime_error(what + " has " + std::to_string(count)
-                                 + " elements, more than the supported " + std::to_string(INT_MAX));
-    }
-}
-
-std::string gpu_name()
-{
-    char buf[256];
-    RF_CHECK(rf_device_name(buf, sizeof buf));
-    return buf;
-}
-
-void report(const Args& args, const char* op, const char* impl, std::size_t num_in,
-            std::size_t num_out, const std::string& path, const Bench& bench)
-{
-    if (!args.has("--bench")) {
-        std::printf("%s[%s]: %zu samples in, %zu out -> %s\n", op, impl, num_in, num_out,
-                    path.c_str());
-        return;
-    }
-    const BenchResult& r = bench.result();
-    std::printf("{\"op\": \"%s\", \"impl\": \"%s\", \"gpu\": \"%s\", \"num_in\": %zu, "
-                "\"num_out\": %zu, \"runs\": %d, \"warmup\": %d, "
-                "\"h2d_ms\": {\"mean\": %.6f, \"std\": %.6f}, "
-                "\"kernel_ms\": {\"mean\": %.6f, \"std\": %.6f}, "
-                "\"d2h_ms\": {\"mean\": %.6f, \"std\": %.6f}}\n",
-                op, impl, gpu_name().c_str(), num_in, num_out, bench.config().runs,
-                bench.config().warmup, r.h2d.mean_ms, r.h2d.std_ms, r.kernel.mean_ms,
-                r.kernel.std_ms, r.d2h.mean_ms, r.d2h.std_ms);
-}
-
-template <class T>
-void finish(const Args& args, const char* op, const char* impl, std::size_t num_in,
-            const std::vector<T>& out, const Bench& bench)
-{
-    const std::string& path = args.required("--out");
-    if constexpr (sizeof(T) == sizeof(cf32)) {
-        write_cf32(path, out);
-    } else {
-        write_f32(path, out);
-    }
-    report(args, op, impl, num_in, out.size(), path, bench);
-}
-
-int run_spectrum(int argc, char** argv)
-{
-    const Args args(argc, argv, {"--in", "--out", "--nfft", "--bench", "--warmup"});
-    args.required("--out");
-    const int nfft = args.integer("--nfft", 1024, 1);
-    Bench bench(bench_config(args));
-    const std::vector<cf32> x = read_cf32(args.required("--in"));
-    require_int_range(x.size() + static_cast<std::size_t>(nfft), "padded spectrum input");
-    finish(args, "spectrum", "fft", x.size(), spectrum_fft(x, nfft, bench), bench);
-    return 0;
-}
-
-int run_magnitude(int argc, char** argv)
-{
-    const Args args(argc, argv, {"--in", "--out", "--bench", "--warmup"});
-    args.required("--out");
-    Bench bench(bench_config(args));
-    const std::vector<cf32> x = read_cf32(args.required("--in"));
-    require_int_range(x.size(), "magnitude input");
-    const std::vector<float> mag = x.empty() ? std::vector<float>() : magnitude_hip(x, bench);
-    finish(args, "magnitude", "elementwise", x.size(), mag, bench);
-    return 0;
-}
-
-int run_fir(int argc, char** argv)
-{
-    const Args args(argc, argv,
-                    {"--in", "--taps", "--out", "--impl", "--block", "--bench", "--warmup"});
-    args.required("--out");
-    const Impl<FirFn>& impl = pick(fir_impls(), args, "fir");
-    const int block = args.integer("--block", kDefaultFirBlock, 32);
-    if (block % 32 != 0 || block > 1024) {
-        throw UsageError("--block needs a multiple of 32 up to 1024, got " + std::to_string(block));
-    }
-    Bench bench(bench_config(args));
-    const std::vector<cf32> x = read_cf32(args.required("--in"));
-    const std::vector<float> taps = read_f32(args.required("--taps"));
-    if (taps.empty()) {
-        throw std::runtime_error(args.required("--taps") + ": no taps");
-    }
-    require_int_range(x.size(), "fir input");
-    require_int_range(taps.size(), "tap array");
-    const std::vector<cf32> y = x.empty() ? std::vector<cf32>() : impl.fn(x, taps, block, bench);
-    finish(args, "fir", impl.name, x.size(), y, bench);
-    return 0;
-}
-
-int run_xcorr(int argc, char** argv)
-{
-    const Args args(argc, argv, {"--a", "--b", "--out", "--impl", "--bench", "--warmup"});
-    args.required("--out");
-    const Impl<XcorrFn>& impl = pick(xcorr_impls(), args, "xcorr");
-    Bench bench(bench_config(args));
-    const std::vector<cf32> a = read_cf32(args.required("--a"));
-    const std::vector<cf32> b = read_cf32(args.required("--b"));
-    if (a.empty() || b.empty()) {
-        throw std::runtime_error("xcorr needs at least one sample in each input");
-    }
-    require_int_range(a.size() + b.size(), "xcorr output");
-    finish(args, "xcorr", impl.name, a.size() + b.size(), impl.fn(a, b, bench), bench);
-    return 0;
-}
-
-int run_channelize(int argc, char** argv)
-{
-    const Args args(argc, argv, {"--in", "--taps", "--channels", "--out", "--bench", "--warmup"});
-    args.required("--out");
-    const int channels = args.integer("--channels", 0, 1);
-    if (!args.has("--channels")) {
-        throw UsageError("missing required option --channels");
-    }
-    Bench bench(bench_config(args));
-    const std::vector<cf32> x = read_cf32(args.required("--in"));
-    const std::vector<float> taps = read_f32(args.required("--taps"));
-    if (taps.empty()) {
-        throw std::runtime_error(args.required("--taps") + ": no taps");
-    }
-    require_int_range(x.size() + static_cast<std::size_t>(channels), "padded channelizer input");
-    require_int_range(taps.size(), "tap array");
-    finish(args, "channelize", "polyphase", x.size(), channelize_hip(x, taps, channels, bench),
-           bench);
-    return 0;
-}
-
-int run_gcc_phat(int argc, char** argv)
-{
-    const Args args(argc, argv, {"--a", "--b", "--out", "--bench", "--warmup"});
-    args.required("--out");
-    Bench bench(bench_config(args));
-    const std::vector<cf32> a = read_cf32(args.required("--a"));
-    const std::vector<cf32> b = read_cf32(args.required("--b"));
-    if (a.empty() || b.empty()) {
-        throw std::runtime_error("gcc_phat needs at least one sample in each input");
-    }
-    require_int_range(a.size() + b.size(), "gcc_phat output");
-    finish(args, "gcc_phat", "fft", a.size() + b.size(), gcc_phat_hip(a, b, bench), bench);
-    return 0;
-}
-
-// Direction of arrival for a uniform linear array: GCC-PHAT of every channel against channel
-// 0 with the peak found on the device, then the least-squares angle (src/main.cpp:352-416).
-int run_doa(int argc, char** argv)
-{
-    const Args args(argc, argv, {"--in", "--spacing", "--speed", "--rate"});
-    std::vector<std::string> paths;
-    const std::string list = args.required("--in");
-    for (std::size_t start = 0; start <= list.size();) {
-        const std::size_t comma = list.find(',', start);
-        const std::size_t end = comma == std::string::npos ? list.size() : comma;
-        paths.push_back(list.substr(start, end - start));
-        start = end + 1;
-    }
-    if (paths.size() < 2) {
-        throw UsageError("--in needs at least two comma-separated channel files");
-    }
-    const double spacing = std::stod(args.required("--spacing"));
-    const double speed = std::stod(args.optional("--speed", "299792458"));
-    double rate = sample_rate(paths[0]);
-    if (args.has("--rate")) {
-        rate = std::stod(args.required("--rate"));
-    }
-    if (!(spacing > 0.0) || !(speed > 0.0) || !(rate > 0.0)) {
-        throw UsageError("--spacing, --speed, and the sample rate (--rate, or the first "
-                         "channel's metadata) must be positive");
-    }
-    const std::vector<cf32> ref = read_cf32(paths[0]);
-    std::vector<int> lags;
-    double sum_it = 0.0;
-    double sum_ii = 0.0;
-    for (std::size_t i = 1; i < paths.size(); ++i) {
-        const std::vector<cf32> ch = read_cf32(paths[i]);
-        if (ref.empty() || ch.empty()) {
-            throw std::runtime_error("doa: every channel needs at least one sample");
-        }
-        const int lag = tdoa_hip(ref, ch);
-        lags.push_back(lag);
-        sum_it += static_cast<double>(i) * lag / rate;
-        sum_ii += static_cast<double>(i * i);
-    }
-    const double sine = std::clamp(speed * (sum_it / sum_ii) / spacing, -1.0, 1.0);
-    const double angle = std::asin(sine) * 180.0 / 3.14159265358979323846;
-    std::string text;
-    for (int lag : lags) {
-        text += (text.empty() ? "" : ", ") + std::to_string(lag);
-    }
-    std::printf("{\"op\": \"doa\", \"channels\": %zu, \"sample_rate\": %.1f, \"spacing_m\": %.6f, "
-                "\"speed_m_s\": %.1f, \"tdoa_samples\": [%s], \"angle_deg\": %.9f}\n",
-                paths.size(), rate, spacing, speed, text.c_str(), angle);
-    return 0;
-}
-
-int run_stream(int argc, char** argv)
-{
-    const Args args(argc, argv, {"--in", "--taps", "--out", "--impl", "--chunk", "--depth",
-                                 "--memory", "--rate", "--bench", "--warmup"});
-    args.required("--out");
-    StreamConfig config;
-    const std::string impl = args.optional("--impl", "tiled");
-    if (!parse_fir_impl(impl.c_str(), config.impl)) {
-        throw UsageError("unknown --impl '" + impl + "' for stream (available: "
-                         + names_of(fir_impls(), ", ") + ")");
-    }
-    const std::string memory = args.optional("--memory", "copy");
-    if (!parse_stream_memory(memory, config.memory)) {
-        throw UsageError("unknown --memory '" + memory + "' (available: copy, registered, zero-copy)");
-    }
-    config.chunk = args.integer("--chunk", config.chunk, 1);
-    config.depth = args.integer("--depth", config.depth, 1);
-    const BenchConfig bench = bench_config(args);
-    const std::vector<cf32> x = read_cf32(args.required("--in"));
-    const std::vector<float> taps = read_f32(args.required("--taps"));
-    if (taps.empty()) {
-        throw std::runtime_error(args.required("--taps") + ": no taps");
-    }
-    require_int_range(x.size(), "stream input");
-    require_int_range(static_cast<std::size_t>(config.chunk) + taps.size(), "chunk with history");
-    double rate = sample_rate(args.required("--in"));
-    if (args.has("--rate")) {
-        rate = std::stod(args.required("--rate"));
-    }
-    std::vector<double> times;
-    std::vector<cf32> y;
-    StreamTiming timing;
-    for (int i = 0; i < bench.warmup + bench.runs; ++i) {
-        y = fir_stream(x, taps, config, timing);
-        if (i >= bench.warmup) {
-            times.push_back(timing.total_ms);
-        }
-    }
-    write_cf32(args.required("--out"), y);
-    if (!args.has("--bench")) {
-        std::printf("stream[%s]: %zu samples in %d chunks of %d, depth %d -> %s\n", impl.c_str(),
-                    x.size(), timing.chunks, config.chunk, config.depth,
-                    args.required("--out").c_str());
+    const std::string name = argc > 1 ? argv[1] : "";
+    if (name == "help" || name == "--help" || name == "-h") {
+        usage();
         return 0;
     }
-    double mean = 0.0;
-    for (double t : times) {
-        mean += t / static_cast<double>(times.size());
+    for (const Command& c : commands()) {
+        if (name == c.name) {
+            std::vector<Flag> flags = c.flags;
+            if (c.benchable) {
+                flags.insert(flags.end(), kBenchFlags.begin(), kBenchFlags.end());
+            }
+            return c.run(Options(name, flags, argc, argv));
+        }
     }
-    double var = 0.0;
-    for (double t : times) {
-        var += (t - mean) * (t - mean);
-    }
-    const double sd = times.size() > 1 ? std::sqrt(var / static_cast<double>(times.size() - 1)) : 0.0;
-    const double msps = static_cast<double>(x.size()) / (mean * 1e3);
-    std::printf("{\"op\": \"stream\", \"impl\": \"%s\", \"memory\": \"%s\", \"gpu\": \"%s\", "
-                "\"num_in\": %zu, \"chunk\": %d, \"depth\": %d, \"chunks\": %d, \"runs\": %d, "
-                "\"warmup\": %d, \"total_ms\": {\"mean\": %.6f, \"std\": %.6f}, \"msps\": %.3f, "
-                "\"sample_rate\": %.1f, \"realtime_factor\": %.3f}\n",
-                impl.c_str(), memory.c_str(), gpu_name().c_str(), x.size(), config.chunk,
-                config.depth, timing.chunks, bench.runs, bench.warmup, mean, sd, msps, rate,
-                rate > 0 ? msps * 1e6 / rate : 0.0);
-    return 0;
-}
-
-int run_info(int argc, char** argv)
-{
-    const Args args(argc, argv, {});
-    size_t memory = 0;
-    RF_CHECK(rf_device_memory(&memory));
-    std::printf("{\"gpu\": \"%s\", \"memory_mb\": %zu, \"abi\": %d, "
-                "\"impls\": {\"spectrum\": [\"fft\"], \"magnitude\": [\"elementwise\"], "
-                "\"fir\": [\"%s\"], \"xcorr\": [\"%s\"]}}\n",
-                gpu_name().c_str(), memory / (1024 * 1024), rf_abi_version(),
-                names_of(fir_impls(), "\", \"").c_str(), names_of(xcorr_impls(), "\", \"").c_str());
-    return 0;
+    throw UsageError(name.empty() ? "missing subcommand" : "unknown subcommand '" + name + "'");
 }
 
 }  // namespace
@@ -446,21 +613,7 @@ int run_info(int argc, char** argv)
 int main(int argc, char** argv)
 {
     try {
-        const std::string op = argc > 1 ? argv[1] : "";
-        if (op == "spectrum") return run_spectrum(argc, argv);
-        if (op == "magnitude") return run_magnitude(argc, argv);
-        if (op == "fir") return run_fir(argc, argv);
-        if (op == "xcorr") return run_xcorr(argc, argv);
-        if (op == "stream") return run_stream(argc, argv);
-        if (op == "channelize") return run_channelize(argc, argv);
-        if (op == "gcc_phat") return run_gcc_phat(argc, argv);
-        if (op == "doa") return run_doa(argc, argv);
-        if (op == "info") return run_info(argc, argv);
-        if (op == "--help" || op == "-h" || op == "help") {
-            usage();
-            return 0;
-        }
-        throw UsageError(op.empty() ? "missing subcommand" : "unknown subcommand '" + op + "'");
+        return dispatch(argc, argv);
     } catch (const UsageError& e) {
         std::fprintf(stderr, "rfgpu: %s\n\n", e.what());
         usage();

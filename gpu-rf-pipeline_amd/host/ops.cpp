@@ -4,6 +4,7 @@
 
 #include "device_ops.hpp"
 
+#include <memory>
 #include <stdexcept>
 
 namespace {
@@ -123,6 +124,52 @@ int tdoa_hip(const std::vector<cf32>& a, const std::vector<cf32>& b)
     rf_peak_result peak{};
     d_peak.download(&peak, 1);
     return peak.index - static_cast<int>(a.size() - 1);
+}
+
+std::vector<int> tdoa_batch_hip(const std::vector<std::vector<cf32>>& channels)
+{
+    const std::vector<cf32>& ref = channels.front();
+    std::vector<int> lags(channels.size() - 1, 0);
+    std::vector<std::size_t> batched;  // channel indices that go through the batch
+    for (std::size_t i = 1; i < channels.size(); ++i) {
+        if (ref.size() + channels[i].size() - 1 > 2048) {
+            batched.push_back(i);
+        } else {
+            lags[i - 1] = tdoa_hip(ref, channels[i]);
+        }
+    }
+    if (batched.empty()) {
+        return lags;
+    }
+    const int count = static_cast<int>(batched.size());
+    DeviceBuffer<cf32> d_ref(ref.size());
+    d_ref.upload(ref.data(), ref.size());
+    std::vector<std::unique_ptr<DeviceBuffer<cf32>>> d_ch;
+    std::vector<int> na(batched.size(), static_cast<int>(ref.size()));
+    std::vector<int> nb;
+    std::vector<const rf_cf32*> pa(batched.size(), as_rf(d_ref.get()));
+    std::vector<const rf_cf32*> pb;
+    for (std::size_t i : batched) {
+        d_ch.push_back(std::make_unique<DeviceBuffer<cf32>>(channels[i].size()));
+        d_ch.back()->upload(channels[i].data(), channels[i].size());
+        nb.push_back(static_cast<int>(channels[i].size()));
+        pb.push_back(as_rf(d_ch.back()->get()));
+    }
+    struct BatchDeleter {
+        void operator()(rf_gcc_batch* h) const { (void)rf_gcc_batch_destroy(h); }
+    };
+    rf_gcc_batch* raw = nullptr;
+    RF_CHECK(rf_gcc_batch_create(count, na.data(), nb.data(), &raw));
+    const std::unique_ptr<rf_gcc_batch, BatchDeleter> batch(raw);
+    RF_CHECK(rf_gcc_batch_bind(batch.get(), pa.data(), pb.data(), nullptr));
+    DeviceBuffer<rf_peak_result> d_peaks(batched.size());
+    RF_CHECK(rf_gcc_batch_run(batch.get(), d_peaks.get(), nullptr));
+    std::vector<rf_peak_result> peaks(batched.size());
+    d_peaks.download(peaks.data(), peaks.size());
+    for (std::size_t k = 0; k < batched.size(); ++k) {
+        lags[batched[k] - 1] = peaks[k].index - static_cast<int>(ref.size() - 1);
+    }
+    return lags;
 }
 
 std::vector<cf32> spectrum_fft(const std::vector<cf32>& x, int nfft, Bench& bench)

@@ -45,6 +45,11 @@ std::vector<cf32> gcc_phat_hip(const std::vector<cf32>& a, const std::vector<cf3
 // GCC-PHAT with the peak searched on the device: the lag of the first maximum of |r|.
 int tdoa_hip(const std::vector<cf32>& a, const std::vector<cf32>& b);
 
+// The lag of every channel 1.. against channel 0 (tdoa_hip of each pair), from one batched
+// GCC-PHAT over all pairs (rf_gcc_batch); pairs too short for the batch (fewer than 2049
+// output lags) run tdoa_hip.
+std::vector<int> tdoa_batch_hip(const std::vector<std::vector<cf32>>& channels);
+
 // Source compatibility with callers written against the reference's names.
 inline std::vector<cf32> spectrum_cufft(const std::vector<cf32>& x, int nfft, Bench& bench)
 {

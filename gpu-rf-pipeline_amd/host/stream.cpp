@@ -1,184 +1,307 @@
-// Streaming FIR over a ring of pinned buffers and HIP streams (stream.hpp).
+// Streaming FIR (stream.hpp) as a three-stage pipeline on HIP streams.
+//
+// Stages: uploads run on one stream, filtering on a second, downloads on a third, so both copy
+// directions and the compute units work on different chunks at the same time. Chunk c uses slot
+// c % depth (device window and output buffers, pinned staging, three events); the compute
+// stream waits for the slot's upload event, the download stream for its compute event, and the
+// host reuses a slot only once its previous download has completed (retire), which also orders
+// the next upload after the filter that read the slot.
+//
+// History: each output needs the T - 1 samples before it. They are uploaded once, with their
+// own chunk, and then kept on the device: after filtering a chunk, the compute stream copies the
+// last min(T - 1, samples so far) samples of its window into `history`, and before filtering
+// the next chunk it copies them in front of that chunk's samples. Chunk c's window is
+// [history | chunk]; the filter over the window gives the chunk's outputs at offset `hist`, so
+// direct variants produce exactly the bits of one call over the whole capture.
+//
+// The filter is prepared once (rf_fir_plan: device taps, fft_ola's spectrum), before the clock.
 
 #include "stream.hpp"
+
+#include "check.hpp"
 
 #include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
+#include <string>
+#include <vector>
 
 namespace {
 
-// Page-locked host buffer; mapped into the device's address space on request.
-class Pinned {
+// --- owning handles over the C ABI ----------------------------------------------------------
+
+struct StreamHandle {
+    rf_stream s = nullptr;
+    StreamHandle() { RF_CHECK(rf_stream_create(&s)); }
+    ~StreamHandle() { (void)rf_stream_destroy(s); }
+    StreamHandle(const StreamHandle&) = delete;
+    StreamHandle& operator=(const StreamHandle&) = delete;
+};
+
+struct EventHandle {
+    void* e = nullptr;
+    EventHandle() { RF_CHECK(rf_event_create(&e)); }
+    ~EventHandle() { (void)rf_event_destroy(e); }
+    EventHandle(const EventHandle&) = delete;
+    EventHandle& operator=(const EventHandle&) = delete;
+};
+
+// Device memory, or page-locked host memory (optionally mapped for the device).
+class Memory {
 public:
-    Pinned(std::size_t bytes, bool mapped)
+    enum class Kind { Device, Pinned, Mapped };
+    Memory(Kind kind, std::size_t samples) : kind_(kind)
     {
-        if (bytes == 0) {
+        const std::size_t bytes = std::max<std::size_t>(samples, 1) * sizeof(cf32);
+        void* p = nullptr;
+        switch (kind_) {
+        case Kind::Device:
+            RF_CHECK(rf_malloc(&p, bytes));
+            host_ = dev_ = static_cast<cf32*>(p);
+            break;
+        case Kind::Pinned:
+            RF_CHECK(rf_host_alloc(&p, bytes));
+            host_ = dev_ = static_cast<cf32*>(p);
+            break;
+        case Kind::Mapped: {
+            void* d = nullptr;
+            RF_CHECK(rf_host_alloc_mapped(&p, &d, bytes));
+            host_ = static_cast<cf32*>(p);
+            dev_ = static_cast<cf32*>(d);
+            break;
+        }
+        }
+    }
+    ~Memory()
+    {
+        if (kind_ == Kind::Device) {
+            (void)rf_free(dev_);
+        } else {
+            (void)rf_host_free(host_);
+        }
+    }
+    Memory(const Memory&) = delete;
+    Memory& operator=(const Memory&) = delete;
+
+    cf32* host() const { return host_; }    // CPU address (pinned kinds)
+    cf32* device() const { return dev_; }   // address for kernels and copies
+
+private:
+    Kind kind_;
+    cf32* host_ = nullptr;
+    cf32* dev_ = nullptr;
+};
+
+using Kind = Memory::Kind;
+
+struct PlanDeleter {
+    void operator()(rf_fir_plan* p) const { (void)rf_fir_plan_destroy(p); }
+};
+using PlanHandle = std::unique_ptr<rf_fir_plan, PlanDeleter>;
+
+// One ring slot and the chunk it holds.
+struct Slot {
+    std::unique_ptr<Memory> window;    // device [history | chunk] (copy, registered)
+    std::unique_ptr<Memory> out;       // filter output over the window: device, or mapped
+    std::unique_ptr<Memory> stage_in;  // pinned staging (copy)
+    std::unique_ptr<Memory> stage_out;
+    EventHandle uploaded, filtered, downloaded;
+    bool busy = false;
+    std::size_t start = 0;  // first capture index of the chunk in flight
+    std::size_t count = 0;  // its samples
+    std::size_t hist = 0;   // history samples in front of it in the window
+};
+
+class Pipeline {
+public:
+    Pipeline(const std::vector<cf32>& x, const std::vector<float>& taps,
+             const StreamConfig& config, std::vector<cf32>& y)
+        : x_(x), y_(y), config_(config), halo_(taps.size() - 1)
+    {
+        const std::size_t n = x.size();
+        const std::size_t chunk = static_cast<std::size_t>(config.chunk);
+        const std::size_t window = chunk + halo_;
+        taps_ = std::make_unique<Memory>(Kind::Device, taps.size());
+        RF_CHECK(rf_memcpy_h2d(taps_->device(), taps.data(), taps.size() * sizeof(float),
+                               nullptr));
+        RF_CHECK(rf_stream_synchronize(nullptr));
+        rf_fir_plan* plan = nullptr;
+        RF_CHECK(rf_fir_plan_create(static_cast<int>(config.impl),
+                                    reinterpret_cast<const float*>(taps_->device()),
+                                    static_cast<int>(taps.size()), 0, compute_.s, &plan));
+        plan_.reset(plan);
+        const bool zero_copy = config.memory == StreamMemory::ZeroCopy;
+        if (zero_copy) {
+            capture_ = std::make_unique<Memory>(Kind::Mapped, n);
+        } else {
+            history_ = std::make_unique<Memory>(Kind::Device, halo_);
+            if (config.memory == StreamMemory::Registered) {
+                capture_ = std::make_unique<Memory>(Kind::Pinned, n);
+                result_ = std::make_unique<Memory>(Kind::Pinned, n);
+            }
+        }
+        if (capture_ && n > 0) {
+            std::memcpy(capture_->host(), x.data(), n * sizeof(cf32));
+        }
+        for (int k = 0; k < config.depth; ++k) {
+            slots_.push_back(std::make_unique<Slot>());
+            Slot& s = *slots_.back();
+            s.out = std::make_unique<Memory>(zero_copy ? Kind::Mapped : Kind::Device, window);
+            if (!zero_copy) {
+                s.window = std::make_unique<Memory>(Kind::Device, window);
+            }
+            if (config.memory == StreamMemory::Copy) {
+                s.stage_in = std::make_unique<Memory>(Kind::Pinned, chunk);
+                s.stage_out = std::make_unique<Memory>(Kind::Pinned, chunk);
+            }
+        }
+    }
+
+    // Nothing may be freed while a stage still uses it.
+    ~Pipeline() { (void)rf_device_synchronize(); }
+
+    Pipeline(const Pipeline&) = delete;
+    Pipeline& operator=(const Pipeline&) = delete;
+
+    // Runs every chunk through the pipeline and waits for the last one; returns the chunk count.
+    int run()
+    {
+        const std::size_t n = x_.size();
+        const std::size_t chunk = static_cast<std::size_t>(config_.chunk);
+        int chunks = 0;
+        for (std::size_t start = 0; start < n; start += chunk, ++chunks) {
+            Slot& s = *slots_[static_cast<std::size_t>(chunks) % slots_.size()];
+            retire(s);
+            s.start = start;
+            s.count = std::min(chunk, n - start);
+            s.hist = std::min(halo_, start);
+            enqueue(s);
+        }
+        for (auto& s : slots_) {
+            retire(*s);
+        }
+        return chunks;
+    }
+
+    // Registered mode: the result array is read after the clock stops.
+    void collect()
+    {
+        if (result_ && !y_.empty()) {
+            std::memcpy(y_.data(), result_->host(), y_.size() * sizeof(cf32));
+        }
+    }
+
+private:
+    void enqueue(Slot& s)
+    {
+        const std::size_t bytes = s.count * sizeof(cf32);
+        const int win = static_cast<int>(s.hist + s.count);
+        if (config_.memory == StreamMemory::ZeroCopy) {
+            // the filter reads the mapped capture in place; the window starts hist samples early
+            RF_CHECK(rf_fir_plan_run(plan_.get(), as_rf(capture_->device() + (s.start - s.hist)),
+                                     as_rf(s.out->device()), win, compute_.s));
+            RF_CHECK(rf_event_record(s.filtered.e, compute_.s));
+            RF_CHECK(rf_stream_wait_event(download_.s, s.filtered.e));
+            RF_CHECK(rf_event_record(s.downloaded.e, download_.s));
+            s.busy = true;
             return;
         }
-        if (mapped) {
-            RF_CHECK(rf_host_alloc_mapped(&host_, &device_, bytes));
+        const cf32* src = nullptr;
+        if (config_.memory == StreamMemory::Copy) {
+            std::memcpy(s.stage_in->host(), x_.data() + s.start, bytes);
+            src = s.stage_in->host();
         } else {
-            RF_CHECK(rf_host_alloc(&host_, bytes));
+            src = capture_->host() + s.start;
         }
+        cf32* window = s.window->device();
+        RF_CHECK(rf_memcpy_h2d(window + s.hist, src, bytes, upload_.s));
+        RF_CHECK(rf_event_record(s.uploaded.e, upload_.s));
+
+        RF_CHECK(rf_stream_wait_event(compute_.s, s.uploaded.e));
+        if (s.hist > 0) {
+            RF_CHECK(rf_memcpy_d2d(window, history_->device(), s.hist * sizeof(cf32),
+                                   compute_.s));
+        }
+        RF_CHECK(rf_fir_plan_run(plan_.get(), as_rf(window), as_rf(s.out->device()), win, compute_.s));
+        const std::size_t keep = std::min(halo_, s.start + s.count);
+        if (keep > 0) {
+            RF_CHECK(rf_memcpy_d2d(history_->device(), window + (s.hist + s.count - keep),
+                                   keep * sizeof(cf32), compute_.s));
+        }
+        RF_CHECK(rf_event_record(s.filtered.e, compute_.s));
+
+        RF_CHECK(rf_stream_wait_event(download_.s, s.filtered.e));
+        cf32* dst = config_.memory == StreamMemory::Copy ? s.stage_out->host()
+                                                          : result_->host() + s.start;
+        RF_CHECK(rf_memcpy_d2h(dst, s.out->device() + s.hist, bytes, download_.s));
+        RF_CHECK(rf_event_record(s.downloaded.e, download_.s));
+        s.busy = true;
     }
-    ~Pinned() { rf_host_free(host_); }
-    Pinned(const Pinned&) = delete;
-    Pinned& operator=(const Pinned&) = delete;
-    cf32* host() const { return static_cast<cf32*>(host_); }
-    cf32* device() const { return static_cast<cf32*>(device_); }
 
-private:
-    void* host_ = nullptr;
-    void* device_ = nullptr;
-};
-
-class Stream {
-public:
-    Stream() { RF_CHECK(rf_stream_create(&s_)); }
-    ~Stream() { rf_stream_destroy(s_); }
-    Stream(const Stream&) = delete;
-    Stream& operator=(const Stream&) = delete;
-    rf_stream get() const { return s_; }
-
-private:
-    rf_stream s_ = nullptr;
-};
-
-class Event {
-public:
-    Event() { RF_CHECK(rf_event_create(&e_)); }
-    ~Event() { rf_event_destroy(e_); }
-    Event(const Event&) = delete;
-    Event& operator=(const Event&) = delete;
-    void* get() const { return e_; }
-
-private:
-    void* e_ = nullptr;
-};
-
-struct Slot {
-    Slot(std::size_t len, StreamMemory memory)
-        : in(memory == StreamMemory::Registered ? 0 : len * sizeof(cf32),
-             memory == StreamMemory::ZeroCopy),
-          out(memory == StreamMemory::Registered ? 0 : len * sizeof(cf32),
-              memory == StreamMemory::ZeroCopy),
-          dev_in(memory == StreamMemory::ZeroCopy ? 0 : len),
-          dev_out(memory == StreamMemory::ZeroCopy ? 0 : len)
+    // Waits for the slot's chunk and moves its outputs where they belong.
+    void retire(Slot& s)
     {
+        if (!s.busy) {
+            return;
+        }
+        RF_CHECK(rf_event_synchronize(s.downloaded.e));
+        const std::size_t bytes = s.count * sizeof(cf32);
+        if (config_.memory == StreamMemory::Copy) {
+            std::memcpy(y_.data() + s.start, s.stage_out->host(), bytes);
+        } else if (config_.memory == StreamMemory::ZeroCopy) {
+            std::memcpy(y_.data() + s.start, s.out->host() + s.hist, bytes);
+        }
+        s.busy = false;
     }
-    Pinned in;
-    Pinned out;
-    DeviceBuffer<cf32> dev_in;
-    DeviceBuffer<cf32> dev_out;
-    Stream stream;
-    Event done;
-    std::size_t start = 0;   // first output of the chunk in flight
-    std::size_t count = 0;   // its outputs
-    std::size_t offset = 0;  // where they sit in the kernel's output
-    bool busy = false;
+
+    const std::vector<cf32>& x_;
+    std::vector<cf32>& y_;
+    StreamConfig config_;
+    std::size_t halo_;
+    StreamHandle upload_, compute_, download_;
+    std::unique_ptr<Memory> taps_;
+    std::unique_ptr<Memory> history_;
+    std::unique_ptr<Memory> capture_;  // registered: pinned copy; zero-copy: mapped copy
+    std::unique_ptr<Memory> result_;   // registered: pinned result
+    PlanHandle plan_;
+    std::vector<std::unique_ptr<Slot>> slots_;
 };
 
 }  // namespace
 
 bool parse_stream_memory(const std::string& name, StreamMemory& memory)
 {
-    if (name == "copy") {
-        memory = StreamMemory::Copy;
-    } else if (name == "registered") {
-        memory = StreamMemory::Registered;
-    } else if (name == "zero-copy") {
-        memory = StreamMemory::ZeroCopy;
-    } else {
-        return false;
+    static const std::pair<const char*, StreamMemory> kModes[] = {
+        {"copy", StreamMemory::Copy},
+        {"registered", StreamMemory::Registered},
+        {"zero-copy", StreamMemory::ZeroCopy},
+    };
+    for (const auto& m : kModes) {
+        if (name == m.first) {
+            memory = m.second;
+            return true;
+        }
     }
-    return true;
+    return false;
 }
 
 std::vector<cf32> fir_stream(const std::vector<cf32>& x, const std::vector<float>& taps,
                              const StreamConfig& config, StreamTiming& timing)
 {
+    if (taps.empty()) {
+        throw std::runtime_error("stream: no taps");
+    }
     if (config.chunk < 1 || config.depth < 1) {
         throw std::runtime_error("stream: chunk and depth must be at least 1");
     }
-    if (taps.empty()) {
-        throw std::runtime_error("stream: need at least one tap");
-    }
-    const std::size_t n = x.size();
-    const std::size_t halo = taps.size() - 1;
-    const std::size_t chunk = static_cast<std::size_t>(config.chunk);
-    const int num_taps = static_cast<int>(taps.size());
-    const bool whole = config.memory == StreamMemory::Registered;
-    std::vector<cf32> y(n);
-
-    // Setup, outside the clock.
-    std::vector<std::unique_ptr<Slot>> slots;
-    for (int s = 0; s < config.depth; ++s) {
-        slots.push_back(std::make_unique<Slot>(halo + chunk, config.memory));
-    }
-    Pinned x_pinned(whole ? n * sizeof(cf32) : 0, false);
-    Pinned y_pinned(whole ? n * sizeof(cf32) : 0, false);
-    if (whole && n > 0) {
-        std::memcpy(x_pinned.host(), x.data(), n * sizeof(cf32));
-    }
-    DeviceBuffer<float> d_taps(taps.size());
-    d_taps.upload(taps.data(), taps.size());
-    fir_load_const_taps(d_taps.get(), num_taps, nullptr);
-    // The filter is the same for every chunk: prepared once (for fft_ola its spectrum too), so
-    // each chunk launches only the filtering kernel.
-    const FirPlan plan(config.impl, d_taps.get(), num_taps);
-    RF_CHECK(rf_device_synchronize());
-
-    const auto retire = [&](Slot& slot) {
-        if (!slot.busy) {
-            return;
-        }
-        RF_CHECK(rf_event_synchronize(slot.done.get()));
-        if (!whole) {
-            std::memcpy(y.data() + slot.start, slot.out.host() + slot.offset,
-                        slot.count * sizeof(cf32));
-        }
-        slot.busy = false;
-    };
-
-    const auto begin = std::chrono::steady_clock::now();
-    int chunks = 0;
-    for (std::size_t start = 0; start < n; start += chunk, ++chunks) {
-        Slot& slot = *slots[static_cast<std::size_t>(chunks % config.depth)];
-        retire(slot);
-        const std::size_t end = std::min(n, start + chunk);
-        const std::size_t in_start = start > halo ? start - halo : 0;
-        const std::size_t in_len = end - in_start;
-        if (!whole) {
-            std::memcpy(slot.in.host(), x.data() + in_start, in_len * sizeof(cf32));
-        }
-        slot.start = start;
-        slot.count = end - start;
-        slot.offset = start - in_start;
-        const rf_stream s = slot.stream.get();
-        if (config.memory == StreamMemory::ZeroCopy) {
-            plan.run(slot.in.device(), slot.out.device(), static_cast<int>(in_len), s);
-        } else {
-            const cf32* source = whole ? x_pinned.host() + in_start : slot.in.host();
-            RF_CHECK(rf_memcpy_h2d(slot.dev_in.get(), source, in_len * sizeof(cf32), s));
-            plan.run(slot.dev_in.get(), slot.dev_out.get(), static_cast<int>(in_len), s);
-            cf32* target = whole ? y_pinned.host() + start : slot.out.host() + slot.offset;
-            RF_CHECK(rf_memcpy_d2h(target, slot.dev_out.get() + slot.offset,
-                                   slot.count * sizeof(cf32), s));
-        }
-        RF_CHECK(rf_event_record(slot.done.get(), s));
-        slot.busy = true;
-    }
-    for (int i = 0; i < config.depth; ++i) {
-        retire(*slots[static_cast<std::size_t>((chunks + i) % config.depth)]);
-    }
-    const auto finish = std::chrono::steady_clock::now();
-    if (whole && n > 0) {
-        std::memcpy(y.data(), y_pinned.host(), n * sizeof(cf32));
-    }
-    timing.total_ms = std::chrono::duration<double, std::milli>(finish - begin).count();
-    timing.chunks = chunks;
+    std::vector<cf32> y(x.size());
+    Pipeline pipeline(x, taps, config, y);
+    const auto t0 = std::chrono::steady_clock::now();
+    timing.chunks = pipeline.run();
+    const auto t1 = std::chrono::steady_clock::now();
+    timing.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    pipeline.collect();
     return y;
 }

@@ -1,19 +1,20 @@
 #pragma once
 
-// Streaming FIR (reference: src/stream.hpp:1-57, src/stream.cu:135-222): the capture is cut
-// into chunks that go round a ring of `depth` slots, each with pinned host buffers, device
-// buffers and its own HIP stream, so the copy engines and the CUs work on different chunks
-// at once. Every chunk after the first is uploaded with num_taps - 1 samples of history and
-// only its own outputs come back, so for the direct FIR variants the result is bit-identical
-// to one rf_fir over the whole capture. fft_ola places its transform blocks relative to each
-// chunk, so its streamed result equals the one-shot result to rounding, not bit for bit.
+// Streaming FIR (interface of the reference's src/stream.hpp:1-57): a capture filtered chunk by
+// chunk the way a receiver delivers it, with copies and filtering of different chunks
+// overlapped. The output equals one FIR over the whole capture: bit for bit for the direct
+// variants; to rounding for fft_ola and the MFMA variants, whose transform / GEMM blocks sit
+// relative to each chunk's window.
 //
+// `depth` chunks may be in flight at once (a ring of that many slots); `chunk` samples each.
 // Memory modes:
-//   copy        host memcpy into and out of the pinned ring, async copies to the device;
-//   registered  the capture and the result live in pinned memory for the whole run (the
-//               receiver writes into pinned memory), chunks are copied straight from and to it;
-//               the pinned arrays are allocated and filled before the clock starts;
-//   zero-copy   mapped pinned buffers, the kernel reads and writes host memory directly.
+//   copy        the host copies each chunk into pinned staging memory, which is copied to the
+//               device, and the outputs back out of pinned staging;
+//   registered  capture and result live in pinned memory for the whole run (a receiver that
+//               writes into pinned memory), chunks are copied straight from and to them; they
+//               are filled before and read after the clock;
+//   zero-copy   capture and outputs in mapped pinned memory; the filter reads and writes host
+//               memory directly, with no copies to or from the device.
 
 #include "device_ops.hpp"
 #include "types.hpp"

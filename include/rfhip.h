@@ -86,6 +86,9 @@ rf_status rf_event_destroy(void* event);
 rf_status rf_event_record(void* event, rf_stream stream);
 rf_status rf_event_synchronize(void* event);
 rf_status rf_event_elapsed_ms(void* start, void* stop, float* ms);
+/* Work enqueued on `stream` after this call waits for `event`'s last record (hipStreamWaitEvent),
+ * so that several streams can pipeline one job (the streaming runner). */
+rf_status rf_stream_wait_event(rf_stream stream, void* event);
 /* HIP graphs by stream capture: the work enqueued on `stream` (a created stream, not NULL)
  * between begin and end is recorded instead of run; end instantiates it (*graph is a
  * hipGraphExec_t). rf_graph_launch replays it on a stream, rf_graph_destroy frees it. Every

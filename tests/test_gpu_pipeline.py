@@ -587,6 +587,25 @@ def test_doa_recovers_the_array_angle() -> None:
     assert report["angle_deg"] == pytest.approx(ref, abs=1e-6)
 
 
+def test_doa_batch_lags_match_the_oracle() -> None:
+    # DOA's lags come from one batched GCC-PHAT over all channel pairs; pairs with at most 2048
+    # output lags are too short for the batch and take the per-pair path. Every lag equals the
+    # oracle's GCC-PHAT peak lag and the per-pair GPU path's.
+    rng = np.random.default_rng(61)
+    src = noise(62, 20000)
+    ref = src[1000:2000]                                      # 1000 samples
+    chans = [ref]
+    for n, d in [(900, 7), (1500, -33), (6000, 120), (1049, 0)]:
+        c = src[1000 - d:1000 - d + n] + 0.3 * noise(int(rng.integers(1 << 30)), n)
+        chans.append(c.astype(np.complex64))
+    lags = rf.tdoa_batch(chans)
+    for c, lag in zip(chans[1:], lags):
+        idx, _ = oracle.peak(oracle.gcc_phat(ref, c))
+        assert lag == idx - (ref.size - 1)
+        assert lag == rf.tdoa(ref, c)
+    assert lags[:3] == [7, -33, 120]
+
+
 # --- channelizer ---------------------------------------------------------------------------
 
 @pytest.mark.parametrize("use_plan", [False, True])
