@@ -542,7 +542,9 @@ def bench_delay(rf, args) -> dict:
         dt = time.perf_counter() - t0
         cpu = {"value": round(12 / dt, 3), "unit": "pairs/s", "cores": 1, "kind": "port",
                "sample": "float64 oracle features + exact GCC-PHAT + peak, one thread, 12 pairs"}
-    release(rf, batch, search, mlp, pres, res, a, b, d, stream)
+    release(rf, batch, search, pres, res, a, b, d)
+    long_capture = delay_long_capture(rf, mlp, stream, args)
+    release(rf, mlp, stream)
 
     def quality(results):
         agree = sum(r.index == e.index for r, e in zip(results, exact))
@@ -588,8 +590,36 @@ def bench_delay(rf, args) -> dict:
         "exact_gcc_batch": {"ms": round(ms_exact, 4),
                             "pairs_per_s": round(count / (ms_exact * 1e-3), 1),
                             "speedup_of_mlp_search": round(ms_exact / ms_mlp, 3)},
+        "long_capture": long_capture,
         "cpu_baseline": cpu,
     }
+
+
+def delay_long_capture(rf, mlp, stream, args, count: int = 1024, n: int = 1 << 18) -> dict:
+    """The same search on longer captures (FFT length 2^19, where the exact path's inverse
+    transform is a multi-pass four-step FFT): 1024 pairs of 2^18 samples of seed 0xC5, the
+    network trained at 2^14 samples, against the exact batched GCC-PHAT."""
+    a, b = rf.DeviceBuffer(count * n), rf.DeviceBuffer(count * n)
+    rf.synth_pairs(0xC5, 0, count, n, a.ptr, b.ptr, None, stream)
+    search = rf.DelaySearch(count, n)
+    res = rf.DeviceBuffer(32 * count, np.uint8)
+    exact = rf.GccBatch([n] * count, [n] * count)
+    exact.bind([a.offset(k * n) for k in range(count)], [b.offset(k * n) for k in range(count)])
+    pres = rf.DeviceBuffer(16 * count, np.uint8)
+    reps = max(2, args.steps // 4)
+    ms_exact = time_steps(rf, lambda i: exact.run(pres.ptr, stream.handle), reps, 1,
+                          stream.handle) / reps
+    want = [p.index for p in rf.read_peaks(pres, count)]
+    out = {"pairs": count, "samples_per_channel": n, "exact_gcc_batch_ms": round(ms_exact, 4)}
+    for name, net, threshold in [("mlp", mlp, 0.9), ("coarse", None, 0.0)]:
+        ms = time_steps(rf, lambda i: search.run(a.ptr, b.ptr, net, threshold, res.ptr, stream),
+                        reps, 1, stream.handle) / reps
+        got = rf.read_delay_results(res, count)
+        out[name] = {"ms": round(ms, 4), "speedup_over_exact": round(ms_exact / ms, 3),
+                     "lag_equal_to_exact": int(sum(g.index == w for g, w in zip(got, want))),
+                     "full_inverse_transforms": int(sum(g.path == 1 for g in got))}
+    release(rf, exact, search, a, b, res, pres)
+    return out
 
 
 def bench_copy(rf, args) -> dict:

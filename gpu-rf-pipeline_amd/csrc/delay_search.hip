@@ -406,8 +406,16 @@ __global__ __launch_bounds__(kRefineThreads) void search_refine_kernel(
         }
         if (t == 0) {
             const int best = static_cast<int>(0xFFFFFFFFu - static_cast<unsigned>(key & 0xFFFFFFFFULL));
-            if (best == 0 || best == kWindow - 1) {
-                active[p] = 1;           // the peak may lie outside the window
+            // The exact window peak judges the proposal. It is rejected (full inverse transform)
+            // when it sits at the window's edge (the peak may lie outside the window), or when
+            // it does not stand out from the PHAT noise floor: with unit-magnitude weights W of
+            // 1/L each, a lag carrying no signal has E|r|^2 = 1/L and P(|r|^2 > t/L) = e^-t, so
+            // all L lags stay below sqrt((ln L + 5) / L) with probability 1 - e^-5. A window peak
+            // below that level is no evidence that the window holds the global peak.
+            const float floor2 = (static_cast<float>(d.log2L) * 0.69314718f + 5.0f)
+                                 / static_cast<float>(L);
+            if (best == 0 || best == kWindow - 1 || mag[best] * mag[best] < floor2) {
+                active[p] = 1;
                 results[p].path = 1;
                 fallback[1 + atomicAdd(&fallback[0], 1)] = p;
             } else {

@@ -387,7 +387,9 @@ rf_status rf_delay_search_features(rf_delay_search* handle, const rf_cf32* a, co
                                    float* features, int* coarse, rf_stream stream);
 /* The search. mlp NULL: the coarse proposal (largest coarse |GCC-PHAT|) for every pair.
  * Pairs whose confidence is below `threshold` (threshold > 1: every pair) take the full
- * inverse transform. results: DEVICE, count records. */
+ * inverse transform, and so do pairs whose exact window peak sits at the window's edge or
+ * below the PHAT noise floor sqrt((ln L + 5) / L) (all L lags of pure noise stay below it
+ * with probability 1 - e^-5). results: DEVICE, count records. */
 rf_status rf_delay_search_run(rf_delay_search* handle, const rf_cf32* a, const rf_cf32* b,
                               rf_delay_mlp* mlp, float threshold, rf_delay_result* results,
                               rf_stream stream);

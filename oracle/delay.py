@@ -280,9 +280,16 @@ def window_lags(bin_index: int) -> np.ndarray:
     return np.arange(lo, lo + WINDOW)
 
 
+def noise_floor(L: int) -> float:
+    """|r| level that the L lags of PHAT noise (E|r|^2 = 1/L, P(|r|^2 > t/L) = e^-t) all stay
+    below with probability 1 - e^-5: sqrt((ln L + 5) / L)."""
+    return float(np.sqrt((np.log(L) + 5.0) / L))
+
+
 def refine(w: np.ndarray, na: int, bin_index: int):
     """(output index, |r|, frac) of the first largest |r| over the proposal's window, or None
-    when it sits at either edge; r(l) = sum_k W[k] exp(2 pi i k (l + na - 1) / L)."""
+    when it sits at either edge or below noise_floor(L); r(l) = sum_k W[k]
+    exp(2 pi i k (l + na - 1) / L)."""
     L = w.size
     lags = window_lags(bin_index)
     n = (lags + na - 1) % L
@@ -290,7 +297,7 @@ def refine(w: np.ndarray, na: int, bin_index: int):
     r = np.array([np.sum(w * np.exp(2j * np.pi * ((k * int(nn)) % L) / L)) for nn in n])
     mag = np.abs(r)
     best = int(np.argmax(mag))
-    if best in (0, WINDOW - 1):
+    if best in (0, WINDOW - 1) or mag[best] < noise_floor(L):
         return None
     m0, ml, mh = mag[best], mag[best - 1], mag[best + 1]
     frac = 0.0

@@ -188,3 +188,19 @@ def test_refine_finds_the_window_peak() -> None:
     out = D.refine(w, n, D.label_of(q["delay"]))
     assert out is not None and out[0] == exact
     assert out[1] == pytest.approx(abs(r[exact]), rel=1e-9)
+
+
+def test_refine_rejects_a_window_of_pure_noise() -> None:
+    # Independent captures: the PHAT output holds no signal. Its lags have E|r|^2 = 1/L (unit
+    # weights of 1/L each), no lag reaches the noise floor sqrt((ln L + 5) / L), and the refine
+    # rejects every window it is offered.
+    n = 1 << 12
+    rng = np.random.default_rng(17)
+    a = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    w = D.phat_spectrum(a, b)
+    L = w.size
+    r = np.fft.ifft(w) * L
+    assert np.mean(np.abs(r) ** 2) == pytest.approx(1.0 / L, rel=1e-9)
+    assert np.max(np.abs(r)) < D.noise_floor(L)
+    assert all(D.refine(w, n, b) is None for b in (0, 17, 31, 63))
