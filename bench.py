@@ -410,10 +410,12 @@ def bench_gcc_batch(rf, args) -> dict:
     h = rf.GccBatch(na, nb)
     h.bind([x.ptr for x in d_a], [x.ptr for x in d_b])
     res = rf.DeviceBuffer(16 * count, np.uint8)
-    tuned = h.tune(res.ptr, stream.handle)
+    # --no-variants: the library's default configuration only (no tuner, no group sweep, no
+    # per-pair path), so that a profiler sees one configuration per launch
+    tuned = None if args.no_variants else h.tune(res.ptr, stream.handle)
     group_bytes, groups = h.group_bytes()
     sweep = {}
-    for budget in (0, 48 << 20, 96 << 20, 160 << 20):
+    for budget in () if args.no_variants else (0, 48 << 20, 96 << 20, 160 << 20):
         h.set_group_bytes(budget)
         ms = time_steps(rf, lambda i: h.run(res.ptr, stream.handle), max(2, args.steps // 2), 1,
                         stream.handle) / max(2, args.steps // 2)
@@ -435,8 +437,12 @@ def bench_gcc_batch(rf, args) -> dict:
             singles[k].run(d_a[k].ptr, d_b[k].ptr, d_r[k].ptr, stream.handle)
             rf.peak_device(d_r[k].ptr, na[k] + nb[k] - 1, pres.offset(16 * k), stream.handle)
 
-    ms_plain = time_steps(rf, plain, max(2, args.steps // 4), 1, stream.handle) / max(2, args.steps // 4)
-    plain_peaks = rf.read_peaks(pres, count)
+    if args.no_variants:
+        ms_plain, plain_peaks = float("nan"), peaks
+    else:
+        ms_plain = time_steps(rf, plain, max(2, args.steps // 4), 1,
+                              stream.handle) / max(2, args.steps // 4)
+        plain_peaks = rf.read_peaks(pres, count)
     release(rf, h, singles, d_r, pres, res, d_a, d_b, stream)
     same = int(sum(p.index == q.index for p, q in zip(peaks, plain_peaks)))
     lag_err = [float(abs(p.index - (n - 1) + p.frac - d)) for p, n, d in zip(peaks, na, delays)]
@@ -478,9 +484,9 @@ def bench_gcc_batch(rf, args) -> dict:
                     "tuned_workgroups_per_cu": tuned,
                     "tuned_group_bytes": group_bytes, "groups": groups,
                     "group_budget_sweep_ms": sweep},
-        "plain_per_pair": {"ms": round(ms_plain, 4), "pairs_per_s": round(count / (ms_plain * 1e-3), 1),
-                           "speedup_of_batched": round(ms_plain / best, 2),
-                           "same_peak_index": same},
+        "plain_per_pair": None if args.no_variants else {
+            "ms": round(ms_plain, 4), "pairs_per_s": round(count / (ms_plain * 1e-3), 1),
+            "speedup_of_batched": round(ms_plain / best, 2), "same_peak_index": same},
         "accuracy": {"pairs_within_0.1_sample": within, "max_abs_lag_error": round(max(lag_err), 4),
                      "oracle_check": checked},
         "cpu_baseline": None if args.no_cpu else {

@@ -512,9 +512,11 @@ void features_launch(rf_delay_search& h, const float2* a, const float2* b, hipSt
         check_launch("search_features_kernel");
         return;
     }
-    batch_column_fwd_kernel<<<persistent_grid(2LL * h.col_total, 4), kItemThreads,
-                              column_lds_bytes(), stream>>>(h.dev, h.col_map, 0, h.col_total,
-                                                            table);
+    allow_large_lds(reinterpret_cast<const void*>(batch_column_fwd_kernel),
+                    column_fwd_lds_bytes());
+    batch_column_fwd_kernel<<<persistent_grid(h.col_total, kFwdWorkgroupsPerCu), kFwdThreads,
+                              column_fwd_lds_bytes(), stream>>>(h.dev, h.col_map, 0, h.col_total,
+                                                                table);
     check_launch("batch_column_fwd_kernel");
     const int rows = h.count << h.fs.log2n2;
     search_row_kernel<<<persistent_grid(rows, 2), k4096Lanes, 0, stream>>>(

@@ -49,23 +49,24 @@ __device__ __forceinline__ float2 four_step_twiddle32(const FourStep& fs, unsign
     return INV ? make_float2(w.x, -w.y) : w;
 }
 
-// Columns per workgroup of the column pass: C * n2 = 4096 values (512 lanes), or all n1
-// columns if there are fewer.
+// Columns per workgroup of the column pass: C * n2 = 2^TILE_LOG2 values (2^TILE_LOG2 / 8
+// lanes; 4096 values and 512 lanes by default), or all n1 columns if there are fewer.
+template <int TILE_LOG2 = kTwiddleLog2>
 __host__ __device__ __forceinline__ int column_tile_log2(const FourStep& fs)
 {
-    const int c = kTwiddleLog2 - fs.log2n2;
+    const int c = TILE_LOG2 - fs.log2n2;
     return c < fs.log2n1 ? c : fs.log2n1;
 }
 
+template <int TILE_LOG2 = kTwiddleLog2>
 __host__ __device__ __forceinline__ int column_tile(const FourStep& fs)
 {
-    return 1 << column_tile_log2(fs);
+    return 1 << column_tile_log2<TILE_LOG2>(fs);
 }
 
 // LDS stride between the columns of a column tile: the padded column, n2 + n2/8 + 1 slots
 // (odd for n2 >= 16, so the lanes that touch one row of consecutive columns fall on
-// different banks). The batched column kernels size their LDS for C * column_slots(n2)
-// = 4096 + 512 + C values.
+// different banks). A tile of 2^T values takes C * column_slots(n2) = 2^T + 2^T / 8 + C.
 __host__ __device__ constexpr int column_slots(int n2)
 {
     return lds_slots(n2);
@@ -75,7 +76,7 @@ __host__ __device__ constexpr int column_slots(int n2)
 // the input at n = j1 + n1 j2; `store(j1, k2, v)` takes the twiddled result. The workgroup has
 // C n2 / 8 lanes and every lane moves 8 elements, all 8 loads in flight before the first LDS
 // write and all 8 twiddle lookups before the first store.
-template <bool INV, class Load, class Store>
+template <bool INV, int TILE_LOG2 = kTwiddleLog2, class Load, class Store>
 __device__ __forceinline__ void column_pass_block(const FourStep& fs, long long c0, Load load,
                                                   Store store, float2* lds,
                                                   const float2* __restrict__ table)
@@ -86,7 +87,7 @@ __device__ __forceinline__ void column_pass_block(const FourStep& fs, long long 
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     const int n2 = 1 << fs.log2n2;
-    const int lc = column_tile_log2(fs);
+    const int lc = column_tile_log2<TILE_LOG2>(fs);
     const int C = 1 << lc;
     const int slots = column_slots(n2);
     const int lanes = (C * n2) >> 3;

@@ -25,17 +25,6 @@ struct StreamScratch {
     StreamScratch& operator=(const StreamScratch&) = delete;
 };
 
-// Raises a kernel's dynamic-LDS limit above 64 KiB where the runtime wants that; a runtime
-// that does not need it may refuse the call, and the launch itself then reports any problem.
-inline void allow_large_lds(const void* kernel, std::size_t bytes)
-{
-    if (bytes > 64 * 1024
-        && hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               static_cast<int>(bytes)) != hipSuccess) {
-        (void)hipGetLastError();
-    }
-}
-
 // Split-precision Toeplitz-GEMM FIR on the matrix cores (fir_mfma.hip). `workspace` holds the
 // split taps: kStreamWorkspaceBytes of device memory, used in stream order.
 void fir_mfma_launch(bool bf16, const float2* x, const float* taps, float2* y, int n,

@@ -223,11 +223,18 @@ void batch_run(rf_gcc_batch& h, rf_peak_result* results, hipStream_t stream)
     auto grid_for = [cap](int items) {
         return static_cast<unsigned>(ceil_div(std::min(items, cap), 8) * 8);
     };
+    // the forward column pass's 80 KiB workgroups: never more than fit on the CUs at once
+    const int fwd_cap = std::min(h.grid_mult, kFwdWorkgroupsPerCu) * cus;
+    auto grid_fwd = [fwd_cap](int items) {
+        return static_cast<unsigned>(ceil_div(std::min(items, fwd_cap), 8) * 8);
+    };
+    allow_large_lds(reinterpret_cast<const void*>(batch_column_fwd_kernel),
+                    column_fwd_lds_bytes());
     for (const rf_gcc_batch::Group& g : h.groups) {
         if (g.col_items == 0) {
             continue;  // only pairs of the single-workgroup path
         }
-        batch_column_fwd_kernel<<<grid_for(2 * g.col_items), kItemThreads, column_lds_bytes(),
+        batch_column_fwd_kernel<<<grid_fwd(g.col_items), kFwdThreads, column_fwd_lds_bytes(),
                                   stream>>>(h.dev, h.col_map, g.col_begin, g.col_items, table);
         check_launch("batch_column_fwd_kernel");
         if (g.row_items[0] > 0) {
@@ -240,8 +247,8 @@ void batch_run(rf_gcc_batch& h, rf_peak_result* results, hipStream_t stream)
                 h.dev, h.rows_generic.map, g.row_begin[1], g.row_items[1], table);
             check_launch("batch_row_kernel");
         }
-        batch_column_inv_peak_kernel<<<grid_for(g.col_items), kItemThreads, column_lds_bytes(),
-                                       stream>>>(h.dev, h.col_map, g.col_begin, g.col_items,
+        batch_column_inv_peak_kernel<<<grid_for(g.col_items), kItemThreads,
+                                       column_lds_bytes(), stream>>>(h.dev, h.col_map, g.col_begin, g.col_items,
                                                  results, ItemSet{nullptr, nullptr, 0}, table);
         check_launch("batch_column_inv_peak_kernel");
     }

@@ -75,6 +75,17 @@ inline void check_launch(const char* what)
     }
 }
 
+// Raises a kernel's dynamic-LDS limit above 64 KiB where the runtime wants that; a runtime
+// that does not need it may refuse the call, and the launch itself then reports any problem.
+inline void allow_large_lds(const void* kernel, std::size_t bytes)
+{
+    if (bytes > 64 * 1024
+        && hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               static_cast<int>(bytes)) != hipSuccess) {
+        (void)hipGetLastError();
+    }
+}
+
 // LDS bytes one workgroup may use on the current device.
 int max_lds_per_block();
 // LDS bytes of one compute unit of the current device.
