@@ -148,6 +148,11 @@ _proto("rf_graph_destroy", _st, _vp)
 _proto("rf_fir_plan_create", _st, _int, _vp, _int, _int, _vp, ctypes.POINTER(_vp))
 _proto("rf_fir_plan_run", _st, _vp, _vp, _vp, _int, _vp)
 _proto("rf_fir_plan_destroy", _st, _vp)
+_proto("rf_fir_batch_create", _st, _int, _vp, _int, _int, ctypes.POINTER(_int), _vp,
+       ctypes.POINTER(_vp))
+_proto("rf_fir_batch_bind", _st, _vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp))
+_proto("rf_fir_batch_run", _st, _vp, _vp)
+_proto("rf_fir_batch_destroy", _st, _vp)
 _proto("rf_fft_plan_create", _st, _int, _int, ctypes.POINTER(_vp))
 _proto("rf_fft_plan_destroy", _st, _vp)
 _proto("rf_fft_plan_bytes", _st, _vp, ctypes.POINTER(_size))
@@ -833,6 +838,36 @@ class FirPlan:
 
     def __del__(self):
         _finalize(self, "handle", _lib.rf_fir_plan_destroy)
+
+
+class FirBatch:
+    """Persistent FIR over many uneven captures sharing one filter (rf_fir_batch_*): one launch
+    per run, each capture's output the bits of rf_fir with the same impl."""
+
+    def __init__(self, impl: str | int, taps: int, num_taps: int, lengths: Sequence[int],
+                 stream=None):
+        idx = fir_impl_index(impl) if isinstance(impl, str) else impl
+        n = (_int * len(lengths))(*lengths)
+        p = _vp()
+        check(_lib.rf_fir_batch_create(idx, taps, num_taps, len(lengths), n, _sp(stream),
+                                       ctypes.byref(p)))
+        self.handle = p.value
+        self.count = len(lengths)
+
+    def bind(self, x: Sequence[int], y: Sequence[int]) -> None:
+        check(_lib.rf_fir_batch_bind(self.handle, (_vp * self.count)(*x),
+                                     (_vp * self.count)(*y)))
+
+    def run(self, stream=None) -> None:
+        check(_lib.rf_fir_batch_run(self.handle, _sp(stream)))
+
+    def destroy(self) -> None:
+        if self.handle:
+            check(_lib.rf_fir_batch_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        _finalize(self, "handle", _lib.rf_fir_batch_destroy)
 
 
 DELAY_FEATURES = 256

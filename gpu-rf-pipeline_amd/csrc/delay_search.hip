@@ -58,8 +58,10 @@ struct rf_delay_search {
     rf::FourStep fs{};
     std::vector<rf::PairDesc> host;
     rf::PairDesc* dev = nullptr;
-    int2* col_map = nullptr;    // col_total items: {pair, column tile}
+    int2* col_map = nullptr;    // col_total items: {pair, column tile} of the inverse pass
     int col_total = 0;
+    int2* fwd_map = nullptr;    // fwd_total items of the forward pass (wider tiles)
+    int fwd_total = 0;
     float2* work = nullptr;     // count * 2 L
     float2* tables = nullptr;   // four-step twiddles of L
     float2* partial = nullptr;  // count * n2 * 1024 band sums per row
@@ -514,9 +516,9 @@ void features_launch(rf_delay_search& h, const float2* a, const float2* b, hipSt
     }
     allow_large_lds(reinterpret_cast<const void*>(batch_column_fwd_kernel),
                     column_fwd_lds_bytes());
-    batch_column_fwd_kernel<<<persistent_grid(h.col_total, kFwdWorkgroupsPerCu), kFwdThreads,
-                              column_fwd_lds_bytes(), stream>>>(h.dev, h.col_map, 0, h.col_total,
-                                                                table);
+    batch_column_fwd_kernel<<<persistent_grid(2LL * h.fwd_total, kFwdWorkgroupsPerCu),
+                              kFwdThreads, column_fwd_lds_bytes(), stream>>>(
+        h.dev, h.fwd_map, 0, h.fwd_total, table);
     check_launch("batch_column_fwd_kernel");
     const int rows = h.count << h.fs.log2n2;
     search_row_kernel<<<persistent_grid(rows, 2), k4096Lanes, 0, stream>>>(
@@ -579,6 +581,7 @@ void destroy_search(rf_delay_search* h)
     }
     (void)hipFree(h->dev);
     (void)hipFree(h->col_map);
+    (void)hipFree(h->fwd_map);
     (void)hipFree(h->work);
     (void)hipFree(h->tables);
     (void)hipFree(h->partial);
@@ -676,7 +679,9 @@ rf_delay_search* create_search(int count, int n)
         RF_HIP_CHECK(hipMalloc(&h->fallback, (static_cast<std::size_t>(count) + 1) * sizeof(int)));
         h->host.resize(static_cast<std::size_t>(count));
         std::vector<int2> col_map;
+        std::vector<int2> fwd_map;
         const int col_items = (1 << h->fs.log2n1) / column_tile(h->fs);
+        const int fwd_items = (1 << h->fs.log2n1) / column_tile<kFwdTileLog2>(h->fs);
         for (int i = 0; i < count; ++i) {
             PairDesc& d = h->host[static_cast<std::size_t>(i)];
             d = PairDesc{};
@@ -686,6 +691,7 @@ rf_delay_search* create_search(int count, int n)
             d.log2n1 = h->fs.log2n1;
             d.log2n2 = h->fs.log2n2;
             d.col_items = col_items;
+            d.fwd_items = fwd_items;
             d.row_items = n2;
             d.work = h->work + 2LL * L * i;
             d.lo = h->tables;
@@ -694,17 +700,25 @@ rf_delay_search* create_search(int count, int n)
             for (int c = 0; c < col_items; ++c) {
                 col_map.push_back(make_int2(i, c));
             }
+            for (int c = 0; c < fwd_items; ++c) {
+                fwd_map.push_back(make_int2(i, c));
+            }
         }
         h->col_total = static_cast<int>(col_map.size());
+        h->fwd_total = static_cast<int>(fwd_map.size());
         RF_HIP_CHECK(hipMalloc(&h->dev, h->host.size() * sizeof(PairDesc)));
         RF_HIP_CHECK(hipMemcpy(h->dev, h->host.data(), h->host.size() * sizeof(PairDesc),
                                hipMemcpyHostToDevice));
         RF_HIP_CHECK(hipMalloc(&h->col_map, col_map.size() * sizeof(int2)));
         RF_HIP_CHECK(hipMemcpy(h->col_map, col_map.data(), col_map.size() * sizeof(int2),
                                hipMemcpyHostToDevice));
+        RF_HIP_CHECK(hipMalloc(&h->fwd_map, fwd_map.size() * sizeof(int2)));
+        RF_HIP_CHECK(hipMemcpy(h->fwd_map, fwd_map.data(), fwd_map.size() * sizeof(int2),
+                               hipMemcpyHostToDevice));
         h->bytes += (tsize + work + partial + L) * sizeof(float2)
                    + static_cast<std::size_t>(count) * (kFeatures + kBins + 2) * sizeof(float)
-                   + h->host.size() * sizeof(PairDesc) + col_map.size() * sizeof(int2);
+                   + h->host.size() * sizeof(PairDesc)
+                   + (col_map.size() + fwd_map.size()) * sizeof(int2);
     } catch (...) {
         destroy_search(h);
         throw;

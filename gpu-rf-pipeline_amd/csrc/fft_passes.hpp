@@ -73,9 +73,10 @@ __host__ __device__ constexpr int column_slots(int n2)
 }
 
 // Column pass over one matrix: workgroup handles columns [c0, c0 + C). `load(j1, j2)` gives
-// the input at n = j1 + n1 j2; `store(j1, k2, v)` takes the twiddled result. The workgroup has
-// C n2 / 8 lanes and every lane moves 8 elements, all 8 loads in flight before the first LDS
-// write and all 8 twiddle lookups before the first store.
+// the input at n = j1 + n1 j2; `store(j1, k2, v)` takes the twiddled result. C n2 / 8 lanes
+// work (a workgroup may have more: a tile narrower than 2^TILE_LOG2 leaves the rest idle, but
+// every lane reaches the barriers) and every working lane moves 8 elements, all 8 loads in
+// flight before the first LDS write and all 8 twiddle lookups before the first store.
 template <bool INV, int TILE_LOG2 = kTwiddleLog2, class Load, class Store>
 __device__ __forceinline__ void column_pass_block(const FourStep& fs, long long c0, Load load,
                                                   Store store, float2* lds,
@@ -91,21 +92,27 @@ __device__ __forceinline__ void column_pass_block(const FourStep& fs, long long 
     const int C = 1 << lc;
     const int slots = column_slots(n2);
     const int lanes = (C * n2) >> 3;
+    const bool working = tid < lanes;
     float2 v[8];
+    if (working) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int e = tid + i * lanes;
-        v[i] = load(c0 + (e & (C - 1)), e >> lc);
-    }
+        for (int i = 0; i < 8; ++i) {
+            const int e = tid + i * lanes;
+            v[i] = load(c0 + (e & (C - 1)), e >> lc);
+        }
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int e = tid + i * lanes;
-        lds[(e & (C - 1)) * slots + lds_pad(e >> lc)] = v[i];
+        for (int i = 0; i < 8; ++i) {
+            const int e = tid + i * lanes;
+            lds[(e & (C - 1)) * slots + lds_pad(e >> lc)] = v[i];
+        }
     }
     __syncthreads();
     const int f = tid >> (fs.log2n2 - 3);  // n2 / 8 lanes per column transform
     const int t = tid & ((n2 >> 3) - 1);
     lds_fft<INV>(lds + (f < C ? f : 0) * slots, fs.log2n2, t, f < C, table);
+    if (!working) {
+        return;  // after the transform's last barrier
+    }
     // The lane's 8 elements share the column j1 and step k2 by n2 / 8: their four-step factors
     // W^(j1 k2) are b s^i with b = W^(j1 k2_0) and s = W^(j1 n2 / 8), two table lookups
     // instead of eight (at most four roundings more).

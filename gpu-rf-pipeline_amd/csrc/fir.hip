@@ -144,17 +144,18 @@ __device__ __forceinline__ int reg_slot(int s)
 // with Tpad = ceil(T / R) * R, so lane l's window for taps [cR, cR + R) is group
 // (Tpad/R + l - c) ("cur") and the group before it ("prev"):
 //   x[first + j - k] = (j >= i) ? cur[j - i] : prev[R + j - i],   k = cR + i.
+// `block` is the index of the blockDim.x * R outputs this workgroup computes.
 template <int R>
 __device__ __forceinline__ void fir_register_block(const float2* __restrict__ x,
                                                    const float* __restrict__ taps,
                                                    float2* __restrict__ y, int n, int num_taps,
-                                                   float2* tile)
+                                                   float2* tile, long long block)
 {
     const int outputs = static_cast<int>(blockDim.x) * R;
     const int chunks = (num_taps + R - 1) / R;
     const int tpad = chunks * R;
     const int len = tpad + outputs;  // a multiple of R, so even
-    const long long block_start = static_cast<long long>(blockIdx.x) * outputs;
+    const long long block_start = block * outputs;
     const long long tile_start = block_start - tpad;
 
     for (int s = 2 * threadIdx.x; s < len; s += 2 * blockDim.x) {
@@ -255,7 +256,7 @@ __global__ void fir_register_kernel(const float2* __restrict__ x, const float* _
                                     float2* __restrict__ y, int n, int num_taps)
 {
     extern __shared__ float2 tile[];
-    fir_register_block<R>(x, taps, y, n, num_taps, tile);
+    fir_register_block<R>(x, taps, y, n, num_taps, tile, blockIdx.x);
 }
 
 // --- batched launches (src/fir_batch.hpp:9-13, src/fused.cu:36-74, src/fir.cu:562-721) ----
@@ -297,7 +298,7 @@ __global__ void fir_register_batched_kernel(FirBatch batch, const float* __restr
     if (static_cast<long long>(blockIdx.x) * blockDim.x * R >= batch.n[c]) {
         return;
     }
-    fir_register_block<R>(batch.x[c], taps, batch.y[c], batch.n[c], num_taps, tile);
+    fir_register_block<R>(batch.x[c], taps, batch.y[c], batch.n[c], num_taps, tile, blockIdx.x);
 }
 
 // --- fft_ola --------------------------------------------------------------------------------
@@ -501,6 +502,47 @@ __global__ __launch_bounds__(k4096Lanes) void fir_ola_batched_kernel(
         return;
     }
     ola_segment(batch.x[c], spec, batch.y[c], batch.n[c], halo, out0, buf, stages);
+}
+
+// --- persistent batch over any number of uneven captures --------------------------------------
+
+// One capture of a persistent FIR batch.
+struct FirCapture {
+    const float2* x;
+    float2* y;
+    int n;
+};
+
+// Work items {capture, segment} (overlap-save) or {capture, output block} (register), built on
+// the host in capture order, walked XCD-major: each XCD takes a contiguous run of items, so a
+// segment's halo was just read by its predecessor behind the same L2.
+__global__ __launch_bounds__(k4096Lanes) void fir_ola_batch_kernel(
+    const FirCapture* __restrict__ caps, const int2* __restrict__ items, int total,
+    const float2* __restrict__ spec, int num_taps, const float2* __restrict__ stages)
+{
+    __shared__ float2 buf[k4096Slots];
+    const int halo = num_taps - 1;
+    xcd_major_items(total, [&](int item) {
+        const int2 it = items[item];
+        const FirCapture c = caps[it.x];
+        ola_segment(c.x, spec, c.y, c.n, halo, static_cast<long long>(it.y) * (kOlaP - halo), buf,
+                    stages);
+        __syncthreads();  // the next segment reuses buf
+    });
+}
+
+template <int R>
+__global__ void fir_register_batch_kernel(const FirCapture* __restrict__ caps,
+                                          const int2* __restrict__ items, int total,
+                                          const float* __restrict__ taps, int num_taps)
+{
+    extern __shared__ float2 tile[];
+    xcd_major_items(total, [&](int item) {
+        const int2 it = items[item];
+        const FirCapture c = caps[it.x];
+        fir_register_block<R>(c.x, taps, c.y, c.n, num_taps, tile, it.y);
+        __syncthreads();  // the next block refills the tile
+    });
 }
 
 // --- host side ------------------------------------------------------------------------------
@@ -1166,6 +1208,176 @@ rf_status rf_fir_plan_run(const rf_fir_plan* plan, const rf_cf32* x, rf_cf32* y,
 rf_status rf_fir_plan_destroy(rf_fir_plan* plan)
 {
     return guarded([&] { destroy_fir_plan(plan); });
+}
+
+}  // extern "C"
+
+// A persistent FIR over many captures of uneven length sharing one filter (rf_fir_batch_*).
+struct rf_fir_batch {
+    int impl = 0;
+    int num_taps = 0;
+    int count = 0;
+    int total = 0;                 // work items
+    bool ola = false;              // overlap-save segments, else register blocks
+    int r = 4;                     // register: outputs per lane
+    std::size_t lds = 0;           // register: dynamic LDS per workgroup
+    float* taps = nullptr;         // device copy
+    float2* spec = nullptr;        // overlap-save tap spectrum
+    rf::FirCapture* caps = nullptr;
+    int2* items = nullptr;
+    std::vector<int> n;
+    bool bound = false;
+};
+
+namespace {
+
+void destroy_fir_batch(rf_fir_batch* b)
+{
+    if (b == nullptr) {
+        return;
+    }
+    (void)hipFree(b->taps);
+    (void)hipFree(b->spec);
+    (void)hipFree(b->caps);
+    (void)hipFree(b->items);
+    delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+rf_status rf_fir_batch_create(int impl, const float* taps, int num_taps, int count,
+                              const int* num_samples, rf_stream stream, rf_fir_batch** out)
+{
+    return guarded([&] {
+        if (out == nullptr) rf::fail(RF_INVALID_ARG, "fir batch: NULL output handle");
+        *out = nullptr;
+        if (impl != RF_FIR_REGISTER && impl != RF_FIR_FFT_OLA) {
+            rf::fail(RF_INVALID_ARG, "fir batch: impl must be register or fft_ola");
+        }
+        if (count < 1 || num_samples == nullptr) {
+            rf::fail(RF_INVALID_ARG, "fir batch: need at least one capture");
+        }
+        static const float2 probe{};
+        rf::check_fir_args(impl, &probe, taps, &probe, 1, num_taps, RF_DEFAULT_FIR_BLOCK);
+        const bool ola = impl == RF_FIR_FFT_OLA && rf::ola_applies(num_taps);
+        if (!ola) {
+            rf::check_const_taps(num_taps, "register");
+        }
+        long long longest = 0;
+        for (int c = 0; c < count; ++c) {
+            if (num_samples[c] < 0) rf::fail(RF_INVALID_ARG, "fir batch: negative sample count");
+            longest = std::max<long long>(longest, num_samples[c]);
+        }
+        rf_fir_batch* b = new rf_fir_batch;
+        b->impl = impl;
+        b->num_taps = num_taps;
+        b->count = count;
+        b->ola = ola;
+        b->n.assign(num_samples, num_samples + count);
+        try {
+            long long per;  // outputs per item
+            if (ola) {
+                per = rf::kOlaP - (num_taps - 1);
+            } else {
+                b->r = rf::register_outputs(num_taps, RF_DEFAULT_FIR_BLOCK, longest, &b->lds);
+                per = static_cast<long long>(RF_DEFAULT_FIR_BLOCK) * b->r;
+            }
+            std::vector<int2> items;
+            for (int c = 0; c < count; ++c) {
+                const long long k = rf::ceil_div(b->n[static_cast<std::size_t>(c)], per);
+                for (long long i = 0; i < k; ++i) {
+                    items.push_back(make_int2(c, static_cast<int>(i)));
+                }
+            }
+            if (items.size() > static_cast<std::size_t>(INT32_MAX)) {
+                rf::fail(RF_LIMIT, "fir batch: more than 2^31 work items");
+            }
+            b->total = static_cast<int>(items.size());
+            hipStream_t s = rf::as_stream(stream);
+            RF_HIP_CHECK(hipMalloc(&b->taps, sizeof(float) * num_taps));
+            RF_HIP_CHECK(hipMemcpyAsync(b->taps, taps, sizeof(float) * num_taps,
+                                        hipMemcpyDeviceToDevice, s));
+            if (ola) {
+                RF_HIP_CHECK(hipMalloc(&b->spec, sizeof(float2) * rf::kOlaP));
+                rf::launch_ola_taps(b->taps, num_taps, b->spec, s);
+            }
+            RF_HIP_CHECK(hipMalloc(&b->caps, sizeof(rf::FirCapture) * count));
+            if (!items.empty()) {
+                RF_HIP_CHECK(hipMalloc(&b->items, sizeof(int2) * items.size()));
+                RF_HIP_CHECK(hipMemcpyAsync(b->items, items.data(), sizeof(int2) * items.size(),
+                                            hipMemcpyHostToDevice, s));
+            }
+            RF_HIP_CHECK(hipStreamSynchronize(s));
+        } catch (...) {
+            destroy_fir_batch(b);
+            throw;
+        }
+        *out = b;
+    });
+}
+
+rf_status rf_fir_batch_bind(rf_fir_batch* batch, const rf_cf32* const* x, rf_cf32* const* y)
+{
+    return guarded([&] {
+        if (batch == nullptr || x == nullptr || y == nullptr) {
+            rf::fail(RF_INVALID_ARG, "fir batch: NULL batch or capture arrays");
+        }
+        std::vector<rf::FirCapture> caps(static_cast<std::size_t>(batch->count));
+        for (int c = 0; c < batch->count; ++c) {
+            const int n = batch->n[static_cast<std::size_t>(c)];
+            if (n > 0 && (x[c] == nullptr || y[c] == nullptr)) {
+                rf::fail(RF_INVALID_ARG, "fir batch: NULL buffer for capture " + std::to_string(c));
+            }
+            caps[static_cast<std::size_t>(c)] =
+                rf::FirCapture{reinterpret_cast<const float2*>(x[c]), reinterpret_cast<float2*>(y[c]), n};
+        }
+        RF_HIP_CHECK(hipMemcpy(batch->caps, caps.data(), sizeof(rf::FirCapture) * caps.size(),
+                               hipMemcpyHostToDevice));
+        batch->bound = true;
+    });
+}
+
+rf_status rf_fir_batch_run(const rf_fir_batch* batch, rf_stream stream)
+{
+    return guarded([&] {
+        if (batch == nullptr) rf::fail(RF_INVALID_ARG, "fir batch: NULL batch");
+        if (!batch->bound) rf::fail(RF_INVALID_ARG, "fir batch: run before bind");
+        if (batch->total == 0) {
+            return;
+        }
+        hipStream_t s = rf::as_stream(stream);
+        // persistent grid: four workgroups per CU (a multiple of 8 for the XCD-major walk)
+        const long long cap = 4LL * rf::cu_count();
+        const unsigned grid = static_cast<unsigned>(
+            rf::ceil_div(std::min<long long>(batch->total, cap), 8) * 8);
+        if (batch->ola) {
+            rf::fir_ola_batch_kernel<<<grid, rf::k4096Lanes, 0, s>>>(
+                batch->caps, batch->items, batch->total, batch->spec, batch->num_taps,
+                rf::fft4096_stage_table());
+            rf::check_launch("fir_ola_batch_kernel");
+            return;
+        }
+        auto launch = [&](auto kernel) {
+            rf::allow_large_lds(reinterpret_cast<const void*>(kernel), batch->lds);
+            kernel<<<grid, RF_DEFAULT_FIR_BLOCK, batch->lds, s>>>(
+                batch->caps, batch->items, batch->total, batch->taps, batch->num_taps);
+        };
+        if (batch->r == 16) {
+            launch(rf::fir_register_batch_kernel<16>);
+        } else if (batch->r == 8) {
+            launch(rf::fir_register_batch_kernel<8>);
+        } else {
+            launch(rf::fir_register_batch_kernel<4>);
+        }
+        rf::check_launch("fir_register_batch_kernel");
+    });
+}
+
+rf_status rf_fir_batch_destroy(rf_fir_batch* batch)
+{
+    return guarded([&] { destroy_fir_batch(batch); });
 }
 
 }  // extern "C"

@@ -38,8 +38,9 @@ struct rf_gcc_batch {
     int count = 0;
     std::vector<PairDesc> host;
     PairDesc* dev = nullptr;
-    int2* col_map = nullptr;    // col_total items: {pair, column tile}
+    int2* col_map = nullptr;    // col_total items: {pair, column tile} of the inverse pass
     int col_total = 0;
+    int2* fwd_map = nullptr;    // the forward pass's items (wider tiles)
     // Row-pass items, split by row length: pairs with 4096-point rows run the register-resident
     // radix-16 kernel, the others the generic LDS kernel, each over its own item map.
     struct RowList {
@@ -52,6 +53,7 @@ struct rf_gcc_batch {
     // Host copies of the item prefixes (column items per pair, row items per list entry) and
     // the pair index of each list entry, for cutting the batch into groups.
     std::vector<int> col_prefix_host;
+    std::vector<int> fwd_prefix_host;
     std::vector<int> row_prefix_host[2];
     std::vector<int> row_pairs_host[2];
     // The batch runs as consecutive groups of pairs whose work buffers (2 L values per pair)
@@ -60,6 +62,7 @@ struct rf_gcc_batch {
     // one group.
     struct Group {
         int col_begin, col_items;
+        int fwd_begin, fwd_items;
         int row_begin[2], row_items[2];
     };
     std::vector<Group> groups;
@@ -185,6 +188,8 @@ void make_groups(rf_gcc_batch& h, std::size_t budget)
         rf_gcc_batch::Group g{};
         g.col_begin = h.col_prefix_host[static_cast<std::size_t>(p0)];
         g.col_items = h.col_prefix_host[static_cast<std::size_t>(p1)] - g.col_begin;
+        g.fwd_begin = h.fwd_prefix_host[static_cast<std::size_t>(p0)];
+        g.fwd_items = h.fwd_prefix_host[static_cast<std::size_t>(p1)] - g.fwd_begin;
         for (int kind = 0; kind < 2; ++kind) {
             const std::vector<int>& pairs = h.row_pairs_host[kind];
             const std::vector<int>& prefix = h.row_prefix_host[kind];
@@ -234,8 +239,9 @@ void batch_run(rf_gcc_batch& h, rf_peak_result* results, hipStream_t stream)
         if (g.col_items == 0) {
             continue;  // only pairs of the single-workgroup path
         }
-        batch_column_fwd_kernel<<<grid_fwd(g.col_items), kFwdThreads, column_fwd_lds_bytes(),
-                                  stream>>>(h.dev, h.col_map, g.col_begin, g.col_items, table);
+        batch_column_fwd_kernel<<<grid_fwd(2 * g.fwd_items), kFwdThreads,
+                                  column_fwd_lds_bytes(), stream>>>(h.dev, h.fwd_map, g.fwd_begin,
+                                                                    g.fwd_items, table);
         check_launch("batch_column_fwd_kernel");
         if (g.row_items[0] > 0) {
             batch_row4096_kernel<<<grid_for(g.row_items[0]), k4096Lanes, 0, stream>>>(
@@ -282,6 +288,7 @@ void destroy_batch(rf_gcc_batch* h)
     }
     (void)hipFree(h->dev);
     (void)hipFree(h->col_map);
+    (void)hipFree(h->fwd_map);
     for (auto* list : {&h->rows4096, &h->rows_generic}) {
         (void)hipFree(list->map);
     }
@@ -315,6 +322,7 @@ rf_status rf_gcc_batch_create(int count, const int* num_a, const int* num_b,
             h->count = count;
             h->host.resize(static_cast<std::size_t>(count));
             std::vector<int> colp(static_cast<std::size_t>(count) + 1, 0);
+            std::vector<int> fwdp(static_cast<std::size_t>(count) + 1, 0);
             std::vector<int> row_pairs[2];            // 4096-point rows, other rows
             std::vector<int> row_prefix[2] = {{0}, {0}};
             std::vector<int> fused;
@@ -352,9 +360,12 @@ rf_status rf_gcc_batch_create(int count, const int* num_a, const int* num_b,
                     d.fused = 1;
                     fused.push_back(i);
                     colp[static_cast<std::size_t>(i) + 1] = colp[static_cast<std::size_t>(i)];
+                    fwdp[static_cast<std::size_t>(i) + 1] = fwdp[static_cast<std::size_t>(i)];
                     continue;
                 }
                 d.col_items = (1 << fs.log2n1) / rf::column_tile(fs);
+                d.fwd_items = (1 << fs.log2n1) / rf::column_tile<rf::kFwdTileLog2>(fs);
+                fwdp[static_cast<std::size_t>(i) + 1] = fwdp[static_cast<std::size_t>(i)] + d.fwd_items;
                 d.row_items = (1 << fs.log2n2) / rf::lds_batch_per_block(fs.log2n1);
                 colp[static_cast<std::size_t>(i) + 1] = colp[static_cast<std::size_t>(i)] + d.col_items;
                 const int kind = fs.log2n1 == rf::k4096Log2 ? 0 : 1;
@@ -364,6 +375,7 @@ rf_status rf_gcc_batch_create(int count, const int* num_a, const int* num_b,
             }
             h->col_total = colp.back();
             h->col_prefix_host = colp;
+            h->fwd_prefix_host = fwdp;
             for (int kind = 0; kind < 2; ++kind) {
                 h->row_prefix_host[kind] = row_prefix[kind];
                 h->row_pairs_host[kind] = row_pairs[kind];
@@ -426,6 +438,13 @@ rf_status rf_gcc_batch_create(int count, const int* num_a, const int* num_b,
                 }
             }
             upload_map(col_map, &h->col_map);
+            std::vector<int2> fwd_map;
+            for (int i = 0; i < count; ++i) {
+                for (int c = 0; c < h->host[static_cast<std::size_t>(i)].fwd_items; ++c) {
+                    fwd_map.push_back(make_int2(i, c));
+                }
+            }
+            upload_map(fwd_map, &h->fwd_map);
             rf_gcc_batch::RowList* lists[2] = {&h->rows4096, &h->rows_generic};
             for (int kind = 0; kind < 2; ++kind) {
                 rf_gcc_batch::RowList& list = *lists[kind];

@@ -25,7 +25,8 @@ struct PairDesc {
     int log2L;
     int log2n1;
     int log2n2;
-    int col_items;    // n1 / C
+    int col_items;    // n1 / C: inverse column-pass items (4096-value tiles)
+    int fwd_items;    // forward column-pass items (8192-value tiles)
     int row_items;    // n2 / F
     int fused;        // 1: L = 2^15, one workgroup per pair (gcc_pair32k.hpp); no work buffer
 };
@@ -34,13 +35,12 @@ namespace {
 
 constexpr int kItemThreads = 512;
 
-// Column items of the batched passes are tiles of 4096 values (C = 4096 / n2 columns, or all
-// n1 if fewer); every four-step pair has an even number of them (L / 4096 >= 2). The forward
-// column pass takes two consecutive items of a pair at once, a tile of 8192 values on 1024
-// lanes, so that it reads and writes row segments twice as long (256 B per row at L = 2^20,
-// 128 B at 2^21): measured 1.43 -> 1.04 ms per C4 batch. Its 80 KiB of LDS allow two
-// workgroups per CU. The inverse pass, whose peak reduction ends every item with a
-// workgroup-wide barrier, measured slower with the wide tile (0.59 -> 0.69 ms) and keeps 4096.
+// The forward column pass works on tiles of 8192 values (C = 8192 / n2 columns, or all n1 if
+// fewer) on 1024 lanes, twice the inverse pass's 4096, so that it reads and writes row segments
+// twice as long (256 B per row at L = 2^20, 128 B at 2^21): measured 1.43 -> 1.04 ms per C4
+// batch. Its 80 KiB of LDS allow two workgroups per CU. The inverse pass, whose peak reduction
+// ends every item with a workgroup-wide barrier, measured slower with the wide tile
+// (0.59 -> 0.69 ms) and keeps 4096. Each pass has its own item map.
 constexpr int kFwdTileLog2 = 13;
 constexpr int kFwdThreads = 1 << (kFwdTileLog2 - 3);
 constexpr int kFwdWorkgroupsPerCu = 2;
@@ -68,23 +68,21 @@ __global__ void batch_init_kernel(R* results, int count)
     }
 }
 
-// Column items begin .. begin + total - 1 (total even, begin even), two at a time, for a and
-// for b.
+// Forward column items begin .. begin + total - 1 of the forward map, for a and for b.
 __global__ __launch_bounds__(kFwdThreads) void batch_column_fwd_kernel(
     const PairDesc* __restrict__ pairs, const int2* __restrict__ map, int begin, int total,
     const float2* __restrict__ table)
 {
     extern __shared__ float2 lds[];
-    const int wide = total >> 1;
-    xcd_major_items(2 * wide, [&](int item) {
-        const int which = item >= wide ? 1 : 0;
-        const int2 m = map[begin + 2 * (item - which * wide)];  // the first of the two
+    xcd_major_items(2 * total, [&](int item) {
+        const int which = item >= total ? 1 : 0;
+        const int2 m = map[begin + item - which * total];
         const PairDesc d = pairs[m.x];
         const FourStep fs = shape_of(d);
         // Indices below L <= 2^24 and lengths below 2^24: 32-bit arithmetic throughout.
         const int L = 1 << d.log2L;
         const int log2n1 = d.log2n1;
-        const long long c0 = static_cast<long long>(m.y >> 1) * column_tile<kFwdTileLog2>(fs);
+        const long long c0 = static_cast<long long>(m.y) * column_tile<kFwdTileLog2>(fs);
         float2* dst = d.work + (which ? L : 0);
         auto store = [=](long long j1, int k2, float2 v) {
             gstore(dst, static_cast<int>(j1) + (k2 << log2n1), v);
@@ -277,8 +275,7 @@ std::size_t column_lds_bytes()
     return tile_lds_bytes<kTwiddleLog2>();
 }
 
-// Host check that a pair's column tiles fit the LDS the batched column kernels allocate and
-// that the pair has an even number of inverse-pass items (the forward pass pairs them up).
+// Host check that a pair's column tiles fit the LDS the batched column kernels allocate.
 void check_column_tile(const FourStep& fs)
 {
     const std::size_t slots = column_slots(1 << fs.log2n2) * sizeof(float2);
@@ -287,10 +284,6 @@ void check_column_tile(const FourStep& fs)
     if (need > column_lds_bytes() || need_fwd > column_fwd_lds_bytes()) {
         fail(RF_LIMIT, "column tile of " + std::to_string(need_fwd) + " bytes exceeds the "
                            + std::to_string(column_fwd_lds_bytes()) + "-byte LDS allocation");
-    }
-    if (((1 << fs.log2n1) / column_tile(fs)) % 2 != 0
-        || column_tile<kFwdTileLog2>(fs) != 2 * column_tile(fs)) {
-        fail(RF_LIMIT, "four-step shape without an even number of column tiles");
     }
 }
 

@@ -114,6 +114,23 @@ inline int ilog2(long long v)
     return l;
 }
 
+// Persistent walk over `total` items in XCD-major order: workgroups are dispatched round-robin
+// over the 8 XCDs (workgroup b on XCD b % 8), so the items are cut into 8 contiguous ranges and
+// range x is walked in order by XCD x's workgroups; neighbouring items, which share cache lines
+// (column tiles narrower than a line) or twiddle rows, then run together behind one L2. Needs a
+// grid that is a multiple of 8; every item is visited exactly once whatever the XCD count.
+template <class Body>
+__device__ __forceinline__ void xcd_major_items(int total, Body body)
+{
+    const int per = (total + 7) >> 3;
+    const int lo = (blockIdx.x & 7) * per;
+    const int hi = lo + per < total ? lo + per : total;
+    const int step = gridDim.x >> 3;
+    for (int item = lo + static_cast<int>(blockIdx.x >> 3); item < hi; item += step) {
+        body(item);
+    }
+}
+
 // Device-side complex helpers on float2 (x = re, y = im).
 __device__ __forceinline__ float2 cadd(float2 a, float2 b)
 {

@@ -182,6 +182,21 @@ rf_status rf_fir_plan_run(const rf_fir_plan* plan, const rf_cf32* x, rf_cf32* y,
                           int num_samples, rf_stream stream);
 rf_status rf_fir_plan_destroy(rf_fir_plan* plan);
 
+/* Persistent batched FIR: any number of captures of uneven length sharing one filter, all
+ * filtered by ONE launch of a persistent kernel that walks {capture, segment} work items
+ * (XCD-major). No reference counterpart beyond fir_batched_device (src/fused.cu:92-109), which
+ * takes at most 8 channels as grid.y. impl is RF_FIR_FFT_OLA (overlap-save segments for
+ * 66..2049 taps, the register kernel's blocks otherwise) or RF_FIR_REGISTER; each capture's
+ * output is bit-identical to rf_fir with that impl. create copies the DEVICE taps, makes the
+ * tap spectrum and the item map (synchronises `stream`); bind takes HOST arrays of `count`
+ * device pointers (synchronous); run only enqueues one launch (no allocation: capturable). */
+typedef struct rf_fir_batch rf_fir_batch;
+rf_status rf_fir_batch_create(int impl, const float* taps, int num_taps, int count,
+                              const int* num_samples, rf_stream stream, rf_fir_batch** batch);
+rf_status rf_fir_batch_bind(rf_fir_batch* batch, const rf_cf32* const* x, rf_cf32* const* y);
+rf_status rf_fir_batch_run(const rf_fir_batch* batch, rf_stream stream);
+rf_status rf_fir_batch_destroy(rf_fir_batch* batch);
+
 /* ---------------------------------------------------------------------------------------
  * Magnitude: mag[n] = |x[n]| (src/magnitude.cu:17-39, tools/reference.py:98-100).
  * ------------------------------------------------------------------------------------- */

@@ -30,7 +30,7 @@ def delayed_pair(seed: int, na: int, nb: int, delay: int, noise: float):
     return a.astype(np.complex64), b.astype(np.complex64)
 
 
-LENGTHS = [(2100, 2000, 13), (4096, 4096, -77), (5000, 3000, 150), (9000, 9001, 0),
+LENGTHS = [(1500, 1200, 17), (2100, 2000, 13), (4096, 4096, -77), (5000, 3000, 150), (9000, 9001, 0),
            (20000, 16000, -200), (33000, 33000, 199), (70000, 65536, 5), (131072, 131072, -1),
            (300000, 200000, 123), (1 << 20, 1 << 20, -150), (3000, 70000, 40), (50000, 2500, -33),
            # FFT length 2^15 (the single-workgroup path): both channels at most 2^14, one channel
@@ -38,6 +38,8 @@ LENGTHS = [(2100, 2000, 13), (4096, 4096, -77), (5000, 3000, 150), (9000, 9001, 
            # 12000 + 4385 - 1 = 2^14 lags stays on the four-step path
            (1 << 14, 1 << 14, 77), (20000, 5000, -120), (3000, 29000, 33), (12000, 4386, 5),
            (32000, 769, 10), (12000, 4385, -9)]
+# (1500, 1200): FFT length 4096, whose forward column tile (512 columns of 8) fills half of
+# the forward pass's 1024 lanes
 
 
 @pytest.fixture(scope="module")
