@@ -94,6 +94,8 @@ int lds_per_cu();
 int cu_count();
 // Enqueues the library's streaming device-to-device copy.
 void copy_device(const void* src, void* dst, size_t bytes, hipStream_t stream);
+// Sets `bytes` bytes at dst to `value` with a kernel (graph-capturable).
+void fill_device(void* dst, int value, size_t bytes, hipStream_t stream);
 
 inline long long ceil_div(long long a, long long b)
 {

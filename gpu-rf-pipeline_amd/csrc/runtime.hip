@@ -117,7 +117,48 @@ __global__ __launch_bounds__(256) void copy_kernel(const unsigned char* __restri
     }
 }
 
+// Byte fill for rf_memset: a kernel rather than hipMemsetAsync, so a stream capture records
+// an ordinary kernel node (memset nodes have misbehaved on graph replays on this platform).
+// 16-byte stores for the aligned body, bytes for the unaligned head and tail.
+__global__ __launch_bounds__(256) void fill_kernel(unsigned char* __restrict__ dst,
+                                                   unsigned char value, size_t head, size_t n16,
+                                                   size_t bytes)
+{
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    const size_t first = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const unsigned v4 = 0x01010101u * value;
+    const uint4 word = make_uint4(v4, v4, v4, v4);
+    uint4* body = reinterpret_cast<uint4*>(dst + head);
+    for (size_t i = first; i < n16; i += stride) {
+        body[i] = word;
+    }
+    const size_t tail = head + n16 * 16;
+    for (size_t i = first; i < head; i += stride) {
+        dst[i] = value;
+    }
+    for (size_t i = tail + first; i < bytes; i += stride) {
+        dst[i] = value;
+    }
+}
+
 }  // namespace
+
+void fill_device(void* dst, int value, size_t bytes, hipStream_t stream)
+{
+    if (bytes == 0) {
+        return;
+    }
+    unsigned char* d = static_cast<unsigned char*>(dst);
+    size_t head = (16 - (reinterpret_cast<uintptr_t>(d) & 15)) & 15;
+    head = head < bytes ? head : bytes;
+    const size_t n16 = (bytes - head) / 16;
+    const long long cap = 8LL * cu_count();
+    long long grid = ceil_div(static_cast<long long>(n16 + 1), 256);
+    grid = grid < cap ? grid : cap;
+    fill_kernel<<<static_cast<unsigned>(grid), 256, 0, stream>>>(
+        d, static_cast<unsigned char>(value), head, n16, bytes);
+    check_launch("fill_kernel");
+}
 
 void copy_device(const void* src, void* dst, size_t bytes, hipStream_t stream)
 {
@@ -270,7 +311,8 @@ rf_status rf_memset(void* dst, int value, size_t bytes, rf_stream stream)
 {
     return guarded([&] {
         if (bytes == 0) return;
-        RF_HIP_CHECK(hipMemsetAsync(dst, value, bytes, rf::as_stream(stream)));
+        if (dst == nullptr) rf::fail(RF_INVALID_ARG, "rf_memset: NULL");
+        rf::fill_device(dst, value, bytes, rf::as_stream(stream));
     });
 }
 

@@ -15,7 +15,7 @@ namespace {
 // A fixed set of device events, destroyed with the object.
 class EventPool {
 public:
-    explicit EventPool(std::size_t count) : events_(count, nullptr)
+    EventPool(std::size_t count, rf_stream stream) : events_(count, nullptr), stream_(stream)
     {
         for (void*& e : events_) {
             RF_CHECK(rf_event_create(&e));
@@ -32,7 +32,7 @@ public:
     EventPool(const EventPool&) = delete;
     EventPool& operator=(const EventPool&) = delete;
 
-    void mark(std::size_t i) { RF_CHECK(rf_event_record(events_[i], nullptr)); }
+    void mark(std::size_t i) { RF_CHECK(rf_event_record(events_[i], stream_)); }
     void wait(std::size_t i) { RF_CHECK(rf_event_synchronize(events_[i])); }
     double between(std::size_t from, std::size_t to) const
     {
@@ -43,6 +43,7 @@ public:
 
 private:
     std::vector<void*> events_;
+    rf_stream stream_;
 };
 
 // Welford's online mean and sum of squared deviations.
@@ -81,7 +82,7 @@ void Bench::run(const std::function<void()>& upload, const std::function<void()>
     }
     const std::size_t runs = static_cast<std::size_t>(config_.runs);
     // marks of run r: 4r before upload, 4r+1 after it, 4r+2 after compute, 4r+3 after download
-    EventPool marks(4 * runs);
+    EventPool marks(4 * runs, config_.stream);
     for (std::size_t r = 0; r < runs; ++r) {
         marks.mark(4 * r);
         upload();

@@ -1,14 +1,18 @@
 #pragma once
 
 // Timing harness with the reference's shape (src/bench.hpp:1-46): `warmup` untimed passes,
-// then `runs` passes with device events around upload, compute and download on the default
-// stream; one-time setup happens before run().
+// then `runs` passes with device events around upload, compute and download on `stream` (the
+// default stream unless the caller enqueues its work elsewhere); one-time setup happens before
+// run().
+
+#include "rfhip.h"
 
 #include <functional>
 
 struct BenchConfig {
     int warmup = 0;
     int runs = 1;
+    rf_stream stream = nullptr;
 };
 
 struct PhaseTime {

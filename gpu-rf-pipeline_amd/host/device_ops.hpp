@@ -4,6 +4,8 @@
 // inline wrappers over the C ABI. Each one only enqueues work: no allocation, no host
 // transfers, no synchronisation. What changed, and why:
 //   - cufftHandle becomes rf_fft_plan* and cudaStream_t becomes rf_stream (a hipStream_t);
+//     every *_device call takes an optional trailing stream (the default stream if omitted),
+//     so a caller can queue a whole pipeline on one stream and capture it as a hipGraph;
 //   - taps are read from the caller's pointer on every launch: there is no constant bank, so
 //     fir_load_const_taps only checks the reference's 8192-tap limit and fir_launch uses the
 //     `taps` it is given for every variant;
@@ -72,52 +74,55 @@ private:
 };
 
 inline void fir_device(FirImpl impl, const cf32* x, const float* taps, cf32* y, int num_samples,
-                       int num_taps)
+                       int num_taps, rf_stream stream = nullptr)
 {
-    fir_launch(impl, x, taps, y, num_samples, num_taps, nullptr);
+    fir_launch(impl, x, taps, y, num_samples, num_taps, stream);
 }
 
 inline void fir_naive_device(const cf32* x, const float* taps, cf32* y, int num_samples,
-                             int num_taps)
+                             int num_taps, rf_stream stream = nullptr)
 {
-    fir_launch(FirImpl::Naive, x, taps, y, num_samples, num_taps, nullptr);
+    fir_launch(FirImpl::Naive, x, taps, y, num_samples, num_taps, stream);
 }
 
 constexpr int kMaxFirBatch = RF_MAX_FIR_BATCH;
 
 inline void fir_batched_device(FirImpl impl, const cf32* const* x, cf32* const* y,
                                const int* num_samples, int count, const float* taps,
-                               int num_taps)
+                               int num_taps, rf_stream stream = nullptr)
 {
     RF_CHECK(rf_fir_batched(static_cast<int>(impl), reinterpret_cast<const rf_cf32* const*>(x),
                             reinterpret_cast<rf_cf32* const*>(y), num_samples, count, taps,
-                            num_taps, nullptr));
+                            num_taps, stream));
 }
 
 inline void fir_naive_batched_device(const cf32* const* x, cf32* const* y,
                                      const int* num_samples, int count, const float* taps,
-                                     int num_taps)
+                                     int num_taps, rf_stream stream = nullptr)
 {
-    fir_batched_device(FirImpl::Naive, x, y, num_samples, count, taps, num_taps);
+    fir_batched_device(FirImpl::Naive, x, y, num_samples, count, taps, num_taps, stream);
 }
 
 // --- magnitude --------------------------------------------------------------------------------
 
-inline void magnitude_device(const cf32* x, float* mag, int num_samples)
+inline void magnitude_device(const cf32* x, float* mag, int num_samples,
+                             rf_stream stream = nullptr)
 {
-    RF_CHECK(rf_magnitude(as_rf(x), mag, num_samples, nullptr));
+    RF_CHECK(rf_magnitude(as_rf(x), mag, num_samples, stream));
 }
 
 // --- cross-correlation ------------------------------------------------------------------------
 
-inline void xcorr_naive_device(const cf32* a, const cf32* b, cf32* r, int num_a, int num_b)
+inline void xcorr_naive_device(const cf32* a, const cf32* b, cf32* r, int num_a, int num_b,
+                               rf_stream stream = nullptr)
 {
-    RF_CHECK(rf_xcorr_direct(RF_XCORR_NAIVE, as_rf(a), as_rf(b), as_rf(r), num_a, num_b, nullptr));
+    RF_CHECK(rf_xcorr_direct(RF_XCORR_NAIVE, as_rf(a), as_rf(b), as_rf(r), num_a, num_b, stream));
 }
 
-inline void xcorr_tiled_device(const cf32* a, const cf32* b, cf32* r, int num_a, int num_b)
+inline void xcorr_tiled_device(const cf32* a, const cf32* b, cf32* r, int num_a, int num_b,
+                               rf_stream stream = nullptr)
 {
-    RF_CHECK(rf_xcorr_direct(RF_XCORR_TILED, as_rf(a), as_rf(b), as_rf(r), num_a, num_b, nullptr));
+    RF_CHECK(rf_xcorr_direct(RF_XCORR_TILED, as_rf(a), as_rf(b), as_rf(r), num_a, num_b, stream));
 }
 
 // FFT cross-correlation for fixed lengths; construction allocates, run() enqueues.
@@ -155,13 +160,23 @@ inline int channelize_frames(int num_samples, int channels)
 
 // `plan`: channels points x channelize_frames() frames, or nullptr for the fused kernel.
 inline void channelize_device(const rf_fft_plan* plan, const cf32* x, const float* taps, cf32* out,
-                              int num_samples, int channels, int num_taps)
+                              int num_samples, int channels, int num_taps,
+                              rf_stream stream = nullptr)
 {
     RF_CHECK(rf_channelize(plan, as_rf(x), taps, as_rf(out), num_samples, channels, num_taps,
-                           nullptr));
+                           stream));
 }
 
-inline void spectrum_device(const rf_fft_plan* plan, const cf32* in, cf32* out)
+inline void spectrum_device(const rf_fft_plan* plan, const cf32* in, cf32* out,
+                            rf_stream stream = nullptr)
 {
-    RF_CHECK(rf_spectrum(plan, as_rf(in), as_rf(out), nullptr));
+    RF_CHECK(rf_spectrum(plan, as_rf(in), as_rf(out), stream));
+}
+
+// |spectrum_device| in one pass (rf_spectrum_magnitude; no reference counterpart): `mag` holds
+// nfft * batch floats and the complex spectrum is never stored.
+inline void spectrum_magnitude_device(const rf_fft_plan* plan, const cf32* in, float* mag,
+                                      rf_stream stream = nullptr)
+{
+    RF_CHECK(rf_spectrum_magnitude(plan, as_rf(in), mag, stream));
 }

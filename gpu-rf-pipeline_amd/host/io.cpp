@@ -54,26 +54,6 @@ std::string read_text(const std::string& path)
     return std::string(b.begin(), b.end());
 }
 
-void write_bytes(const std::string& path, const void* data, std::size_t bytes)
-{
-    const fs::path dir = fs::path(path).parent_path();
-    std::error_code ec;
-    if (!dir.empty() && !fs::exists(dir, ec)) {
-        fs::create_directories(dir, ec);
-        if (ec) {
-            file_error(dir.string(), "cannot create directory (" + ec.message() + ")");
-        }
-    }
-    std::FILE* f = std::fopen(path.c_str(), "wb");
-    if (f == nullptr) {
-        file_error(path, std::string("cannot open for writing (") + std::strerror(errno) + ")");
-    }
-    const std::size_t put = bytes == 0 ? 0 : std::fwrite(data, 1, bytes, f);
-    const bool failed = put != bytes || std::fclose(f) != 0;
-    if (failed) {
-        file_error(path, "write failed");
-    }
-}
 
 // Metadata documents are small JSON files. JsonScalars walks one with a recursive-descent
 // scanner and keeps every `"key": scalar` pair it meets, at any depth (SigMF puts its fields
@@ -305,6 +285,27 @@ double sample_rate(const std::string& path)
     return rate == nullptr ? 0.0 : std::strtod(rate->c_str(), nullptr);
 }
 
+void write_raw(const std::string& path, const void* data, std::size_t bytes)
+{
+    const fs::path dir = fs::path(path).parent_path();
+    std::error_code ec;
+    if (!dir.empty() && !fs::exists(dir, ec)) {
+        fs::create_directories(dir, ec);
+        if (ec) {
+            file_error(dir.string(), "cannot create directory (" + ec.message() + ")");
+        }
+    }
+    std::FILE* f = std::fopen(path.c_str(), "wb");
+    if (f == nullptr) {
+        file_error(path, std::string("cannot open for writing (") + std::strerror(errno) + ")");
+    }
+    const std::size_t put = bytes == 0 ? 0 : std::fwrite(data, 1, bytes, f);
+    const bool failed = put != bytes || std::fclose(f) != 0;
+    if (failed) {
+        file_error(path, "write failed");
+    }
+}
+
 std::vector<cf32> read_cf32(const std::string& path)
 {
     return is_sigmf(path) ? read_recording(path) : decode(read_bytes(path), kEncodings[0], path);
@@ -326,10 +327,10 @@ std::vector<float> read_f32(const std::string& path)
 
 void write_cf32(const std::string& path, const std::vector<cf32>& samples)
 {
-    write_bytes(path, samples.data(), samples.size() * sizeof(cf32));
+    write_raw(path, samples.data(), samples.size() * sizeof(cf32));
 }
 
 void write_f32(const std::string& path, const std::vector<float>& values)
 {
-    write_bytes(path, values.data(), values.size() * sizeof(float));
+    write_raw(path, values.data(), values.size() * sizeof(float));
 }

@@ -14,6 +14,8 @@ std::vector<cf32> read_cf32(const std::string& path);
 void write_cf32(const std::string& path, const std::vector<cf32>& samples);
 std::vector<float> read_f32(const std::string& path);
 void write_f32(const std::string& path, const std::vector<float>& values);
+// `bytes` bytes to `path`, creating its parent directories.
+void write_raw(const std::string& path, const void* data, std::size_t bytes);
 
 // True for NAME.sigmf-meta or NAME.sigmf-data.
 bool is_sigmf(const std::string& path);

@@ -75,6 +75,7 @@ rf_status rf_host_alloc_mapped(void** host, void** device, size_t bytes);
 rf_status rf_memcpy_h2d(void* dst, const void* src, size_t bytes, rf_stream stream);
 rf_status rf_memcpy_d2h(void* dst, const void* src, size_t bytes, rf_stream stream);
 rf_status rf_memcpy_d2d(void* dst, const void* src, size_t bytes, rf_stream stream);
+/* A fill kernel (not hipMemsetAsync), so it can be recorded in a graph. */
 rf_status rf_memset(void* dst, int value, size_t bytes, rf_stream stream);
 rf_status rf_stream_create(rf_stream* stream);
 rf_status rf_stream_destroy(rf_stream stream);
