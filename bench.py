@@ -7,7 +7,7 @@ phases, seeded), filtered by a 1025-tap Kaiser (beta 8) low-pass with cutoff 0.1
 gain. One step is one rf_fir call over the whole capture with inputs already in HBM.
 
   python bench.py [--steps K] [--warmup W] [--impl fft_ola] [--workload fir_long|fir_plain|
-                   channelize] [--gpus 1]
+                   channelize|gcc_batch|delay|copy|sigflow] [--gpus 1]
 
 Input/output buffers rotate over enough sets (at least 320 MiB: three for the long FIR, 20 for
 the 2^20-sample FIR) that the 256 MB Infinity Cache cannot hold a step's input from the previous
@@ -678,6 +678,77 @@ def bench_copy(rf, args) -> dict:
     }
 
 
+def bench_sigflow(rf, args) -> dict:
+    """The SigFlow two_channel pipeline (two captures low-passed by one 63-tap filter, a
+    1024-point magnitude spectrum of the first, and their cross-correlation) at 2^18 samples per
+    channel, as sigflowc compiles it: the example program built for the register FIR and the FFT
+    cross-correlation, with fusion (fir_batched + spectrum-magnitude) and without, run eagerly and
+    as a replayed hipGraph (--no-variants: fused and eager only). value = samples of both
+    channels per second of the faster fused kernel phase. The reference quotes 0.303 ms kernel phase and 1.82 ms end to end for its fastest
+    variant of the same program and size (its docs/PROFILING_LOG.md:676-681)."""
+    import subprocess
+    import tempfile
+
+    n = 1 << 18
+    rng = np.random.Generator(np.random.Philox(0x5F))
+    src = (rng.standard_normal(n + 64) + 1j * rng.standard_normal(n + 64)) * np.sqrt(0.5)
+    noise = (rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))) * 0.3 * np.sqrt(0.5)
+    ch0 = (src[64:] + noise[0]).astype(np.complex64)
+    ch1 = (src[64 - 37:n + 64 - 37] + noise[1]).astype(np.complex64)  # ch0's source 37 later
+    programs = ROOT / "gpu-rf-pipeline_amd" / "sigflow" / "_programs"
+    taps = ROOT / "tests" / "golden" / "fixtures" / "lowpass.f32"
+    runs = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        work = Path(args.sigflow_dir) if args.sigflow_dir else Path(tmp)
+        (work / "fixtures").mkdir(parents=True, exist_ok=True)
+        ch0.tofile(work / "fixtures" / "ch0.cf32")
+        ch1.tofile(work / "fixtures" / "ch1.cf32")
+        (work / "fixtures" / "lowpass.f32").write_bytes(taps.read_bytes())
+        configs = [("fused_eager", "register_fft", []),
+                   ("fused_graph", "register_fft", ["--graph"]),
+                   ("unfused_eager", "register_no_fuse", []),
+                   ("unfused_graph", "register_no_fuse", ["--graph"])]
+        for label, variant, extra in configs[:1] if args.no_variants else configs:
+            exe = programs / f"two_channel.{variant}"
+            if not exe.exists():
+                raise RuntimeError(f"{exe} is not built (gpu-rf-pipeline_amd/sigflow/"
+                                   "build_examples.py)")
+            proc = subprocess.run([str(exe), "--bench", str(args.steps), "--warmup",
+                                   str(args.warmup), *extra], cwd=work, capture_output=True,
+                                  text=True, timeout=300, check=True)
+            report = json.loads(proc.stdout.strip().splitlines()[-1])
+            runs[label] = {k: report[k] for k in ("kernel_ms", "kernel_std_ms", "h2d_ms",
+                                                  "d2h_ms", "calls", "calls_by_kind",
+                                                  "allocated_bytes")}
+            runs[label]["end_to_end_ms"] = round(report["h2d_ms"] + report["kernel_ms"]
+                                                 + report["d2h_ms"], 4)
+            if label == "fused_eager":
+                spec = np.fromfile(work / "out" / "spectrum.f32", dtype=np.float32)
+                corr = np.fromfile(work / "out" / "xcorr.cf32", dtype=np.complex64)
+    oracle = load_oracle()
+    h = np.fromfile(taps, dtype=np.float32)
+    want = np.abs(np.fft.fft(oracle.fir(ch0, h).reshape(-1, 1024), axis=1)).ravel()
+    err = np.abs(spec.astype(np.float64) - want)
+    check = {"spectrum_max_abs_err": float(err.max()),
+             "spectrum_within_rtol_1e-5_atol_1e-3": bool(np.all(err <= 1e-3 + 1e-5 * want)),
+             "xcorr_peak_lag": int(np.argmax(np.abs(corr))) - (n - 1), "true_delay": 37}
+    best = min((r for label, r in runs.items() if label.startswith("fused")),
+               key=lambda r: r["kernel_ms"])
+    return {
+        "metric": "SigFlow two_channel pipeline: Msamples/s of kernel phase (both channels)",
+        "value": round(2 * n / (best["kernel_ms"] * 1e-3) / 1e6, 1), "unit": "Msamples/s",
+        "ms_per_step": best["kernel_ms"], "dtype": "f32",
+        "config": {"workload": "two_channel.sf: 2 x 2^18 complex64 samples, 63-tap FIR each "
+                               "(one batched call), 1024-point |FFT| of one, FFT xcorr of both",
+                   "samples_per_channel": n, "parallelism": "single-gpu"},
+        "runs": runs,
+        "reference_published": {"kernel_ms": 0.303, "end_to_end_ms": 1.82,
+                                "gpu": "RTX 5080 Laptop, --fir-impl=tiled_const --xcorr-impl=fft"},
+        "check": check,
+        "cpu_baseline": None,
+    }
+
+
 def traffic_from_profiles(name: str):
     """HBM bytes per launch from a committed rocprofv3 counter summary, if one exists."""
     path = ROOT / "profiles" / f"{name}_traffic.json"
@@ -696,8 +767,11 @@ def main() -> int:
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--impl", default="fft_ola")
     p.add_argument("--workload", default="fir_long",
-                   choices=["fir_long", "fir_plain", "channelize", "gcc_batch", "delay", "copy"])
+                   choices=["fir_long", "fir_plain", "channelize", "gcc_batch", "delay", "copy",
+                            "sigflow"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--sigflow-dir", default=None,
+                   help="sigflow: write the 2^18-sample inputs and outputs here and keep them")
     p.add_argument("--no-variants", action="store_true",
                    help="FIR: time only --impl, not the other variants")
     args = p.parse_args()
@@ -714,6 +788,8 @@ def main() -> int:
         out = bench_delay(rf, args)
     elif args.workload == "gcc_batch":
         out = bench_gcc_batch(rf, args)
+    elif args.workload == "sigflow":
+        out = bench_sigflow(rf, args)
     else:
         out = bench_fir(rf, args, args.workload == "fir_long")
     line = {"metric": out.pop("metric"), "value": out.pop("value"), "unit": out.pop("unit"),
