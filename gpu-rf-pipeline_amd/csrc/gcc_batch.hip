@@ -74,7 +74,10 @@ struct rf_gcc_batch {
     float2* scratch = nullptr;  // pair32k_scratch_values()
     std::map<int, float2*> tables;
     std::size_t bytes = 0;
-    int grid_mult = 2;          // persistent workgroups per CU
+    // workgroups per CU of the row and inverse passes (more than fit at once: the surplus
+    // queues behind the resident ones, which evens out the uneven items); rf_gcc_batch_tune
+    // picks from 2..8, and 8 was the fastest on the 256 uneven pairs it was tuned on
+    int grid_mult = 8;
     hipStream_t capture = nullptr;
     hipGraphExec_t graph = nullptr;
     // When the batch has pairs of both kinds, the single-workgroup pairs run on a side stream
