@@ -7,7 +7,7 @@ phases, seeded), filtered by a 1025-tap Kaiser (beta 8) low-pass with cutoff 0.1
 gain. One step is one rf_fir call over the whole capture with inputs already in HBM.
 
   python bench.py [--steps K] [--warmup W] [--impl fft_ola] [--workload fir_long|fir_plain|
-                   channelize|gcc_batch|delay|copy|sigflow] [--gpus 1]
+                   channelize|gcc_batch|delay|copy|sigflow|stream] [--gpus 1]
 
 Input/output buffers rotate over enough sets (at least 320 MiB: three for the long FIR, 20 for
 the 2^20-sample FIR) that the 256 MB Infinity Cache cannot hold a step's input from the previous
@@ -812,6 +812,55 @@ def bench_sigflow(rf, args) -> dict:
     }
 
 
+def bench_stream(rf, args) -> dict:
+    """The streaming FIR (rfgpu stream: chunks uploaded, filtered and downloaded on three
+    streams over a ring of page-locked slots) on 2^24 complex64 samples, PCIe included: chunk
+    2^20, depth 4, each memory mode, 63 taps (register kernel) and 1025 taps (register and
+    fft_ola). value = the 1025-tap fft_ola rate in copy mode, the mode with no help from the
+    caller. The reference's best streaming times for 2^24 samples are 15.8 ms (63 taps) and
+    21.6 ms (1025 taps) (its README.md:77-79, docs/PROFILING_LOG.md:464-471)."""
+    import subprocess
+    import tempfile
+
+    n = 1 << 24
+    x, h1025 = fir_long_input(n)
+    h63 = hamming_lowpass(63, 0.2)
+    rfgpu = ROOT / "gpu-rf-pipeline_amd" / "rfgpu"
+    runs = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        work = Path(tmp)
+        x.tofile(work / "x.cf32")
+        h63.tofile(work / "h63.f32")
+        h1025.tofile(work / "h1025.f32")
+        for taps, impl in [(63, "register"), (1025, "register"), (1025, "fft_ola")]:
+            for memory in ["copy", "registered", "zero-copy"]:
+                proc = subprocess.run(
+                    [str(rfgpu), "stream", "--in", str(work / "x.cf32"), "--taps",
+                     str(work / f"h{taps}.f32"), "--out", str(work / "y.cf32"), "--impl", impl,
+                     "--memory", memory, "--chunk", str(1 << 20), "--depth", "4", "--rate", "1e6",
+                     "--bench", str(max(args.steps, 10)), "--warmup", str(args.warmup)],
+                    capture_output=True, text=True, timeout=300, check=True)
+                report = json.loads(proc.stdout.strip().splitlines()[-1])
+                runs[f"{taps}taps_{impl}_{memory}"] = {
+                    "ms": report["total_ms"]["mean"], "std_ms": report["total_ms"]["std"],
+                    "msamples_per_s": report["msps"]}
+        y = np.fromfile(work / "y.cf32", dtype=np.complex64)  # the last run: 1025 taps fft_ola
+    err = sampled_fir_error(x, h1025, y)
+    main = runs["1025taps_fft_ola_copy"]
+    return {
+        "metric": "streaming FIR Msamples/s, host memory to host memory (PCIe included)",
+        "value": main["msamples_per_s"], "unit": "Msamples/s", "ms_per_step": main["ms"],
+        "dtype": "f32",
+        "config": {"workload": "rfgpu stream: 2^24 complex64 samples, chunk 2^20, depth 4",
+                   "samples": n, "parallelism": "single-gpu"},
+        "runs": runs,
+        "reference_published_ms": {"63taps": 15.8, "1025taps": 21.6},
+        "accuracy_last_run": {"max_abs_err": err[0], "max_rel_err": err[1],
+                              "checked_outputs": "sampled, against float64 dot products"},
+        "cpu_baseline": None,
+    }
+
+
 def traffic_from_profiles(name: str):
     """HBM bytes per launch from a committed rocprofv3 counter summary, if one exists."""
     path = ROOT / "profiles" / f"{name}_traffic.json"
@@ -831,7 +880,7 @@ def main() -> int:
     p.add_argument("--impl", default="fft_ola")
     p.add_argument("--workload", default="fir_long",
                    choices=["fir_long", "fir_plain", "channelize", "gcc_batch", "delay", "copy",
-                            "sigflow"])
+                            "sigflow", "stream"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--sigflow-dir", default=None,
                    help="sigflow: write the 2^18-sample inputs and outputs here and keep them")
@@ -853,6 +902,8 @@ def main() -> int:
         out = bench_gcc_batch(rf, args)
     elif args.workload == "sigflow":
         out = bench_sigflow(rf, args)
+    elif args.workload == "stream":
+        out = bench_stream(rf, args)
     else:
         out = bench_fir(rf, args, args.workload == "fir_long")
     line = {"metric": out.pop("metric"), "value": out.pop("value"), "unit": out.pop("unit"),

@@ -15,6 +15,12 @@
 // direct variants produce exactly the bits of one call over the whole capture.
 //
 // The filter is prepared once (rf_fir_plan: device taps, fft_ola's spectrum), before the clock.
+//
+// Copy mode's host copies (pageable capture -> pinned staging, pinned staging -> pageable
+// result) run on a small pool of host threads: one thread copies about 35 GB/s, a PCIe 5 DMA
+// engine moves 50+ GB/s per direction, so a single-threaded copy would set the pace. Staging is
+// touched once at setup: page-locked memory is not faulted in on the CPU side until first use,
+// and those faults would otherwise land in the first chunks.
 
 #include "stream.hpp"
 
@@ -22,13 +28,111 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
+
+// Host threads that split one memcpy between them; copy() returns when every part is done.
+class CopyPool {
+public:
+    explicit CopyPool(int threads)
+    {
+        for (int t = 0; t < threads; ++t) {
+            workers_.emplace_back([this, t] { work(t); });
+        }
+    }
+    ~CopyPool()
+    {
+        {
+            std::lock_guard<std::mutex> lock(mutex_);
+            stop_ = true;
+        }
+        wake_.notify_all();
+        for (std::thread& w : workers_) {
+            w.join();
+        }
+    }
+    CopyPool(const CopyPool&) = delete;
+    CopyPool& operator=(const CopyPool&) = delete;
+
+    void copy(void* dst, const void* src, std::size_t bytes)
+    {
+        constexpr std::size_t kMinPart = std::size_t(1) << 20;
+        const std::size_t parts = std::min<std::size_t>(workers_.size() + 1,
+                                                        std::max<std::size_t>(1, bytes / kMinPart));
+        if (parts <= 1) {
+            std::memcpy(dst, src, bytes);
+            return;
+        }
+        const auto part = [=](std::size_t k) {
+            const std::size_t b = bytes * k / parts;
+            const std::size_t e = bytes * (k + 1) / parts;
+            std::memcpy(static_cast<char*>(dst) + b, static_cast<const char*>(src) + b, e - b);
+        };
+        {
+            std::lock_guard<std::mutex> lock(mutex_);
+            job_ = part;
+            parts_ = parts;
+            pending_ = parts - 1;  // parts 1.. go to the workers, part 0 stays here
+            ++generation_;
+        }
+        wake_.notify_all();
+        part(0);
+        std::unique_lock<std::mutex> lock(mutex_);
+        done_.wait(lock, [this] { return pending_ == 0; });
+    }
+
+private:
+    void work(int t)
+    {
+        std::size_t seen = 0;
+        for (;;) {
+            std::function<void(std::size_t)> job;
+            std::size_t parts = 0;
+            {
+                std::unique_lock<std::mutex> lock(mutex_);
+                wake_.wait(lock, [&] { return stop_ || generation_ != seen; });
+                if (stop_) {
+                    return;
+                }
+                seen = generation_;
+                job = job_;
+                parts = parts_;
+            }
+            const std::size_t k = static_cast<std::size_t>(t) + 1;
+            if (k < parts) {
+                job(k);
+                std::lock_guard<std::mutex> lock(mutex_);
+                if (--pending_ == 0) {
+                    done_.notify_one();
+                }
+            }
+        }
+    }
+
+    std::vector<std::thread> workers_;
+    std::mutex mutex_;
+    std::condition_variable wake_, done_;
+    std::function<void(std::size_t)> job_;
+    std::size_t parts_ = 0;
+    std::size_t pending_ = 0;
+    std::size_t generation_ = 0;
+    bool stop_ = false;
+};
+
+int copy_threads()
+{
+    const unsigned hw = std::thread::hardware_concurrency();
+    return static_cast<int>(std::min(7u, hw > 1 ? hw - 1 : 0u));
+}
 
 // --- owning handles over the C ABI ----------------------------------------------------------
 
@@ -155,7 +259,18 @@ public:
             if (config.memory == StreamMemory::Copy) {
                 s.stage_in = std::make_unique<Memory>(Kind::Pinned, chunk);
                 s.stage_out = std::make_unique<Memory>(Kind::Pinned, chunk);
+                std::memset(s.stage_in->host(), 0, chunk * sizeof(cf32));
+                std::memset(s.stage_out->host(), 0, chunk * sizeof(cf32));
             }
+            if (zero_copy) {
+                std::memset(s.out->host(), 0, window * sizeof(cf32));
+            }
+        }
+        if (config.memory == StreamMemory::Registered && n > 0) {
+            std::memset(result_->host(), 0, n * sizeof(cf32));
+        }
+        if (config.memory != StreamMemory::Registered) {
+            pool_ = std::make_unique<CopyPool>(copy_threads());
         }
     }
 
@@ -210,7 +325,7 @@ private:
         }
         const cf32* src = nullptr;
         if (config_.memory == StreamMemory::Copy) {
-            std::memcpy(s.stage_in->host(), x_.data() + s.start, bytes);
+            pool_->copy(s.stage_in->host(), x_.data() + s.start, bytes);
             src = s.stage_in->host();
         } else {
             src = capture_->host() + s.start;
@@ -249,9 +364,9 @@ private:
         RF_CHECK(rf_event_synchronize(s.downloaded.e));
         const std::size_t bytes = s.count * sizeof(cf32);
         if (config_.memory == StreamMemory::Copy) {
-            std::memcpy(y_.data() + s.start, s.stage_out->host(), bytes);
+            pool_->copy(y_.data() + s.start, s.stage_out->host(), bytes);
         } else if (config_.memory == StreamMemory::ZeroCopy) {
-            std::memcpy(y_.data() + s.start, s.out->host() + s.hist, bytes);
+            pool_->copy(y_.data() + s.start, s.out->host() + s.hist, bytes);
         }
         s.busy = false;
     }
@@ -267,6 +382,7 @@ private:
     std::unique_ptr<Memory> result_;   // registered: pinned result
     PlanHandle plan_;
     std::vector<std::unique_ptr<Slot>> slots_;
+    std::unique_ptr<CopyPool> pool_;  // copy and zero-copy modes
 };
 
 }  // namespace
