@@ -17,6 +17,7 @@
 #include "sigflow.hpp"
 
 #include <algorithm>
+#include <cstdint>
 #include <filesystem>
 #include <set>
 
@@ -38,27 +39,36 @@ std::vector<std::vector<int>> users_of(const Graph& g)
     return users;
 }
 
-// Whether `to` is reachable from `from` along input -> user edges.
-bool reaches(const std::vector<std::vector<int>>& users, int from, int to)
-{
-    std::vector<char> seen(users.size(), 0);
-    std::vector<int> todo = {from};
-    while (!todo.empty()) {
-        const int v = todo.back();
-        todo.pop_back();
-        if (v == to) {
-            return true;
-        }
-        if (seen[static_cast<std::size_t>(v)] != 0) {
-            continue;
-        }
-        seen[static_cast<std::size_t>(v)] = 1;
-        for (int u : users[static_cast<std::size_t>(v)]) {
-            todo.push_back(u);
+// Reachability along input -> user edges as one bit set of descendants per node, built in
+// decreasing id order (every user has a larger id than its inputs): n^2 / 8 bytes, which the
+// parser's bound on operations keeps to a few MB.
+class Reach {
+public:
+    Reach(const Graph& g, const std::vector<std::vector<int>>& users)
+        : words_((g.nodes.size() + 63) / 64), bits_(g.nodes.size() * words_, 0)
+    {
+        for (std::size_t v = g.nodes.size(); v-- > 0;) {
+            std::uint64_t* dv = row(v);
+            for (int u : users[v]) {
+                const std::uint64_t* du = row(static_cast<std::size_t>(u));
+                for (std::size_t w = 0; w < words_; ++w) {
+                    dv[w] |= du[w];
+                }
+                dv[static_cast<std::size_t>(u) / 64] |= std::uint64_t(1) << (u % 64);
+            }
         }
     }
-    return false;
-}
+    bool operator()(int from, int to) const
+    {
+        return (bits_[static_cast<std::size_t>(from) * words_ + static_cast<std::size_t>(to) / 64]
+                >> (to % 64)) & 1;
+    }
+
+private:
+    std::uint64_t* row(std::size_t v) { return bits_.data() + v * words_; }
+    std::size_t words_;
+    std::vector<std::uint64_t> bits_;
+};
 
 // Launch units: unit[i] is the representative of node i's unit (itself outside a group).
 // in_degree counts edges into a unit from other units (and self-loops, which only a cycle has);
@@ -146,6 +156,7 @@ void eliminate_dead_nodes(Graph& g)
 void fuse_firs(Graph& g)
 {
     const std::vector<std::vector<int>> users = users_of(g);
+    const Reach reaches(g, users);
     std::vector<int> firs;
     for (const Node& n : g.nodes) {
         if (g.live[static_cast<std::size_t>(n.id)] && n.op == Op::Fir) {
@@ -169,7 +180,7 @@ void fuse_firs(Graph& g)
             }
             bool independent = true;
             for (int m : grp) {
-                if (reaches(users, m, f) || reaches(users, f, m)) {
+                if (reaches(m, f) || reaches(f, m)) {
                     independent = false;
                     break;
                 }

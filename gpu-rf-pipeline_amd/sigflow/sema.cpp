@@ -104,6 +104,9 @@ std::string op_list()
 }
 
 constexpr long long kMaxFft = 1LL << 24;      // the library's longest transform
+// Depth of bindings resolved through one another (each level is a few stack frames here and in
+// the graph builder).
+constexpr std::size_t kMaxBindingDepth = 1000;
 constexpr long long kMaxChannels = 1LL << 20;
 
 class Analyzer {
@@ -197,20 +200,28 @@ private:
         if (done != types_.end()) {
             return done->second;
         }
-        for (std::size_t k = 0; k < stack_.size(); ++k) {
-            if (stack_[k] == name) {
-                std::string path;
-                for (std::size_t j = k; j < stack_.size(); ++j) {
-                    path += stack_[j] + " -> ";
-                }
-                diag_.error(at, "cycle: '" + name + "' depends on itself (" + path + name + ")");
-                return std::nullopt;
+        const auto open = on_stack_.find(name);
+        if (open != on_stack_.end()) {
+            std::string path;
+            for (std::size_t j = open->second; j < stack_.size(); ++j) {
+                path += stack_[j] + " -> ";
             }
+            diag_.error(at, "cycle: '" + name + "' depends on itself (" + path + name + ")");
+            return std::nullopt;
         }
+        if (stack_.size() >= kMaxBindingDepth) {
+            diag_.error(at, "bindings nested too deeply (more than "
+                                + std::to_string(kMaxBindingDepth) + " levels through '" + name
+                                + "')");
+            types_.emplace(name, std::nullopt);
+            return std::nullopt;
+        }
+        on_stack_.emplace(name, stack_.size());
         stack_.push_back(name);
         const Stmt& st = prog_.stmts[first_def_.at(name)];
         std::optional<Type> t = expr_type(*st.value, true);
         stack_.pop_back();
+        on_stack_.erase(name);
         types_.emplace(name, t);
         return t;
     }
@@ -518,7 +529,8 @@ private:
     std::map<std::string, std::size_t> first_def_;
     std::map<std::string, std::optional<Type>> types_;
     std::map<const Expr*, Type> expr_types_;
-    std::vector<std::string> stack_;
+    std::vector<std::string> stack_;           // bindings being resolved, outermost first
+    std::map<std::string, std::size_t> on_stack_;  // their positions in stack_
     Graph g_;
     std::map<std::string, int> nodes_;
 };

@@ -223,6 +223,12 @@ std::vector<Token> lex(const Source& src, Diagnostics& diag)
 
 namespace {
 
+// Bounds that keep every later stage's recursion and passes small whatever the input: nesting
+// of one expression (parentheses, call arguments and '|>' stages together) and operations
+// (calls and emits) in a program.
+constexpr int kMaxNesting = 256;
+constexpr int kMaxOperations = 10000;
+
 std::string describe(const Token& t)
 {
     switch (t.kind) {
@@ -242,7 +248,7 @@ public:
     Program program()
     {
         Program prog;
-        while (peek().kind != Token::Kind::End) {
+        while (peek().kind != Token::Kind::End && operations_ <= kMaxOperations) {
             try {
                 prog.stmts.push_back(statement());
                 if (peek().kind != Token::Kind::Newline) {
@@ -251,6 +257,7 @@ public:
                 }
                 take();
             } catch (const Failed&) {
+                depth_ = 0;
                 skip_line();
             }
         }
@@ -285,6 +292,7 @@ private:
         const Token& first = peek();
         st.pos = first.pos;
         if (first.kind == Token::Kind::Emit) {
+            count_operation(first);
             take();
             st.is_emit = true;
             if (peek().kind != Token::Kind::Ident) {
@@ -322,12 +330,19 @@ private:
 
     ExprPtr expr()
     {
+        const Nest level(*this, peek());
         ExprPtr e = term();
+        int stages = 0;
         while (peek().kind == Token::Kind::Pipe) {
             take();
             if (peek().kind != Token::Kind::Ident) {
                 fail_at(peek(), "expected an operation name after '|>', found " + describe(peek()));
             }
+            if (depth_ + ++stages > kMaxNesting) {
+                fail_at(peek(), "expression nested too deeply (more than "
+                                    + std::to_string(kMaxNesting) + " levels)");
+            }
+            count_operation(peek());
             ExprPtr call = std::make_unique<Expr>();
             call->kind = Expr::Kind::Call;
             call->pos = peek().pos;
@@ -377,6 +392,7 @@ private:
         case Token::Kind::Ident:
             e->text = take().text;
             if (peek().kind == Token::Kind::LParen) {
+                count_operation(t);
                 e->kind = Expr::Kind::Call;
                 args(*e);
             }
@@ -427,9 +443,33 @@ private:
         }
     }
 
+    // One more level of nesting for the rest of the enclosing scope.
+    struct Nest {
+        Parser& p;
+        Nest(Parser& parser, const Token& at) : p(parser)
+        {
+            if (p.depth_ + 1 > kMaxNesting) {
+                p.fail_at(at, "expression nested too deeply (more than "
+                                  + std::to_string(kMaxNesting) + " levels)");
+            }
+            ++p.depth_;
+        }
+        ~Nest() { --p.depth_; }
+    };
+
+    void count_operation(const Token& at)
+    {
+        if (++operations_ > kMaxOperations) {
+            fail_at(at, "program has more than " + std::to_string(kMaxOperations)
+                            + " operations");
+        }
+    }
+
     const std::vector<Token>& t_;
     Diagnostics& diag_;
     std::size_t i_ = 0;
+    int depth_ = 0;
+    int operations_ = 0;
 };
 
 }  // namespace

@@ -322,3 +322,45 @@ def test_mutated_programs_never_crash(tmp_path: Path) -> None:
             assert re.search(r"^m\.sf:\d+:\d+: error: ", proc.stderr, re.M), proc.stderr
         outcomes[proc.returncode] += 1
     assert outcomes[0] > 20 and outcomes[1] > 20
+
+
+def test_limits_end_in_diagnostics(tmp_path: Path) -> None:
+    """Inputs that would exhaust the stack or run long: one error, exit 1, quickly."""
+    fir = 'fir(taps="h.f32")'
+    cases = {
+        "nest": "a = " + "(" * 300 + 'capture("x.cf32", rate=1e6)' + ")" * 300
+                + '\nemit a -> "o.cf32"\n',
+        "pipe": 'a = capture("x.cf32", rate=1e6)' + f" |> {fir}" * 300 + '\nemit a -> "o.cf32"\n',
+        # used before bound, so resolving the emit walks the whole chain at once
+        "chain": "\n".join(['emit a1499 -> "o.cf32"']
+                           + [f"a{i} = a{i - 1} |> {fir}" for i in range(1499, 0, -1)]
+                           + ['a0 = capture("x.cf32", rate=1e6)']) + "\n",
+        "operations": "\n".join(['a0 = capture("x.cf32", rate=1e6)']
+                                + [f"a{i} = a0 |> {fir}" for i in range(1, 10050)]
+                                + ['emit a1 -> "o.cf32"']) + "\n",
+    }
+    messages = {"nest": "expression nested too deeply", "pipe": "expression nested too deeply",
+                "chain": "bindings nested too deeply", "operations": "more than 10000 operations"}
+    for name, text in cases.items():
+        (tmp_path / f"{name}.sf").write_text(text)
+        proc = sigflowc(f"{name}.sf", "--emit=hip", cwd=tmp_path)
+        assert proc.returncode == 1, name
+        assert proc.stderr.count(": error: ") == 1 and messages[name] in proc.stderr, proc.stderr[:300]
+
+
+def test_large_programs_compile(tmp_path: Path) -> None:
+    """Near the limits the passes stay fast: 9000 FIRs of one capture, 1000 of them emitted
+    (125 fusion groups of 8 after dead-node elimination), and a chain of 900 filters."""
+    fir = 'fir(taps="h.f32")'
+    wide = (['a0 = capture("x.cf32", rate=1e6)'] + [f"a{i} = a0 |> {fir}" for i in range(1, 9000)]
+            + [f'emit a{i} -> "o{i}.cf32"' for i in range(1, 9000, 9)])
+    chain = (['a0 = capture("x.cf32", rate=1e6)'] + [f"a{i} = a{i - 1} |> {fir}" for i in range(1, 900)]
+             + ['emit a899 -> "o.cf32"'])
+    for name, lines in [("wide", wide), ("chain", chain)]:
+        (tmp_path / f"{name}.sf").write_text("\n".join(lines) + "\n")
+        proc = sigflowc(f"{name}.sf", "--emit=json", cwd=tmp_path)
+        assert proc.returncode == 0, proc.stderr[:300]
+        graph = json.loads(proc.stdout)
+        check_plan(graph)
+        groups = {n["group"] for n in graph["nodes"] if n.get("group") is not None}
+        assert len(groups) == (125 if name == "wide" else 0)
