@@ -32,16 +32,21 @@ const std::vector<std::string>& xcorr_impl_names()
 
 namespace {
 
-// A C++ string literal of `text`.
+// A C++ string literal of `text`: quotes and backslashes escaped, control characters as
+// three-digit octal escapes (a fixed length, so a following digit cannot extend them).
 std::string quote(const std::string& text)
 {
     std::string out = "\"";
     for (char c : text) {
+        const unsigned char u = static_cast<unsigned char>(c);
         if (c == '"' || c == '\\') {
             out += '\\';
             out += c;
-        } else if (c == '\n') {
-            out += "\\n";
+        } else if (u < 0x20 || u == 0x7f) {
+            const char octal[] = {'\\', static_cast<char>('0' + (u >> 6)),
+                                  static_cast<char>('0' + ((u >> 3) & 7)),
+                                  static_cast<char>('0' + (u & 7)), '\0'};
+            out += octal;
         } else {
             out += c;
         }
