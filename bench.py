@@ -579,6 +579,20 @@ def bench_delay(rf, args) -> dict:
     ms_exact = time_steps(rf, lambda i: batch.run(pres.ptr, stream.handle, graph=True),
                           args.steps, args.warmup, stream.handle) / args.steps
     exact = rf.read_peaks(pres, count)
+    # The plain FP32 path the batch replaces: one XcorrFft (GCC-PHAT) and one device peak
+    # search per pair, the pairs one after another on the stream.
+    plain = rf.XcorrFft(n, n, True)
+    d_r = rf.DeviceBuffer(2 * n - 1)
+    pplain = rf.DeviceBuffer(16 * count, np.uint8)
+
+    def plain_step(i):
+        for k in range(count):
+            plain.run(a.offset(k * n), b.offset(k * n), d_r.ptr, stream.handle)
+            rf.peak_device(d_r.ptr, 2 * n - 1, pplain.offset(16 * k), stream.handle)
+
+    ms_plain = time_steps(rf, plain_step, 2, 1, stream.handle) / 2
+    plain_same = int(sum(p.index == e.index for p, e in zip(rf.read_peaks(pplain, count), exact)))
+    release(rf, plain, d_r, pplain)
 
     labels = [int(min(max((int(np.rint(x)) + 512) >> 4, 0), 63)) for x in delays]
     proposals_right = sum(r.proposal == lab for r, lab in zip(r_mlp, labels))
@@ -658,6 +672,11 @@ def bench_delay(rf, args) -> dict:
         "exact_gcc_batch": {"ms": round(ms_exact, 4),
                             "pairs_per_s": round(count / (ms_exact * 1e-3), 1),
                             "speedup_of_mlp_search": round(ms_exact / ms_mlp, 3)},
+        "plain_per_pair": {"ms": round(ms_plain, 3),
+                           "pairs_per_s": round(count / (ms_plain * 1e-3), 1),
+                           "speedup_of_exact_batch": round(ms_plain / ms_exact, 2),
+                           "speedup_of_mlp_search": round(ms_plain / ms_mlp, 2),
+                           "same_peak_index_as_exact": plain_same},
         "long_capture": long_capture,
         "cpu_baseline": cpu,
         "cpu_oracle": cpu_oracle,
