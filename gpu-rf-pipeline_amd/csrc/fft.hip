@@ -3,7 +3,6 @@
 // transforms; see fft_core.hpp and fft_passes.hpp for the method.
 
 #include "fft_plan.hpp"
-#include "fir_internal.hpp"
 #include "twiddle.hpp"
 
 #include <cstdint>
@@ -13,38 +12,54 @@
 namespace rf {
 namespace {
 
+// Where a transform's result goes: X itself, or with MAG, |X| (cabs_f, the magnitude kernel's
+// formula, so spectrum-then-magnitude and the fused store agree bit for bit) into `mag`.
+template <bool MAG>
+struct SpectrumOut {
+    float2* out;
+    float* mag;
+    __device__ __forceinline__ void put(long long i, float2 v) const
+    {
+        if constexpr (MAG) {
+            mag[i] = cabs_f(v);
+        } else {
+            out[i] = v;
+        }
+    }
+};
+
 // nfft in {1, 2, 4}: one lane per transform, in registers.
-template <bool INV>
-__global__ void fft_tiny_kernel(const float2* in, float2* out, int n, long long batch)
+template <bool INV, bool MAG>
+__global__ void fft_tiny_kernel(const float2* in, SpectrumOut<MAG> dst, int n, long long batch)
 {
     const long long f = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (f >= batch) {
         return;
     }
     const float2* src = in + f * n;
-    float2* dst = out + f * n;
+    const long long o = f * n;
     if (n == 1) {
-        dst[0] = src[0];
+        dst.put(o, src[0]);
     } else if (n == 2) {
         float2 a = src[0];
         float2 b = src[1];
         dft2<INV>(a, b);
-        dst[0] = a;
-        dst[1] = b;
+        dst.put(o, a);
+        dst.put(o + 1, b);
     } else {
         float2 v0 = src[0], v1 = src[1], v2 = src[2], v3 = src[3];
         dft4<INV>(v0, v1, v2, v3);
-        dst[0] = v0;
-        dst[1] = v1;
-        dst[2] = v2;
-        dst[3] = v3;
+        dst.put(o, v0);
+        dst.put(o + 1, v1);
+        dst.put(o + 2, v2);
+        dst.put(o + 3, v3);
     }
 }
 
 // Exact DFT for lengths that are not powers of two: one lane per output bin, the frame read
 // in order, twiddle index (j * k) mod n kept in integers, table built in float64.
-template <bool INV>
-__global__ void dft_kernel(const float2* __restrict__ in, float2* __restrict__ out, int n,
+template <bool INV, bool MAG>
+__global__ void dft_kernel(const float2* __restrict__ in, SpectrumOut<MAG> out, int n,
                            long long batch, const float2* __restrict__ table)
 {
     const long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -72,15 +87,14 @@ __global__ void dft_kernel(const float2* __restrict__ in, float2* __restrict__ o
             e -= n;
         }
     }
-    out[idx] = make_float2(re, im);
+    out.put(idx, make_float2(re, im));
 }
 
 // Row pass of the four-step transform for one transform of the batch, natural-order output:
 // row k2 (n1 contiguous points) -> X[k2 + n2 * k1].
-template <bool INV>
+template <bool INV, bool MAG>
 __global__ __launch_bounds__(512) void four_step_row_kernel(const float2* __restrict__ in,
-                                                            float2* __restrict__ out,
-                                                            FourStep fs,
+                                                            SpectrumOut<MAG> out, FourStep fs,
                                                             const float2* __restrict__ table)
 {
     extern __shared__ float2 lds[];
@@ -102,11 +116,11 @@ __global__ __launch_bounds__(512) void four_step_row_kernel(const float2* __rest
     const int f = threadIdx.x / per;
     const int t = threadIdx.x - f * per;
     lds_fft<INV>(lds + (f < count ? f : 0) * slots, fs.log2n1, t, f < count, table);
-    float2* dst = out + transform * L;
+    const long long o = transform * L;
     for (int e = threadIdx.x; e < total; e += blockDim.x) {
         const int r = e >> fs.log2n1;
         const int k1 = e & (n1 - 1);
-        dst[(row0 + r) + static_cast<long long>(n2) * k1] = lds[r * slots + lds_pad(k1)];
+        out.put(o + (row0 + r) + static_cast<long long>(n2) * k1, lds[r * slots + lds_pad(k1)]);
     }
 }
 
@@ -130,8 +144,8 @@ __global__ __launch_bounds__(512) void four_step_column_kernel(const float2* __r
         lds, table);
 }
 
-// The same batched transform with |X| = sqrtf(re^2 + im^2) as the store epilogue (the
-// magnitude kernel's formula, so the result equals spectrum-then-magnitude bit for bit).
+// The same batched transform with |X| (cabs_f) as the store epilogue (the magnitude kernel's
+// formula, so the result equals spectrum-then-magnitude bit for bit).
 __global__ __launch_bounds__(512) void fft_lds_batch_magnitude_kernel(
     const float2* in, float* out, int log2P, long long batch, const float2* __restrict__ table)
 {
@@ -155,7 +169,7 @@ __global__ __launch_bounds__(512) void fft_lds_batch_magnitude_kernel(
     for (int e = threadIdx.x; e < total; e += blockDim.x) {
         const int fe = e >> log2P;
         const float2 v = lds[fe * slots + lds_pad(e & (P - 1))];
-        out[base + e] = sqrtf(v.x * v.x + v.y * v.y);
+        out[base + e] = cabs_f(v);
     }
 }
 
@@ -165,18 +179,25 @@ std::size_t lds_batch_bytes(int log2P)
            * sizeof(float2);
 }
 
-template <bool INV>
-void exec_dir(const rf_fft_plan& plan, const float2* in, float2* out, hipStream_t stream)
+// The plan's transform into `out`, or with MAG its magnitude into `mag` (out unused); MAG
+// only for the plan kinds other than kPlanLds, which has its own fused kernel.
+template <bool INV, bool MAG = false>
+void exec_dir(const rf_fft_plan& plan, const float2* in, float2* out, hipStream_t stream,
+              float* mag = nullptr)
 {
     const long long batch = plan.batch;
+    const SpectrumOut<MAG> dst{out, mag};
     switch (plan.kind) {
     case kPlanTiny: {
         const unsigned grid = static_cast<unsigned>(ceil_div(batch, 256));
-        fft_tiny_kernel<INV><<<grid, 256, 0, stream>>>(in, out, plan.nfft, batch);
+        fft_tiny_kernel<INV, MAG><<<grid, 256, 0, stream>>>(in, dst, plan.nfft, batch);
         check_launch("fft_tiny_kernel");
         return;
     }
     case kPlanLds: {
+        if constexpr (MAG) {
+            fail(RF_INVALID_ARG, "exec_dir: LDS plans have their own magnitude kernel");
+        }
         const int F = lds_batch_per_block(plan.log2n);
         const int threads = F * (plan.nfft >> 3);
         const unsigned grid = static_cast<unsigned>(ceil_div(batch, F));
@@ -198,24 +219,25 @@ void exec_dir(const rf_fft_plan& plan, const float2* in, float2* out, hipStream_
         const int F = lds_batch_per_block(fs.log2n1);
         const dim3 rgrid(static_cast<unsigned>(ceil_div(1 << fs.log2n2, F)),
                          static_cast<unsigned>(batch));
-        four_step_row_kernel<INV><<<rgrid, F * ((1 << fs.log2n1) >> 3),
-                                    lds_batch_bytes(fs.log2n1), stream>>>(plan.scratch, out, fs,
-                                                                          table);
+        four_step_row_kernel<INV, MAG><<<rgrid, F * ((1 << fs.log2n1) >> 3),
+                                         lds_batch_bytes(fs.log2n1), stream>>>(plan.scratch, dst,
+                                                                               fs, table);
         check_launch("four_step_row_kernel");
         return;
     }
     case kPlanDft: {
         const long long total = batch * plan.nfft;
         const unsigned grid = static_cast<unsigned>(ceil_div(total, 256));
-        if (in == out) {
+        if (!MAG && in == out) {
             // The DFT reads whole frames while writing bins, so it needs a separate output.
-            dft_kernel<INV><<<grid, 256, 0, stream>>>(in, plan.scratch, plan.nfft, batch,
-                                                      plan.tables);
+            dft_kernel<INV, false><<<grid, 256, 0, stream>>>(
+                in, SpectrumOut<false>{plan.scratch, nullptr}, plan.nfft, batch, plan.tables);
             check_launch("dft_kernel");
             RF_HIP_CHECK(hipMemcpyAsync(out, plan.scratch, total * sizeof(float2),
                                         hipMemcpyDeviceToDevice, stream));
         } else {
-            dft_kernel<INV><<<grid, 256, 0, stream>>>(in, out, plan.nfft, batch, plan.tables);
+            dft_kernel<INV, MAG><<<grid, 256, 0, stream>>>(in, dst, plan.nfft, batch,
+                                                           plan.tables);
             check_launch("dft_kernel");
         }
         return;
@@ -375,7 +397,6 @@ rf_status rf_spectrum_magnitude(const rf_fft_plan* plan, const rf_cf32* in, floa
             return;
         }
         hipStream_t s = rf::as_stream(stream);
-        const long long total = static_cast<long long>(plan->batch) * plan->nfft;
         if (plan->kind == rf::kPlanLds) {
             const int F = rf::lds_batch_per_block(plan->log2n);
             const int threads = F * (plan->nfft >> 3);
@@ -387,14 +408,8 @@ rf_status rf_spectrum_magnitude(const rf_fft_plan* plan, const rf_cf32* in, floa
             rf::check_launch("fft_lds_batch_magnitude_kernel");
             return;
         }
-        // Other lengths: the spectrum into stream-ordered scratch, then the magnitude pass.
-        rf::StreamScratch spec(static_cast<std::size_t>(total) * sizeof(float2), s);
-        rf::fft_exec(*plan, reinterpret_cast<const float2*>(in), static_cast<float2*>(spec.ptr),
-                     RF_FFT_FORWARD, s);
-        if (total > INT32_MAX) rf::fail(RF_LIMIT, "spectrum magnitude: more than 2^31 values");
-        const rf_status st = rf_magnitude(static_cast<const rf_cf32*>(spec.ptr), mag,
-                                          static_cast<int>(total), stream);
-        if (st != RF_OK) rf::fail(st, rf_last_error());
+        // Other lengths: the plan's last pass (row pass, DFT, tiny transform) stores |X|.
+        rf::exec_dir<false, true>(*plan, reinterpret_cast<const float2*>(in), nullptr, s, mag);
     });
 }
 

@@ -7,11 +7,6 @@
 namespace rf {
 namespace {
 
-__device__ __forceinline__ float cabs_f(float2 v)
-{
-    return sqrtf(v.x * v.x + v.y * v.y);
-}
-
 __global__ __launch_bounds__(256) void magnitude_kernel(const float2* __restrict__ x,
                                                         float* __restrict__ mag, int n)
 {

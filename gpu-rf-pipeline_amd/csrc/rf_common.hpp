@@ -155,6 +155,13 @@ __device__ __forceinline__ float2 cscale(float2 a, float s)
 {
     return make_float2(a.x * s, a.y * s);
 }
+// |a| = sqrt(re^2 + im^2) with one fixed rounding sequence, fma(re, re, im * im): left to
+// contraction, the compiler may fuse either product depending on the surrounding code, so the
+// magnitude pass and the transforms that store |X| directly would differ in the last bit.
+__device__ __forceinline__ float cabs_f(float2 a)
+{
+    return sqrtf(fmaf(a.x, a.x, a.y * a.y));
+}
 
 // Complex loads and stores through the global address space. A pointer read from memory (a
 // descriptor struct) is generic to the compiler, which then emits flat instructions; those

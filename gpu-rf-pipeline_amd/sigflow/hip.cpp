@@ -9,7 +9,8 @@
 //   - captures and results live in page-locked host memory: copies are asynchronous DMA;
 //   - an FFT input's last-frame padding is cleared by a fill kernel, which a graph can hold;
 //   - fft |> magnitude with nothing else reading the spectrum is one spectrum-magnitude
-//     kernel (fuse_spectrum), so the complex spectrum never goes to HBM.
+//     call (fuse_spectrum): the complex spectrum is never stored (up to 4096 points it never
+//     leaves the kernel; longer transforms keep only the first pass in the plan's scratch).
 
 #include "sigflow.hpp"
 

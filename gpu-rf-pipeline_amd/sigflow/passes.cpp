@@ -405,8 +405,8 @@ MemoryPlan plan_memory(const Liveness& l, const std::vector<long long>& bytes)
 
 // An fft whose only live reader is a magnitude, and which nothing emits, folds into that
 // magnitude: one spectrum-magnitude call reads the fft's input and writes |X|, and the spectrum
-// itself is never stored. Lengths 8..4096 (powers of two) only, where the library does this in
-// one kernel.
+// itself is never stored. Every length: the library's last pass of each plan kind (the
+// one-workgroup transform, the four-step row pass, the DFT) stores the magnitude.
 void fuse_spectrum(Graph& g)
 {
     const std::vector<std::vector<int>> users = users_of(g);
@@ -416,9 +416,7 @@ void fuse_spectrum(Graph& g)
         }
         const int f = m.inputs[0];
         const Node& fft = g.nodes[static_cast<std::size_t>(f)];
-        const long long n = fft.n;
-        if (fft.op != Op::Fft || users[static_cast<std::size_t>(f)].size() != 1 || n < 8
-            || n > 4096 || (n & (n - 1)) != 0) {
+        if (fft.op != Op::Fft || users[static_cast<std::size_t>(f)].size() != 1) {
             continue;
         }
         m.spectrum_of = f;

@@ -227,7 +227,10 @@ rf_status rf_spectrum(const rf_fft_plan* plan, const rf_cf32* in, rf_cf32* out,
 /* Spectrum then magnitude in one pass (no reference counterpart; the fusion deferred in
  * docs/ROADMAP.md:57-59 of the reference): mag (DEVICE, nfft * batch floats) = |rf_spectrum|,
  * sqrtf(re^2 + im^2) as rf_magnitude computes it. Plans with 8 <= nfft <= 4096 run as one
- * kernel whose store writes the magnitude; other lengths go through stream-ordered scratch. */
+ * kernel whose store writes the magnitude; for other lengths the plan's last pass (the row
+ * pass of the four-step transform, the DFT, the 1/2/4-point transform) stores the magnitude.
+ * No allocation on any path; calls that share a four-step plan on two streams race on the
+ * plan's scratch, as rf_fft_exec does. */
 rf_status rf_spectrum_magnitude(const rf_fft_plan* plan, const rf_cf32* in, float* mag,
                                 rf_stream stream);
 

@@ -75,3 +75,61 @@ def test_first_transform_fir_calls_inside_a_capture() -> None:
     proc = subprocess.run([sys.executable, "-c", CHILD, str(tests_dir)], capture_output=True,
                           text=True, timeout=300)
     assert proc.returncode == 0 and "capture ok" in proc.stdout, proc.stdout + proc.stderr
+
+
+SPECTRUM_CHILD = r"""
+import ctypes, sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from _common import load_rfhip
+rf = load_rfhip()
+hip = ctypes.CDLL("libamdhip64.so")
+raw = ctypes.c_void_p()
+assert hip.hipStreamCreateWithFlags(ctypes.byref(raw), 1) == 0  # hipStreamNonBlocking
+stream = raw.value
+rng = np.random.default_rng(5)
+cases = []
+for nfft, frames in [(4, 1000), (100, 37), (1024, 9), (16384, 3), (1 << 18, 1)]:
+    x = (rng.standard_normal(nfft * frames)
+         + 1j * rng.standard_normal(nfft * frames)).astype(np.complex64)
+    plan = rf.FftPlan(nfft, frames)
+    d_mag = rf.DeviceBuffer(x.size, np.float32)
+    cases.append((nfft, x, plan, rf.DeviceBuffer.from_array(x), d_mag))
+print("capturing rf_spectrum_magnitude at 4, 100, 1024, 16384 and 2^18 points", flush=True)
+rf.check(rf._lib.rf_graph_begin_capture(stream))
+status = rf.RF_OK
+try:
+    for _, _, plan, d_x, d_mag in cases:
+        status = rf._lib.rf_spectrum_magnitude(plan.handle, d_x.ptr, d_mag.ptr, stream)
+        if status != rf.RF_OK:
+            break
+finally:
+    graph = ctypes.c_void_p()
+    end = rf._lib.rf_graph_end_capture(stream, ctypes.byref(graph))
+rf.check(status)
+rf.check(end)
+want = [rf.spectrum_magnitude(x, nfft) for nfft, x, _, _, _ in cases]
+for rep in range(3):
+    for *_, d_mag in cases:
+        d_mag.zero()
+    rf.check(rf._lib.rf_stream_synchronize(None))
+    rf.check(rf._lib.rf_graph_launch(graph.value, stream))
+    rf.check(rf._lib.rf_stream_synchronize(stream))
+    for (nfft, *_, d_mag), w in zip(cases, want):
+        assert d_mag.download().tobytes() == w.tobytes(), f"nfft {nfft} replay {rep}"
+    rf.spectrum_magnitude(cases[rep][1], cases[rep][0])  # eager work between replays
+rf.check(rf._lib.rf_graph_destroy(graph.value))
+for _, _, plan, _, _ in cases:
+    plan.destroy()
+rf.check(rf._lib.rf_stream_destroy(stream))
+print("capture ok", flush=True)
+"""
+
+
+def test_spectrum_magnitude_inside_a_capture() -> None:
+    """rf_spectrum_magnitude allocates nothing for any plan kind (the last pass stores |X|), so
+    it records into a graph at every length, and each replay gives the eager bits."""
+    tests_dir = Path(__file__).resolve().parent
+    proc = subprocess.run([sys.executable, "-c", SPECTRUM_CHILD, str(tests_dir)],
+                          capture_output=True, text=True, timeout=300)
+    assert proc.returncode == 0 and "capture ok" in proc.stdout, proc.stdout + proc.stderr

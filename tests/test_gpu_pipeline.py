@@ -95,10 +95,12 @@ def test_spectrum_any_length_matches_oracle(nfft: int) -> None:
     np.testing.assert_allclose(got, want, rtol=SPECTRUM_RTOL, atol=SPECTRUM_ATOL * scale)
 
 
-@pytest.mark.parametrize("nfft", [1, 8, 64, 100, 1024, 4096, 16384])
+@pytest.mark.parametrize("nfft", [1, 2, 4, 8, 48, 64, 100, 1024, 4096, 8192, 16384, 65536,
+                                  1 << 18])
 def test_spectrum_magnitude_fusion(nfft: int) -> None:
     """One-pass |spectrum| equals spectrum then magnitude bit for bit, and the oracle's |X|
-    within the spectrum tolerances."""
+    within the spectrum tolerances, for every plan kind: 1/2/4-point transforms, exact DFT,
+    one-workgroup transforms and the four-step transform (whose row pass stores |X|)."""
     x = cf32("tone")
     fused = rf.spectrum_magnitude(x, nfft)
     two_pass = rf.magnitude(rf.spectrum(x, nfft))
