@@ -56,6 +56,16 @@ constexpr int kChanThreads = 512;
 constexpr int kChanOutputs = 4096;  // frames * M per workgroup
 constexpr int kChanMaxWindowTaps = 16;
 
+// LDS stride between the frames of the inverse FFTs. For M = 64 each radix-8 pass reads and
+// writes slots f S + 9 r + t (t < 8 the lane within frame f): with S = 72 (8 mod 32) the four
+// frames of a 32-lane group fill the 32 bank pairs once, where the padded transform's own
+// stride (73) made the first pass's writes 9 (f + t) mod 32 collide. Other M keep lds_slots.
+__host__ __device__ constexpr int chan_frame_slots(int log2M)
+{
+    return log2M == 6 ? 72 : lds_slots(1 << log2M);
+}
+static_assert(lds_pad(63) < 72, "a 64-point frame fits 72 slots");
+
 // Branch sums of one lane for M <= 512: branch m, frames 8g .. 8g+7 of the workgroup.
 // PB (1..16) taps per branch at compile time; lds holds x[t0 M - pb M + 1 + i]. The window of
 // 8 + PB - 1 samples along the branch is streamed from the newest sample to the oldest, each
@@ -115,7 +125,12 @@ __global__ __launch_bounds__(kChanThreads) void channelize_fused_kernel(
     const float2* __restrict__ x, const float* __restrict__ taps, float2* __restrict__ out,
     int n, int log2M_rt, int num_taps, long long frames, const float2* __restrict__ table)
 {
-    extern __shared__ float2 lds[];
+    // The staged samples sit one slot in (st = lds + 1), so that the interior groups' sample
+    // pairs, which start one sample before the group, land on 16-byte LDS units: one 16-byte
+    // LDS write per pair instead of two 8-byte writes to alternate slots (two-way bank
+    // conflicts). Slot 0 and the slot after the last sample take the dropped ends.
+    extern __shared__ __attribute__((aligned(16))) float2 lds[];
+    float2* const st = lds + 1;
     const int log2M = LOG2M ? LOG2M : log2M_rt;
     const int M = 1 << log2M;
     const int F = kChanOutputs >> log2M;
@@ -138,7 +153,8 @@ __global__ __launch_bounds__(kChanThreads) void channelize_fused_kernel(
         constexpr int kIters = (kLen + kChanThreads - 1) / kChanThreads;
         // Interior groups whose samples base - 1 .. base + kLen start 16-byte aligned load
         // them as pairs (one 16-byte load per two samples, half the load instructions; the
-        // first and last sample of the pairs fall outside the group's range and are dropped).
+        // first and last sample of the pairs fall outside the group's range and land in the
+        // two spare slots).
         constexpr int kPairs = kLen / 2 + 1;
         constexpr int kPairIters = (kPairs + kChanThreads - 1) / kChanThreads;
         const unsigned long long pair0 =
@@ -153,17 +169,13 @@ __global__ __launch_bounds__(kChanThreads) void channelize_fused_kernel(
                 const int q = threadIdx.x + k * kChanThreads;
                 w[k] = q < kPairs ? src[q] : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
             }
+            // pair q holds samples 2 q - 1 and 2 q of the group: st[2 q - 1 .. 2 q] = lds[2 q ..]
+            f32x4* const lds4 = reinterpret_cast<f32x4*>(lds);
 #pragma unroll
             for (int k = 0; k < kPairIters; ++k) {
                 const int q = threadIdx.x + k * kChanThreads;
-                const int i = 2 * q - 1;  // LDS index of the pair's first sample
                 if (q < kPairs) {
-                    if (i >= 0) {
-                        lds[i] = make_float2(w[k].x, w[k].y);
-                    }
-                    if (i + 1 < kLen) {
-                        lds[i + 1] = make_float2(w[k].z, w[k].w);
-                    }
+                    lds4[q] = w[k];
                 }
             }
         } else {
@@ -186,13 +198,13 @@ __global__ __launch_bounds__(kChanThreads) void channelize_fused_kernel(
             for (int k = 0; k < kIters; ++k) {
                 const int i = threadIdx.x + k * kChanThreads;
                 if (i < kLen) {
-                    lds[i] = v[k];
+                    st[i] = v[k];
                 }
             }
         }
     } else {
         for (int i = threadIdx.x; i < len; i += kChanThreads) {
-            lds[i] = load_sample(x, base + i, n);
+            st[i] = load_sample(x, base + i, n);
         }
     }
     __syncthreads();
@@ -203,17 +215,17 @@ __global__ __launch_bounds__(kChanThreads) void channelize_fused_kernel(
     if (windowed) {
         const int m = threadIdx.x & (M - 1);
         const int g = threadIdx.x >> log2M;
-        branch_window<PB ? PB : 1>(lds, taps, num_taps, M, m, g, acc);
+        branch_window<PB ? PB : 1>(st, taps, num_taps, M, m, g, acc);
     } else {
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int e = threadIdx.x + j * kChanThreads;
-            acc[j] = branch_generic(lds, taps, num_taps, M, pb, e >> log2M, e & (M - 1));
+            acc[j] = branch_generic(st, taps, num_taps, M, pb, e >> log2M, e & (M - 1));
         }
     }
     __syncthreads();  // the staged samples are dead; the frames reuse the LDS
 
-    const int slots = lds_slots(M);
+    const int slots = chan_frame_slots(log2M);
     if (windowed) {
         const int m = threadIdx.x & (M - 1);
         const int g = threadIdx.x >> log2M;
@@ -266,8 +278,8 @@ std::size_t fused_lds_bytes(int log2M, int pb)
 {
     const int M = 1 << log2M;
     const int F = kChanOutputs >> log2M;
-    const std::size_t stage = static_cast<std::size_t>(F + pb - 1) * M;
-    const std::size_t frames = static_cast<std::size_t>(F) * lds_slots(M);
+    const std::size_t stage = static_cast<std::size_t>(F + pb - 1) * M + 2;  // + spare slots
+    const std::size_t frames = static_cast<std::size_t>(F) * chan_frame_slots(log2M);
     return (stage > frames ? stage : frames) * sizeof(float2);
 }
 
