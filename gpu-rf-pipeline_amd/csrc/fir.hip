@@ -378,16 +378,15 @@ __device__ __forceinline__ void buffer_store2(float2 v, __amdgpu_buffer_rsrc_t r
 }
 
 // One segment: seg[i] = x[in0 + i], i < 4096 (zero outside the input), filtered by the
-// spectrum, outputs y[out0 + i - (T-1)] for T-1 <= i < 4096. Lane t holds positions t + 256 r.
-// The twiddle factors are loaded after the samples, so they arrive while the first pass runs,
-// and serve both transforms.
+// spectrum, outputs y[out0 + i - (T-1)] for T-1 <= i < 4096. Lane t (threadIdx.x, passed in)
+// holds positions t + 256 r. The twiddle factors are loaded after the samples, so they arrive
+// while the first pass runs, and serve both transforms.
 __device__ __forceinline__ void ola_segment(const float2* __restrict__ x,
                                             const float2* __restrict__ spec,
                                             float2* __restrict__ y, int n, int halo,
                                             long long out0, float2* buf,
-                                            const float2* __restrict__ stages)
+                                            const float2* __restrict__ stages, int t)
 {
-    const int t = threadIdx.x;
     const long long in0 = out0 - halo;
     const unsigned lane_bytes = static_cast<unsigned>(t) * 8u;
     constexpr unsigned kBlockBytes = k4096Lanes * 8u;
@@ -487,7 +486,7 @@ __global__ __launch_bounds__(k4096Lanes) void fir_ola_kernel(const float2* __res
         return;  // the whole workgroup: padding of the grid to a multiple of 8
     }
     const int halo = num_taps - 1;
-    ola_segment(x, spec, y, n, halo, seg * (kOlaP - halo), buf, stages);
+    ola_segment(x, spec, y, n, halo, seg * (kOlaP - halo), buf, stages, threadIdx.x);
 }
 
 __global__ __launch_bounds__(k4096Lanes) void fir_ola_batched_kernel(
@@ -501,7 +500,7 @@ __global__ __launch_bounds__(k4096Lanes) void fir_ola_batched_kernel(
     if (out0 >= batch.n[c]) {
         return;
     }
-    ola_segment(batch.x[c], spec, batch.y[c], batch.n[c], halo, out0, buf, stages);
+    ola_segment(batch.x[c], spec, batch.y[c], batch.n[c], halo, out0, buf, stages, threadIdx.x);
 }
 
 // --- persistent batch over any number of uneven captures --------------------------------------
@@ -516,7 +515,11 @@ struct FirCapture {
 // Work items {capture, segment} (overlap-save) or {capture, output block} (register), built on
 // the host in capture order, walked XCD-major: each XCD takes a contiguous run of items, so a
 // segment's halo was just read by its predecessor behind the same L2.
-__global__ __launch_bounds__(k4096Lanes) void fir_ola_batch_kernel(
+// The lane index goes through an empty asm inside the walk: without it the compiler hoists
+// every per-lane address of the segment out of the loop into held registers (166 VGPRs, three
+// workgroups per CU); with it and a four-wave bound the kernel keeps the single-capture
+// kernel's four workgroups per CU (111 VGPRs, no spills).
+__global__ __launch_bounds__(k4096Lanes, 4) void fir_ola_batch_kernel(
     const FirCapture* __restrict__ caps, const int2* __restrict__ items, int total,
     const float2* __restrict__ spec, int num_taps, const float2* __restrict__ stages)
 {
@@ -525,8 +528,10 @@ __global__ __launch_bounds__(k4096Lanes) void fir_ola_batch_kernel(
     xcd_major_items(total, [&](int item) {
         const int2 it = items[item];
         const FirCapture c = caps[it.x];
+        int t = threadIdx.x;
+        asm volatile("" : "+v"(t));
         ola_segment(c.x, spec, c.y, c.n, halo, static_cast<long long>(it.y) * (kOlaP - halo), buf,
-                    stages);
+                    stages, t);
         __syncthreads();  // the next segment reuses buf
     });
 }
