@@ -72,11 +72,127 @@ __host__ __device__ constexpr int column_slots(int n2)
     return lds_slots(n2);
 }
 
+// Register-held Stockham passes for the column transforms. A transform of P = 2^log2P points
+// (8 <= P <= 4096) is done by n8 = P / 8 lanes, lane t holding 8 values; the passes are those
+// of lds_fft (radix 8 first, then one radix-4 or radix-2 pass; span p = 1, 8, 64, ...) with the
+// same arithmetic, but a lane keeps its values in registers from one pass to the next and the
+// LDS only carries the exchange between passes: the first pass works on the values as loaded
+// and the last pass's results stay in registers. Value b of a pass of radix R = 2^LR is input
+// position t + (b / R) n8 + (b mod R) P / R and, after the butterfly, output position
+// (i - k) R + k + (b mod R) p with i = t + (b / R) n8, k = i mod p.
+template <int LR>
+__device__ __forceinline__ int stockham_in(int t, int n8, int P, int b)
+{
+    return t + (b >> LR) * n8 + (b & ((1 << LR) - 1)) * (P >> LR);
+}
+
+template <int LR>
+__device__ __forceinline__ int stockham_out(int t, int n8, int p, int b)
+{
+    const int i = t + (b >> LR) * n8;
+    const int k = i & (p - 1);
+    return (i - k) * (1 << LR) + k + (b & ((1 << LR) - 1)) * p;
+}
+
+// The butterflies of one pass (span p) on the lane's 8 values, as lds_pass computes them.
+template <bool INV, int LR>
+__device__ __forceinline__ void stockham_pass_regs(float2 (&u)[8], int t, int n8, int p,
+                                                   const float2* __restrict__ table)
+{
+    constexpr int R = 1 << LR;
+    const int tw_step = kTwiddleSize / (p * R);
+#pragma unroll
+    for (int bi = 0; bi < (8 >> LR); ++bi) {
+        const int i = t + bi * n8;
+        twiddle_and_dft<INV>(u + bi * R, LR, i & (p - 1), table, tw_step);
+    }
+}
+
+// The outputs of a pass of radix 2^LRP and span pp go to the column's LDS, the inputs of the
+// next pass (radix 2^LR) come back. TAIL: a second barrier, so that the next exchange may
+// write; the last exchange of a transform leaves that to the caller.
+template <int LRP, int LR, bool TAIL>
+__device__ __forceinline__ void stockham_exchange(float2* col, float2 (&u)[8], int t, int n8,
+                                                  int P, int pp, bool active)
+{
+    if (active) {
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            col[lds_pad(stockham_out<LRP>(t, n8, pp, b))] = u[b];
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            u[b] = col[lds_pad(stockham_in<LR>(t, n8, P, b))];
+        }
+    }
+    if (TAIL) {
+        __syncthreads();
+    }
+}
+
+// The transform of one column: on entry u[r] holds position t + n8 r, on exit u[b] holds
+// output position t + n8 stockham_order(lr, b), where lr (returned) is the radix log2 of the
+// last pass. Every lane of the workgroup calls it (it has barriers); `col` is the column's
+// padded LDS (lds_slots(P) values) and is only touched by active lanes. After it returns, the
+// caller puts a barrier before the LDS is written again.
+template <bool INV>
+__device__ __forceinline__ int stockham_column(float2* col, float2 (&u)[8], int log2P, int t,
+                                               bool active, const float2* __restrict__ table)
+{
+    const int P = 1 << log2P;
+    const int n8 = P >> 3;
+    const int full8 = log2P / 3;
+    const int rem = log2P - 3 * full8;
+    if (active) {
+        stockham_pass_regs<INV, 3>(u, t, n8, 1, table);
+    }
+    int p = 8;
+    for (int pass = 1; pass < full8; ++pass) {
+        if (pass + 1 < full8 || rem != 0) {
+            stockham_exchange<3, 3, true>(col, u, t, n8, P, p >> 3, active);
+        } else {
+            stockham_exchange<3, 3, false>(col, u, t, n8, P, p >> 3, active);
+        }
+        if (active) {
+            stockham_pass_regs<INV, 3>(u, t, n8, p, table);
+        }
+        p <<= 3;
+    }
+    if (rem == 2) {
+        stockham_exchange<3, 2, false>(col, u, t, n8, P, p >> 3, active);
+        if (active) {
+            stockham_pass_regs<INV, 2>(u, t, n8, p, table);
+        }
+        return 2;
+    }
+    if (rem == 1) {
+        stockham_exchange<3, 1, false>(col, u, t, n8, P, p >> 3, active);
+        if (active) {
+            stockham_pass_regs<INV, 1>(u, t, n8, p, table);
+        }
+        return 1;
+    }
+    return 3;
+}
+
+// Which multiple of n8 past t value b holds after a last pass of radix 2^lr (a permutation of
+// 0..7): (b / R) + (b mod R) (8 / R).
+__device__ __forceinline__ int stockham_order(int lr, int b)
+{
+    return (b >> lr) + (b & ((1 << lr) - 1)) * (8 >> lr);
+}
+
 // Column pass over one matrix: workgroup handles columns [c0, c0 + C). `load(j1, j2)` gives
 // the input at n = j1 + n1 j2; `store(j1, k2, v)` takes the twiddled result. C n2 / 8 lanes
 // work (a workgroup may have more: a tile narrower than 2^TILE_LOG2 leaves the rest idle, but
-// every lane reaches the barriers) and every working lane moves 8 elements, all 8 loads in
-// flight before the first LDS write and all 8 twiddle lookups before the first store.
+// every lane reaches the barriers). Lane tid takes column tid mod C and rows t + (n2 / 8) r
+// (t = tid / C), so a load or store instruction covers consecutive columns of one row and every
+// LDS access of the transform steps by the (odd) column stride across lanes; the transform runs
+// from registers to registers (stockham_column): one LDS exchange per pass after the first,
+// where staging the tile through LDS for every pass took four accesses per value more.
 template <bool INV, int TILE_LOG2 = kTwiddleLog2, class Load, class Store>
 __device__ __forceinline__ void column_pass_block(const FourStep& fs, long long c0, Load load,
                                                   Store store, float2* lds,
@@ -88,38 +204,33 @@ __device__ __forceinline__ void column_pass_block(const FourStep& fs, long long 
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     const int n2 = 1 << fs.log2n2;
+    const int n8 = n2 >> 3;
     const int lc = column_tile_log2<TILE_LOG2>(fs);
     const int C = 1 << lc;
     const int slots = column_slots(n2);
     const int lanes = (C * n2) >> 3;
     const bool working = tid < lanes;
+    const int col = tid & (C - 1);
+    const int t = tid >> lc;
+    const long long j1l = c0 + col;
     float2 v[8];
     if (working) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int e = tid + i * lanes;
-            v[i] = load(c0 + (e & (C - 1)), e >> lc);
-        }
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int e = tid + i * lanes;
-            lds[(e & (C - 1)) * slots + lds_pad(e >> lc)] = v[i];
+        for (int r = 0; r < 8; ++r) {
+            v[r] = load(j1l, t + r * n8);
         }
     }
-    __syncthreads();
-    const int f = tid >> (fs.log2n2 - 3);  // n2 / 8 lanes per column transform
-    const int t = tid & ((n2 >> 3) - 1);
-    lds_fft<INV>(lds + (f < C ? f : 0) * slots, fs.log2n2, t, f < C, table);
+    const int lr = stockham_column<INV>(lds + col * slots, v, fs.log2n2, t, working, table);
     if (!working) {
-        return;  // after the transform's last barrier
+        return;
     }
     // The lane's 8 elements share the column j1 and step k2 by n2 / 8: their four-step factors
     // W^(j1 k2) are b s^i with b = W^(j1 k2_0) and s = W^(j1 n2 / 8), two table lookups
     // instead of eight (at most four roundings more).
-    const unsigned j1 = static_cast<unsigned>(c0) + static_cast<unsigned>(tid & (C - 1));
-    const unsigned k20 = static_cast<unsigned>(tid >> lc);
+    const unsigned j1 = static_cast<unsigned>(c0) + static_cast<unsigned>(col);
+    const unsigned k20 = static_cast<unsigned>(t);
     const float2 b = four_step_twiddle32<INV>(fs, j1 * k20);
-    const float2 s1 = four_step_twiddle32<INV>(fs, j1 * static_cast<unsigned>(n2 >> 3));
+    const float2 s1 = four_step_twiddle32<INV>(fs, j1 * static_cast<unsigned>(n8));
     float2 sp[8];
     sp[0] = b;
     sp[1] = cmul(b, s1);
@@ -131,17 +242,23 @@ __device__ __forceinline__ void column_pass_block(const FourStep& fs, long long 
     sp[5] = cmul(sp[1], s4);
     sp[6] = cmul(sp[2], s4);
     sp[7] = cmul(sp[3], s4);
+    if (lr == 3) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int e = tid + i * lanes;
-        const int k2 = e >> lc;
-        const int col = e & (C - 1);
-        v[i] = cmul(lds[col * slots + lds_pad(k2)], sp[i]);
-    }
+        for (int i = 0; i < 8; ++i) {
+            store(j1l, t + i * n8, cmul(v[i], sp[i]));
+        }
+    } else if (lr == 2) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int e = tid + i * lanes;
-        store(c0 + (e & (C - 1)), e >> lc, v[i]);
+        for (int q = 0; q < 8; ++q) {
+            const int i = stockham_order(2, q);
+            store(j1l, t + i * n8, cmul(v[q], sp[i]));
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int i = stockham_order(1, q);
+            store(j1l, t + i * n8, cmul(v[q], sp[i]));
+        }
     }
 }
 

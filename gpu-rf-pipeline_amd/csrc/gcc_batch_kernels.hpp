@@ -138,6 +138,7 @@ __global__ __launch_bounds__(kItemThreads) void batch_column_inv_peak_kernel(
         const int L = 1 << d.log2L;
         const int log2n1 = d.log2n1;
         const int n2 = 1 << d.log2n2;
+        const int n8 = n2 >> 3;
         const int lc = column_tile_log2(fs);
         const int C = 1 << lc;
         const int slots = column_slots(n2);
@@ -145,32 +146,26 @@ __global__ __launch_bounds__(kItemThreads) void batch_column_inv_peak_kernel(
         const float2* src = d.work + L;
         int tid = threadIdx.x;  // through an empty asm: no per-lane indices hoisted out of the walk
         asm volatile("" : "+v"(tid));
-        // C n2 = 4096 values, 8 per lane, all loads in flight before the LDS writes
+        // C n2 = 4096 values, 8 per lane: column c0 + tid mod C, rows t + n8 r (t = tid / C), all
+        // loads in flight before the transform, which runs from registers to registers
+        // (stockham_column, as in column_pass_block)
+        const int col = tid & (C - 1);
+        const int t = tid >> lc;
+        const int j1 = c0 + col;
         float2 v[8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int e = tid + i * kItemThreads;
-            v[i] = gload(src, (c0 + (e & (C - 1))) + ((e >> lc) << log2n1));
+        for (int r = 0; r < 8; ++r) {
+            v[r] = gload(src, j1 + ((t + r * n8) << log2n1));
         }
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int e = tid + i * kItemThreads;
-            lds[(e & (C - 1)) * slots + lds_pad(e >> lc)] = v[i];
-        }
-        __syncthreads();
-        const int f = tid >> (d.log2n2 - 3);
-        const int t = tid & ((n2 >> 3) - 1);
-        lds_fft<true>(lds + f * slots, d.log2n2, t, true, table);
+        const int lr = stockham_column<true>(lds + col * slots, v, d.log2n2, t, true, table);
         const long long lags = static_cast<long long>(d.num_a) + d.num_b - 1;
         unsigned long long best = 0ULL;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int e = tid + i * kItemThreads;
-            const int j2 = e >> lc;
-            const int col = e & (C - 1);
-            const int n = (c0 + col) + (j2 << log2n1);
+        for (int q = 0; q < 8; ++q) {
+            const int j2 = t + stockham_order(lr, q) * n8;
+            const int n = j1 + (j2 << log2n1);
             if (n < lags) {
-                const float2 val = lds[col * slots + lds_pad(j2)];
+                const float2 val = v[q];
                 const unsigned long long key = peak_key(hypotf(val.x, val.y), n);
                 best = key > best ? key : best;
                 if (d.r != nullptr) {
