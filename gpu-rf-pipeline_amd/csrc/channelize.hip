@@ -247,11 +247,34 @@ __global__ __launch_bounds__(kChanThreads) void channelize_fused_kernel(
     const int f = threadIdx.x / per;
     const int t = threadIdx.x - f * per;
     if (LOG2M == 6) {
-        lds_pass<true, 3>(lds + f * slots, 64, t, 1, f < count, table);
-        lds_pass<true, 3>(lds + f * slots, 64, t, 8, f < count, table);
-    } else {
-        lds_fft<true>(lds + f * slots, log2M, t, f < count, table);
+        // The frame's two radix-8 passes (lds_pass's arithmetic) from registers: lane t of
+        // frame f takes positions t + 8 r, runs the first pass, swaps through the frame's LDS
+        // once and runs the second, ending with outputs t + 8 b (stockham_column's order for a
+        // radix-8 last pass), which it stores itself: one LDS exchange instead of a pass's read
+        // and write each plus the final read.
+        const bool active = f < count;
+        float2* fr = lds + f * slots;
+        float2 u[8];
+        if (active) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                u[r] = fr[lds_pad(t + 8 * r)];
+            }
+            stockham_pass_regs<true, 3>(u, t, 8, 1, table);
+        }
+        __syncthreads();  // every lane has read the branch sums it transforms
+        stockham_exchange<3, 3, false>(fr, u, t, 8, 64, 1, active);
+        if (active) {
+            stockham_pass_regs<true, 3>(u, t, 8, 8, table);
+            float2* zf = out + (t0 + f) * 64;
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                gstore(zf, t + 8 * b, u[b]);
+            }
+        }
+        return;
     }
+    lds_fft<true>(lds + f * slots, log2M, t, f < count, table);
     const int total = count * M;
     float2* dst = out + t0 * M;
     if (total == kChanOutputs) {
