@@ -100,23 +100,6 @@ __device__ __forceinline__ void pair32k_fold(const float2* __restrict__ x, int n
     }
 }
 
-// W = B conj(A) scale / |B conj(A)| (0 where that is 0), the product as in weigh(), without
-// branches: with z = 2^e (zr, zi) scaled so that max(|zr|, |zi|) is in [0.5, 1), 1/|z| =
-// 2^-e rsq(zr^2 + zi^2), the hardware reciprocal square root (about 1 ulp) of a value in
-// [0.25, 2]. Equal to weigh() (hypotf and a division) within a few ulp.
-__device__ __forceinline__ float2 phat_weight(float2 fa, float2 fb, float scale)
-{
-    const float re = fb.x * fa.x + fb.y * fa.y;
-    const float im = fb.y * fa.x - fb.x * fa.y;
-    const float big = fmaxf(fabsf(re), fabsf(im));
-    const int e = __builtin_amdgcn_frexp_expf(big);
-    const float zr = ldexpf(re, -e);
-    const float zi = ldexpf(im, -e);
-    float w = ldexpf(scale * __builtin_amdgcn_rsqf(zr * zr + zi * zi), -e);
-    w = big > 0.0f ? w : 0.0f;
-    return make_float2(re * w, im * w);
-}
-
 // A (into u) and B (into v) of one half of pair d's spectra: the even bins, or the odd ones.
 // Every input load goes through opaque() here, where it is meant to be issued: the inputs are
 // never written by the kernels, so the compiler would otherwise move the loads across the

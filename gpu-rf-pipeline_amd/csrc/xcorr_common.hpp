@@ -23,18 +23,33 @@ __device__ __forceinline__ float2 padded_b(const float2* __restrict__ b, int num
     return n < num_b ? gload(b, n) : make_float2(0.0f, 0.0f);
 }
 
-// B conj(A) / L, or for GCC-PHAT (B conj(A)) * (1/L) / |B conj(A)| (0 where that is 0),
-// with the reference's operation order (src/xcorr.cu:203-233).
-__device__ __forceinline__ float2 weigh(float2 fa, float2 fb, float scale, int phat)
+// W = B conj(A) scale / |B conj(A)| (0 where that is 0; src/xcorr.cu:219-233), without
+// branches: with z = 2^e (zr, zi) scaled so that max(|zr|, |zi|) is in [0.5, 1), 1/|z| =
+// 2^-e rsq(zr^2 + zi^2), the hardware reciprocal square root (about 1 ulp) of a value in
+// [0.25, 2]: within a few ulp of the reference's hypotf and division, in a fraction of the
+// instructions. Every GCC-PHAT path of the library weighs with it.
+__device__ __forceinline__ float2 phat_weight(float2 fa, float2 fb, float scale)
 {
     const float re = fb.x * fa.x + fb.y * fa.y;
     const float im = fb.y * fa.x - fb.x * fa.y;
-    if (!phat) {
-        return make_float2(re * scale, im * scale);
-    }
-    const float mag = hypotf(re, im);
-    const float w = mag > 0.0f ? scale / mag : 0.0f;
+    const float big = fmaxf(fabsf(re), fabsf(im));
+    const int e = __builtin_amdgcn_frexp_expf(big);
+    const float zr = ldexpf(re, -e);
+    const float zi = ldexpf(im, -e);
+    float w = ldexpf(scale * __builtin_amdgcn_rsqf(zr * zr + zi * zi), -e);
+    w = big > 0.0f ? w : 0.0f;
     return make_float2(re * w, im * w);
+}
+
+// B conj(A) / L (src/xcorr.cu:203-214), or for GCC-PHAT phat_weight.
+__device__ __forceinline__ float2 weigh(float2 fa, float2 fb, float scale, int phat)
+{
+    if (phat) {
+        return phat_weight(fa, fb, scale);
+    }
+    const float re = fb.x * fa.x + fb.y * fa.y;
+    const float im = fb.y * fa.x - fb.x * fa.y;
+    return make_float2(re * scale, im * scale);
 }
 
 // Key ordering (magnitude, then the smaller index): magnitude bits (>= 0 floats order like
