@@ -166,7 +166,9 @@ __global__ __launch_bounds__(kItemThreads) void batch_column_inv_peak_kernel(
             const int n = j1 + (j2 << log2n1);
             if (n < lags) {
                 const float2 val = v[q];
-                const unsigned long long key = peak_key(hypotf(val.x, val.y), n);
+                // |c|^2 orders the lags like |c| (ties aside), as in the 2^15 kernel; the square
+                // root is taken for the peak only (batch_finish_kernel)
+                const unsigned long long key = peak_key(val.x * val.x + val.y * val.y, n);
                 best = key > best ? key : best;
                 if (d.r != nullptr) {
                     gstore(d.r, n, val);
@@ -234,7 +236,7 @@ __global__ __launch_bounds__(64) void batch_finish_kernel(const PairDesc* __rest
     const unsigned long long key = *reinterpret_cast<const unsigned long long*>(res);
     const long long lags = static_cast<long long>(d.num_a) + d.num_b - 1;
     const int idx = static_cast<int>(0xFFFFFFFFu - static_cast<unsigned>(key & 0xFFFFFFFFULL));
-    const float mag = __uint_as_float(static_cast<unsigned>(key >> 32));
+    const float mag = sqrtf(__uint_as_float(static_cast<unsigned>(key >> 32)));  // key: |c|^2
     double frac = 0.0;
     if (idx > 0 && idx + 1 < lags) {
         const double ml = lag_magnitude(d, fs, idx - 1);
