@@ -336,18 +336,43 @@ __global__ void search_decide_kernel(const float* __restrict__ probs,
     }
 }
 
+// Radix-8 butterfly of the second pass of a 64-point inverse transform (span 8), as
+// twiddle_and_dft computes it, with the factors w1, w2, w4 (already conjugated) in registers.
+__device__ __forceinline__ void refine_radix8(float2* u, int k, float2 w1, float2 w2, float2 w4)
+{
+    if (k != 0) {
+        u[1] = cmul(u[1], w1);
+        const float2 w3 = cmul(w1, w2);
+        u[2] = cmul(u[2], w2);
+        u[3] = cmul(u[3], w3);
+        u[4] = cmul(u[4], w4);
+        u[5] = cmul(u[5], cmul(w1, w4));
+        u[6] = cmul(u[6], cmul(w2, w4));
+        u[7] = cmul(u[7], cmul(w3, w4));
+    }
+    dft8<true>(u);
+}
+
 // Exact GCC-PHAT over the 64-lag window of a refined pair; one workgroup per pair.
 // With k = p + R q (p < R = L / 64, q < 64), w^(R q n) = exp(2 pi i q n / 64), so
 //   r(n) = sum_k W[k] w^(k n) = sum_p w^(p n) U_p[n mod 64],  U_p[j] = sum_q W[p + R q] e^(2 pi i q j / 64).
 // W sits in four-step order (bin k at (k mod n2) 4096 + k / n2) and R is a multiple of n2, so
 // the residues p = n2 s + c (s < 64) of row c of W are its 4096 values viewed as a 64 x 64
-// matrix [q][s]: each row is read once, contiguously, transformed as 64 columns of 64 points in
-// LDS, and every lane accumulates one window lag over 16 of the columns.
+// matrix [q][s]: each row is read once, contiguously, transformed as 64 columns of 64 points,
+// and every lane accumulates one window lag over 16 of the columns.
+// Lane t holds column s = t mod 64 at rows q = g + 4 i (g = t / 64, i < 16): the inputs of the
+// two first-pass butterflies g and g + 4 of that column's radix-8 x radix-8 transform (the
+// passes of lds_fft, same arithmetic), so the first pass runs in registers, one LDS exchange
+// feeds the second (its factors held in registers), whose outputs go to LDS once for the
+// accumulation. The next row's loads are issued before the current row's LDS work and every
+// barrier of the row orders LDS only, so they stay in flight while the row is transformed.
 __global__ __launch_bounds__(kRefineThreads) void search_refine_kernel(
     const PairDesc* __restrict__ pairs, rf_delay_result* __restrict__ results,
     int* __restrict__ active, int* __restrict__ fallback, const float2* __restrict__ table)
 {
     constexpr int slots = column_slots(kWindow);  // odd stride between the 64 columns
+    constexpr int kPerLane = k4096 / kRefineThreads;
+    static_assert(kRefineThreads == 256 && kPerLane == 16, "4 lanes per column, 16 rows each");
     __shared__ float2 lds[kWindow * slots];
     __shared__ float2 part[4][kWindow];
     __shared__ float mag[kWindow];
@@ -370,30 +395,72 @@ __global__ __launch_bounds__(kRefineThreads) void search_refine_kernel(
     const long long n = (n0 + m) & (L - 1);
     const int j = static_cast<int>(n & (kWindow - 1));
     const float2 step = four_step_twiddle<true>(fs, (n << log2n2) & (L - 1));  // w^(n2 n)
+    // second-pass factors of butterflies g and g + 4: conj(table[64 k]), conj(table[128 k]),
+    // conj(table[256 k]) for k = g, g + 4
+    auto tw = [&](int idx) {
+        const float2 w = table[idx];
+        return make_float2(w.x, -w.y);
+    };
+    const float2 a1 = tw(64 * g), a2 = tw(128 * g), a4 = tw(256 * g);
+    const float2 b1 = tw(64 * (g + 4)), b2 = tw(128 * (g + 4)), b4 = tw(256 * (g + 4));
+    float2* col = lds + m * slots;   // this lane's column s = m
     float2 acc = make_float2(0.0f, 0.0f);
+    float2 v[kPerLane];
+#pragma unroll
+    for (int i = 0; i < kPerLane; ++i) {
+        v[i] = gload(W, t + i * kRefineThreads);
+    }
     for (int c = 0; c < n2; ++c) {
-        const float2* row = W + (static_cast<long long>(c) << k4096Log2);
-        float2 v[k4096 / kRefineThreads];
+        // first pass in registers: butterfly g takes rows g + 8 r (v[2 r]), g + 4 rows
+        // g + 4 + 8 r (v[2 r + 1])
+        float2 ua[8];
+        float2 ub[8];
 #pragma unroll
-        for (int i = 0; i < k4096 / kRefineThreads; ++i) {
-            v[i] = gload(row, t + i * kRefineThreads);
+        for (int r = 0; r < 8; ++r) {
+            ua[r] = v[2 * r];
+            ub[r] = v[2 * r + 1];
         }
-#pragma unroll
-        for (int i = 0; i < k4096 / kRefineThreads; ++i) {
-            const int e = t + i * kRefineThreads;                     // (q, s) = (e/64, e%64)
-            lds[(e & (kWindow - 1)) * slots + lds_pad(e >> 6)] = v[i];
-        }
-        __syncthreads();
-        lds_fft<true>(lds + (t >> 3) * slots, 6, t & 7, true, table);
-        lds_fft<true>(lds + (kRefineBatch + (t >> 3)) * slots, 6, t & 7, true, table);
+        dft8<true>(ua);
+        dft8<true>(ub);
         const long long p0 = (static_cast<long long>(16 * g) << log2n2) + c;
         float2 w = four_step_twiddle<true>(fs, (p0 * n) & (L - 1));  // w^(p n), p = n2 s + c
+        {
+            // unconditional (the last row reloads itself), so that the wait for w below can
+            // leave these 16 loads in flight on every path
+            const int cn = c + 1 < n2 ? c + 1 : c;
+            const float2* next = W + (static_cast<long long>(cn) << k4096Log2);
+#pragma unroll
+            for (int i = 0; i < kPerLane; ++i) {
+                v[i] = gload(next, t + i * kRefineThreads);
+            }
+        }
+        // exchange: first-pass outputs at 8 k + r, second-pass inputs k + 8 r (k = g, g + 4)
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            col[lds_pad(8 * g + r)] = ua[r];
+            col[lds_pad(8 * (g + 4) + r)] = ub[r];
+        }
+        lds_barrier();
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            ua[r] = col[lds_pad(g + 8 * r)];
+            ub[r] = col[lds_pad(g + 4 + 8 * r)];
+        }
+        refine_radix8(ua, g, a1, a2, a4);
+        refine_radix8(ub, g + 4, b1, b2, b4);
+        // the outputs k + 8 r overwrite exactly this lane's inputs
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            col[lds_pad(g + 8 * r)] = ua[r];
+            col[lds_pad(g + 4 + 8 * r)] = ub[r];
+        }
+        lds_barrier();
 #pragma unroll 4
         for (int s = 16 * g; s < 16 * g + 16; ++s) {
             acc = cadd(acc, cmul(lds[s * slots + lds_pad(j)], w));
             w = cmul(w, step);
         }
-        __syncthreads();
+        lds_barrier();
     }
     part[g][m] = acc;
     __syncthreads();

@@ -283,6 +283,16 @@ __device__ __forceinline__ void twiddle_and_dft(float2* u, int lr, int k,
     }
 }
 
+// Workgroup barrier that orders LDS accesses only: unlike __syncthreads() it does not wait for
+// the lanes' outstanding global loads, so a load issued for later work stays in flight across
+// it.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // One pass of radix R = 2^LR (compile-time, so the eight values stay in registers): each
 // thread reads eight values (one radix-8, two radix-4 or four radix-2 butterflies), waits at a
 // barrier, and writes its results.
