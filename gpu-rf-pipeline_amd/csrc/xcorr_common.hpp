@@ -80,15 +80,20 @@ __device__ __forceinline__ void xcorr_row4096(const float2* __restrict__ wa,
         ua[r] = gload(wa, t + r * k4096Lanes);
         ub[r] = gload(wb, t + r * k4096Lanes);
     }
-    fft4096_staged<false>(buf, t, ua, stages);
+    // the 30 factors of the three transforms loaded once and held (168 VGPRs in the batched
+    // row pass, still three waves per SIMD): measured a little faster than loading each pass's
+    // factors before the pass (fft4096_staged)
+    Fft4096Twiddles tw;
+    fft4096_load_twiddles(t, stages, tw);
+    fft4096<false>(buf, t, ua, tw);
     __syncthreads();  // every lane has read a's last pass from buf
-    fft4096_staged<false>(buf, t, ub, stages);
+    fft4096<false>(buf, t, ub, tw);
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         ub[r] = weigh(ua[r], ub[r], scale, phat);
     }
     __syncthreads();
-    fft4096_staged<true>(buf, t, ub, stages);
+    fft4096<true>(buf, t, ub, tw);
     // w_L^-(j1 row), j1 = t + 256 r, as b s^r with b = w_L^(t row) and s = w_L^(256 row) (the
     // same for every lane): four table lookups instead of 32, s^2, s^4, s^8 by squaring and
     // each factor one product from an earlier one (at most five roundings deep). j1 row < L <=
