@@ -133,21 +133,6 @@ __device__ __forceinline__ void fft4096(float2* buf, int t, float2* u, const Fft
     fft4096_last<INV>(buf, t, u, tw.w256);
 }
 
-// The same with each pass's 15 factors loaded just before the pass that needs them: 30 fewer
-// registers held than with all factors preloaded, for kernels that keep a second 4096-point
-// operand in registers.
-template <bool INV>
-__device__ __forceinline__ void fft4096_staged(float2* buf, int t, float2* u,
-                                               const float2* stages)
-{
-    float2 w[16];
-    load_factors(stages + 16 * (t & 15), w);
-    fft4096_first<INV>(buf, t, u);
-    fft4096_middle<INV>(buf, t, w);
-    load_factors(stages + 256 + 16 * t, w);
-    fft4096_last<INV>(buf, t, u, w);
-}
-
 // The same, loading the factors first (they arrive while the first pass computes).
 template <bool INV>
 __device__ __forceinline__ void fft4096(float2* buf, int t, float2* u, const float2* stages)

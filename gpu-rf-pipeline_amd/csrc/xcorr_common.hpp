@@ -82,7 +82,7 @@ __device__ __forceinline__ void xcorr_row4096(const float2* __restrict__ wa,
     }
     // the 30 factors of the three transforms loaded once and held (168 VGPRs in the batched
     // row pass, still three waves per SIMD): measured a little faster than loading each pass's
-    // factors before the pass (fft4096_staged)
+    // 15 factors just before the pass
     Fft4096Twiddles tw;
     fft4096_load_twiddles(t, stages, tw);
     fft4096<false>(buf, t, ua, tw);
