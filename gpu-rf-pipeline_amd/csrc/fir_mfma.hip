@@ -184,44 +184,74 @@ __global__ __launch_bounds__(kMfmaThreads) void fir_mfma_kernel(
     const long long n0 = static_cast<long long>(blockIdx.x) * kMfmaOutputs;
     const long long x0 = n0 - kp + 32;
 
-    // Samples: every lane loads its (at most kTileLoads) tile samples at once, then the block
-    // max for the FP16 scale, then the split tile.
-    float2 xs[kTileLoads];
-    float m = 0.0f;
-#pragma unroll
-    for (int j = 0; j < kTileLoads; ++j) {
-        const int u = threadIdx.x + j * kMfmaThreads;
+    // Samples and taps go to LDS in chunks of kChunk loads per lane, so that the prologue holds
+    // at most kChunk values in registers: with all of a lane's loads in flight at once it took
+    // more registers than the matrix-core loop (228 VGPRs and 64 AGPRs, one wave per SIMD).
+    // FP16 first needs the block's max |x| for its scale: one pass over the samples for the
+    // max, a second (L2 hits) to split them; BF16 splits in the first pass.
+    constexpr int kChunk = 8;
+    static_assert(kTileLoads % kChunk == 0, "whole chunks of tile loads");
+    auto sample = [&](int u) {
         const long long s = x0 + u;
-        xs[j] = (u < U && s >= 0 && s < n) ? x[s] : make_float2(0.0f, 0.0f);
-        m = fmaxf(m, fmaxf(fabsf(xs[j].x), fabsf(xs[j].y)));
+        return (u < U && s >= 0 && s < n) ? x[s] : make_float2(0.0f, 0.0f);
+    };
+    auto put = [&](int u, float2 v, int ex) {
+        if (u < U) {
+            const int slot = tile_slot(u);
+            Split<BF16>::split(ldexpf(v.x, ex), rh[slot], rl[slot]);
+            Split<BF16>::split(ldexpf(v.y, ex), ih[slot], il[slot]);
+        }
+    };
+    int ex = 0;
+    if (!BF16) {
+        float m = 0.0f;
+#pragma unroll 1
+        for (int c = 0; c < kTileLoads; c += kChunk) {
+            float2 xs[kChunk];
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j) {
+                xs[j] = sample(threadIdx.x + (c + j) * kMfmaThreads);
+            }
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j) {
+                m = fmaxf(m, fmaxf(fabsf(xs[j].x), fabsf(xs[j].y)));
+            }
+        }
+        ex = fp16_scale_exponent(block_max(m, scratch));
     }
-    // Taps: 16-byte copies of the hi/lo tables (16 len 16-bit values), all in flight.
+#pragma unroll 1
+    for (int c = 0; c < kTileLoads; c += kChunk) {
+        float2 xs[kChunk];
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+            xs[j] = sample(threadIdx.x + (c + j) * kMfmaThreads);
+        }
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+            put(threadIdx.x + (c + j) * kMfmaThreads, xs[j], ex);
+        }
+    }
+    // Taps: 16-byte copies of the hi/lo tables (16 len 16-bit values); th and tl are
+    // contiguous, as in the workspace.
     const int vecs = 2 * len;  // 16 len values / 8 per 16-byte vector
     const u32x4* wsv = reinterpret_cast<const u32x4*>(ws);
     u32x4* thv = reinterpret_cast<u32x4*>(th);
-    u32x4 tv[kTapLoads];
+#pragma unroll 1
+    for (int c = 0; c < kTapLoads; c += kChunk) {
+        u32x4 tv[kChunk];
 #pragma unroll
-    for (int j = 0; j < kTapLoads; ++j) {
-        const int idx = threadIdx.x + j * kMfmaThreads;
-        if (idx < vecs) {
-            tv[j] = wsv[idx];
+        for (int j = 0; j < kChunk; ++j) {
+            const int idx = threadIdx.x + (c + j) * kMfmaThreads;
+            if (c + j < kTapLoads && idx < vecs) {
+                tv[j] = wsv[idx];
+            }
         }
-    }
-    const int ex = BF16 ? 0 : fp16_scale_exponent(block_max(m, scratch));
 #pragma unroll
-    for (int j = 0; j < kTileLoads; ++j) {
-        const int u = threadIdx.x + j * kMfmaThreads;
-        if (u < U) {
-            const int slot = tile_slot(u);
-            Split<BF16>::split(ldexpf(xs[j].x, ex), rh[slot], rl[slot]);
-            Split<BF16>::split(ldexpf(xs[j].y, ex), ih[slot], il[slot]);
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < kTapLoads; ++j) {
-        const int idx = threadIdx.x + j * kMfmaThreads;
-        if (idx < vecs) {
-            thv[idx] = tv[j];  // th and tl are contiguous, as in the workspace
+        for (int j = 0; j < kChunk; ++j) {
+            const int idx = threadIdx.x + (c + j) * kMfmaThreads;
+            if (c + j < kTapLoads && idx < vecs) {
+                thv[idx] = tv[j];
+            }
         }
     }
     __syncthreads();
