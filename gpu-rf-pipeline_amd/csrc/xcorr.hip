@@ -282,32 +282,34 @@ __global__ __launch_bounds__(512) void xcorr_column_inv_kernel(const float2* __r
                                                                const float2* __restrict__ table)
 {
     // The conjugate twiddles were applied by the row kernel, so this pass is a plain column
-    // FFT (unlike column_pass_block, which twiddles after the transform).
+    // FFT (unlike column_pass_block, which twiddles after the transform), run from registers
+    // to registers as there (stockham_column): the launch has C n2 / 8 lanes, lane tid takes
+    // column tid mod C and rows t + (n2 / 8) r, and stores its outputs itself.
     extern __shared__ float2 lds[];
     const long long L = 1LL << fs.log2L;
     const int log2n1 = fs.log2n1;
     const float2* src = work + L;
     const long long c0 = static_cast<long long>(blockIdx.x) * column_tile(fs);
     const int n2 = 1 << fs.log2n2;
+    const int n8 = n2 >> 3;
     const int lc = column_tile_log2(fs);
     const int C = 1 << lc;
     const int slots = column_slots(n2);
-    const int total = C * n2;
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        const int row = e >> lc;
-        const int col = e & (C - 1);
-        lds[col * slots + lds_pad(row)] = src[(c0 + col) + (static_cast<long long>(row) << log2n1)];
+    const int col = threadIdx.x & (C - 1);
+    const int t = threadIdx.x >> lc;
+    const long long j1 = c0 + col;
+    float2 v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        v[q] = src[j1 + (static_cast<long long>(t + q * n8) << log2n1)];
     }
-    __syncthreads();
-    const int f = threadIdx.x >> (fs.log2n2 - 3);
-    const int t = threadIdx.x & ((n2 >> 3) - 1);
-    lds_fft<true>(lds + (f < C ? f : 0) * slots, fs.log2n2, t, f < C, table);
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        const int j2 = e >> lc;
-        const int col = e & (C - 1);
-        const long long n = (c0 + col) + (static_cast<long long>(j2) << log2n1);
+    const int lr = stockham_column<true>(lds + col * slots, v, fs.log2n2, t, true, table);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int j2 = t + stockham_order(lr, q) * n8;
+        const long long n = j1 + (static_cast<long long>(j2) << log2n1);
         if (n < lags) {
-            r[n] = lds[col * slots + lds_pad(j2)];
+            r[n] = v[q];
         }
     }
 }
