@@ -214,11 +214,14 @@ def bench_fir(rf, args, long_filter: bool) -> dict:
 
     # Arithmetic: the direct form costs 4 flop per tap and output (complex x real MAC); the
     # overlap-save form two 4096-point FFTs (5 P log2 P each) and 4096 complex products (6 flop)
-    # per segment of 4097 - T outputs.
+    # per segment of 4097 - T outputs (512-point transforms and 513 - T outputs for 17..225 taps).
     direct_flop = 4 * n * t
-    if args.impl == "fft_ola" and 65 < t <= 2049:
+    if args.impl == "fft_ola" and 225 < t <= 2049:
         segments = -(-n // (4097 - t))
         algo_flop = segments * (2 * 5 * 4096 * 12 + 6 * 4096)
+    elif args.impl == "fft_ola" and 17 <= t <= 225:
+        segments = -(-n // (513 - t))   # the 512-point form for short filters
+        algo_flop = segments * (2 * 5 * 512 * 9 + 6 * 512)
     else:
         algo_flop = direct_flop
     compute = {"algorithm_flop_per_step": algo_flop, "direct_form_flop_per_step": direct_flop,

@@ -14,9 +14,11 @@
 //   - the register variant keeps R = 4, 8 or 16 outputs per lane and walks the taps R at a
 //     time, reading one 8R-byte window per R taps from an LDS tile padded by 16 bytes per
 //     R samples (conflict-free 16-byte LDS reads).
-// fft_ola is new: overlap-save with a 4096-point LDS FFT for long filters.
+// fft_ola is new: overlap-save with a 4096-point transform for long filters and a 512-point
+// one held by a single wavefront for short ones.
 
 #include "fft4096.hpp"
+#include "fft512.hpp"
 #include "fft_core.hpp"
 #include "fir_internal.hpp"
 #include "rf_common.hpp"
@@ -311,7 +313,7 @@ __global__ void fir_register_batched_kernel(FirBatch batch, const float* __restr
 // and the inverse transform in registers, and the inverse transform writes y straight back.
 // The tap spectrum is made once per call by one workgroup with the same transform.
 constexpr int kOlaP = k4096;
-constexpr int kOlaMinTaps = 65;   // at or below this, the direct register kernel is cheaper
+constexpr int kOlaMinTaps = 65;   // the matrix-core variants start above this
 constexpr int kOlaMaxTaps = 2049; // keeps Q >= P / 2
 
 // Tap spectrum, scaled by 1/P (exact: a power of two): one workgroup, the register-resident
@@ -550,6 +552,195 @@ __global__ void fir_register_batch_kernel(const FirCapture* __restrict__ caps,
     });
 }
 
+// --- fft_ola for short filters ----------------------------------------------------------------
+
+// Overlap-save as above with a 512-point transform held by one wavefront (fft512.hpp), for
+// kOla512MinTaps <= T <= kOla512MaxTaps: Q = 513 - T outputs per segment (288 at 225 taps).
+// The direct register kernel costs 2 T FMA per output component; this form about 60 VALU
+// instructions per output at 65 taps (two transforms of three radix-8 passes and one product
+// per bin, over Q outputs), against about 110 for the 4096-point form, which overtakes it
+// only where Q has shrunk to about half a segment (measured on 2^24 samples: 63.6 against
+// 76.4 us at 66 taps, 62.4 against 66.7 at 193, 74.7 against 66.6 at 257). A workgroup is one
+// wave (64 lanes), so a segment's barriers hold only its own wave. Each persistent wave makes the tap spectrum itself before its first segment (one
+// transform more per wave, where a separate spectrum launch cost about 2.6 us per call on
+// 2^20-sample captures), so every call is one launch and needs no device memory beyond its
+// inputs and outputs; the wave then keeps its 14 transform factors and 8 spectrum bins in
+// registers across its segments.
+constexpr int kOla512P = k512;
+constexpr int kOla512MinTaps = 17;
+constexpr int kOla512MaxTaps = 225;
+constexpr int kOla512WavesPerCu = 16;
+
+// What a lane keeps across the segments it filters.
+struct Ola512Lane {
+    Fft512Twiddles tw;
+    float2 h[8];  // tap spectrum H[t + 64 r], H = DFT_512(h / 512)
+};
+
+// The lane's factors and spectrum bins; ends with a barrier (the transform used buf).
+__device__ __forceinline__ void ola512_lane_setup(const float* __restrict__ taps, int num_taps,
+                                                  const float2* __restrict__ table, int t,
+                                                  float2* buf, Ola512Lane& s)
+{
+    fft512_load_twiddles(t, table, s.tw);
+    const float scale = 1.0f / static_cast<float>(kOla512P);  // exact: a power of two
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int k = t + r * k512Lanes;
+        s.h[r] = make_float2(k < num_taps ? taps[k] * scale : 0.0f, 0.0f);
+    }
+    fft512<false>(buf, t, s.h, s.tw);
+    lds_barrier();
+}
+
+// A segment: its capture and the index of its first output; seg[i] = x[out0 - halo + i],
+// i < 512 (zero outside the input), gives y[out0 + i - halo] for halo <= i < 512.
+struct Ola512Seg {
+    const float2* x;
+    float2* y;
+    int n;
+    long long out0;
+};
+
+__device__ __forceinline__ void ola512_load(const Ola512Seg& g, int halo, int t, float2 (&u)[8])
+{
+    const long long in0 = g.out0 - halo;
+    if (in0 >= 0 && in0 + kOla512P <= g.n) {
+        const __amdgpu_buffer_rsrc_t src = unchecked_buffer(g.x + in0);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            u[r] = buffer_load2(src, static_cast<unsigned>(t) * 8u,
+                                static_cast<unsigned>(r * k512Lanes) * 8u);
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            u[r] = load_sample(g.x, in0 + t + r * k512Lanes, g.n);
+        }
+    }
+}
+
+// Filters the loaded segment u and stores its outputs; ends with a barrier before buf is
+// written again.
+__device__ __forceinline__ void ola512_filter(const Ola512Seg& g, int halo, float2 (&u)[8],
+                                              float2* buf, const Ola512Lane& s, int t)
+{
+    fft512<false>(buf, t, u, s.tw);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        u[r] = cmul(u[r], s.h[r]);
+    }
+    lds_barrier();  // every lane has read the forward transform's last exchange
+    fft512<true>(buf, t, u, s.tw);
+    const unsigned lane_bytes = static_cast<unsigned>(t) * 8u;
+    if (g.out0 + kOla512P - halo <= g.n) {
+        // position p = t + 64 r is an output when p >= halo: blocks wholly above the halo store
+        // without a per-lane test
+        const __amdgpu_buffer_rsrc_t dst = unchecked_buffer(g.y + g.out0);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int first = r * k512Lanes;
+            if (first >= halo) {
+                buffer_store2(u[r], dst, lane_bytes, static_cast<unsigned>(first - halo) * 8u);
+            } else if (first + k512Lanes > halo && t + first >= halo) {
+                buffer_store2(u[r], dst, static_cast<unsigned>(t + first - halo) * 8u, 0u);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int pos = t + r * k512Lanes;
+            const long long o = g.out0 + pos - halo;
+            if (pos >= halo && o < g.n) {
+                g.y[o] = u[r];
+            }
+        }
+    }
+    lds_barrier();
+}
+
+// A persistent wave's walk over items 0 .. total - 1 (XCD-major, as xcd_major_items), with
+// seg_of(item) giving each item's segment. The samples of the wave's next segment are loaded
+// before the current one is transformed (and the first segment's before the tap spectrum), so
+// the wave keeps a segment's loads in flight behind its transforms.
+template <class SegOf>
+__device__ __forceinline__ void ola512_walk(int total, const float* __restrict__ taps,
+                                            int num_taps, const float2* __restrict__ table,
+                                            float2* buf, SegOf seg_of)
+{
+    const int per = (total + 7) >> 3;
+    const int lo = (blockIdx.x & 7) * per;
+    const int hi = lo + per < total ? lo + per : total;
+    const int step = gridDim.x >> 3;
+    int item = lo + static_cast<int>(blockIdx.x >> 3);
+    if (item >= hi) {
+        return;  // the whole wave
+    }
+    const int halo = num_taps - 1;
+    int t = threadIdx.x;
+    Ola512Seg g = seg_of(item);
+    float2 next[8];
+    ola512_load(g, halo, t, next);
+    Ola512Lane s;
+    ola512_lane_setup(taps, num_taps, table, t, buf, s);
+    while (item < hi) {
+        // the lane index through an empty asm: keeps per-lane addresses out of held registers
+        asm volatile("" : "+v"(t));
+        float2 u[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            u[r] = next[r];
+        }
+        const Ola512Seg cur = g;
+        item += step;
+        if (item < hi) {
+            g = seg_of(item);
+            ola512_load(g, halo, t, next);
+        }
+        ola512_filter(cur, halo, u, buf, s, t);
+    }
+}
+
+__global__ __launch_bounds__(k512Lanes) void fir_ola512_kernel(
+    const float2* __restrict__ x, const float* __restrict__ taps, float2* __restrict__ y, int n,
+    int num_taps, int segments, const float2* __restrict__ table)
+{
+    __shared__ float2 buf[k512Slots];
+    const long long q = kOla512P - (num_taps - 1);
+    ola512_walk(segments, taps, num_taps, table, buf,
+                [&](int seg) { return Ola512Seg{x, y, n, seg * q}; });
+}
+
+// rf_fir_batched: channel blockIdx.y, persistent over its segments.
+__global__ __launch_bounds__(k512Lanes) void fir_ola512_batched_kernel(
+    FirBatch batch, const float* __restrict__ taps, int num_taps,
+    const float2* __restrict__ table)
+{
+    __shared__ float2 buf[k512Slots];
+    const int c = blockIdx.y;
+    const int n = batch.n[c];
+    const int q = kOla512P - (num_taps - 1);
+    const float2* x = batch.x[c];
+    float2* y = batch.y[c];
+    ola512_walk((n + q - 1) / q, taps, num_taps, table, buf, [&](int seg) {
+        return Ola512Seg{x, y, n, static_cast<long long>(seg) * q};
+    });
+}
+
+// rf_fir_batch: the {capture, segment} items of all captures.
+__global__ __launch_bounds__(k512Lanes) void fir_ola512_batch_kernel(
+    const FirCapture* __restrict__ caps, const int2* __restrict__ items, int total,
+    const float* __restrict__ taps, int num_taps, const float2* __restrict__ table)
+{
+    __shared__ float2 buf[k512Slots];
+    const long long q = kOla512P - (num_taps - 1);
+    ola512_walk(total, taps, num_taps, table, buf, [&](int item) {
+        const int2 it = items[item];
+        const FirCapture c = caps[it.x];
+        return Ola512Seg{c.x, c.y, c.n, it.y * q};
+    });
+}
+
 // --- host side ------------------------------------------------------------------------------
 
 struct ImplInfo {
@@ -562,8 +753,8 @@ const ImplInfo kFirImpls[RF_FIR_IMPL_COUNT] = {
     {"tiled", "samples staged in an LDS tile with a halo"},
     {"tiled_const", "tiled, with the taps read through the scalar (constant) cache"},
     {"register", "4, 8 or 16 outputs per lane from a register window over a padded LDS tile"},
-    {"fft_ola", "overlap-save with a 4096-point LDS FFT for 66..2049 taps; "
-                "the register kernel outside that range"},
+    {"fft_ola", "overlap-save with a 512-point transform per wavefront for 17..225 taps and a "
+                "4096-point one for 226..2049 taps; the register kernel outside those ranges"},
     {"mfma_f16x3", "Toeplitz GEMM on FP16 matrix cores, hi/lo split, FP32 accumulate, for "
                    "66..2049 taps; the register kernel outside that range"},
     {"mfma_bf16x3", "Toeplitz GEMM on BF16 matrix cores, hi/lo split, FP32 accumulate, for "
@@ -682,9 +873,11 @@ void launch_register(const float2* x, const float* taps, float2* y, int n, int n
     check_launch("fir_register_kernel");
 }
 
+// The 4096-point overlap-save: from where the 512-point form stops (kOla512MaxTaps, where the
+// two measured equal on 2^24 samples) up to kOlaMaxTaps.
 bool ola_applies(int num_taps)
 {
-    return num_taps > kOlaMinTaps && num_taps <= kOlaMaxTaps;
+    return num_taps > kOla512MaxTaps && num_taps <= kOlaMaxTaps;
 }
 
 // The matrix-core variants cover the same tap range as overlap-save; shorter filters run the
@@ -692,6 +885,40 @@ bool ola_applies(int num_taps)
 bool mfma_applies(int num_taps)
 {
     return num_taps > kOlaMinTaps && num_taps <= kOlaMaxTaps;
+}
+
+// Short filters: the 512-point overlap-save (a single tap stays direct, and exact; below
+// kOla512MinTaps the direct form is cheaper).
+bool ola512_applies(int num_taps)
+{
+    return num_taps >= kOla512MinTaps && num_taps <= kOla512MaxTaps;
+}
+
+// Points of the overlap-save transform fft_ola uses for num_taps taps: 4096, 512, or 0 (the
+// register kernel).
+int ola_points(int num_taps)
+{
+    return ola_applies(num_taps) ? kOlaP : (ola512_applies(num_taps) ? kOla512P : 0);
+}
+
+// Persistent one-wave workgroups: kOla512WavesPerCu per CU, never more than the items,
+// rounded up to a multiple of 8 for the XCD-major walk.
+unsigned ola512_grid(long long items)
+{
+    const long long cap = static_cast<long long>(kOla512WavesPerCu) * cu_count();
+    return static_cast<unsigned>(ceil_div(std::min<long long>(std::max<long long>(items, 1), cap),
+                                          8LL) * 8);
+}
+
+// The whole short-filter overlap-save in one launch (the waves make the tap spectrum).
+void launch_ola512(const float2* x, const float* taps, float2* y, int n, int num_taps,
+                   hipStream_t stream)
+{
+    const long long q = kOla512P - (num_taps - 1);
+    const long long segments = ceil_div(n, q);
+    fir_ola512_kernel<<<ola512_grid(segments), k512Lanes, 0, stream>>>(
+        x, taps, y, n, num_taps, static_cast<int>(segments), twiddle_table_4096());
+    check_launch("fir_ola512_kernel");
 }
 
 void launch_ola_taps(const float* taps, int num_taps, float2* spec, hipStream_t stream)
@@ -712,6 +939,7 @@ void launch_ola(const float2* x, const float2* spec, float2* y, int n, int num_t
         x, spec, y, n, num_taps, segments, per_xcd, fft4096_stage_table());
     check_launch("fir_ola_kernel");
 }
+
 
 // `workspace`: at least kStreamWorkspaceBytes of device memory for the tap spectrum / split
 // taps of fft_ola and the matrix-core variants, or nullptr for the stream's own block.
@@ -747,6 +975,10 @@ void fir_launch(int impl, const float2* x, const float* taps, float2* y, int n, 
         launch_register(x, taps, y, n, num_taps, block, stream);
         return;
     case RF_FIR_FFT_OLA: {
+        if (ola512_applies(num_taps)) {
+            launch_ola512(x, taps, y, n, num_taps, stream);
+            return;
+        }
         if (!ola_applies(num_taps)) {
             launch_register(x, taps, y, n, num_taps, block, stream);
             return;
@@ -818,6 +1050,14 @@ void fir_batched_launch(int impl, const FirBatch& batch, int count, int longest,
             fir_ola_batched_kernel<<<grid, k4096Lanes, 0, stream>>>(
                 batch, spec, num_taps, fft4096_stage_table());
             check_launch("fir_ola_batched_kernel");
+            return;
+        }
+        if (impl == RF_FIR_FFT_OLA && ola512_applies(num_taps)) {
+            const long long q = kOla512P - (num_taps - 1);
+            const dim3 grid(ola512_grid(ceil_div(longest, q)), count);
+            fir_ola512_batched_kernel<<<grid, k512Lanes, 0, stream>>>(batch, taps, num_taps,
+                                                                       twiddle_table_4096());
+            check_launch("fir_ola512_batched_kernel");
             return;
         }
         if (impl == RF_FIR_REGISTER) {
@@ -943,7 +1183,9 @@ const char* rf_fir_kernel_name(int impl, int num_taps)
     case RF_FIR_TILED_CONST:
         return "fir_tiled_kernel";
     case RF_FIR_FFT_OLA:
-        return rf::ola_applies(num_taps) ? "fir_ola_kernel" : "fir_register_kernel";
+        return rf::ola_applies(num_taps)
+                   ? "fir_ola_kernel"
+                   : (rf::ola512_applies(num_taps) ? "fir_ola512_kernel" : "fir_register_kernel");
     case RF_FIR_MFMA_F16X3:
     case RF_FIR_MFMA_BF16X3:
         return rf::mfma_applies(num_taps) ? "fir_mfma_kernel" : "fir_register_kernel";
@@ -1119,9 +1361,9 @@ rf_status rf_fir_batched(int impl, const rf_cf32* const* x, rf_cf32* const* y,
 
 }  // extern "C"
 
-// A filter prepared once for many calls: device copies of the taps and, when overlap-save
-// applies, their 4096-point spectrum (made by the same kernel rf_fir runs per call, so the
-// results are the same bits).
+// A filter prepared once for many calls: device copies of the taps and, when the 4096-point
+// overlap-save applies, their spectrum (made by the same kernel rf_fir runs per call, so the
+// results are the same bits; the 512-point form makes its spectrum inside its one launch).
 struct rf_fir_plan {
     int impl = 0;
     int num_taps = 0;
@@ -1223,7 +1465,8 @@ struct rf_fir_batch {
     int num_taps = 0;
     int count = 0;
     int total = 0;                 // work items
-    bool ola = false;              // overlap-save segments, else register blocks
+    int points = 0;                // overlap-save segments of this transform length, else
+                                   // register blocks (0)
     int r = 4;                     // register: outputs per lane
     std::size_t lds = 0;           // register: dynamic LDS per workgroup
     float* taps = nullptr;         // device copy
@@ -1266,8 +1509,8 @@ rf_status rf_fir_batch_create(int impl, const float* taps, int num_taps, int cou
         }
         static const float2 probe{};
         rf::check_fir_args(impl, &probe, taps, &probe, 1, num_taps, RF_DEFAULT_FIR_BLOCK);
-        const bool ola = impl == RF_FIR_FFT_OLA && rf::ola_applies(num_taps);
-        if (!ola) {
+        const int points = impl == RF_FIR_FFT_OLA ? rf::ola_points(num_taps) : 0;
+        if (points == 0) {
             rf::check_const_taps(num_taps, "register");
         }
         long long longest = 0;
@@ -1279,12 +1522,12 @@ rf_status rf_fir_batch_create(int impl, const float* taps, int num_taps, int cou
         b->impl = impl;
         b->num_taps = num_taps;
         b->count = count;
-        b->ola = ola;
+        b->points = points;
         b->n.assign(num_samples, num_samples + count);
         try {
             long long per;  // outputs per item
-            if (ola) {
-                per = rf::kOlaP - (num_taps - 1);
+            if (points != 0) {
+                per = points - (num_taps - 1);
             } else {
                 b->r = rf::register_outputs(num_taps, RF_DEFAULT_FIR_BLOCK, longest, &b->lds);
                 per = static_cast<long long>(RF_DEFAULT_FIR_BLOCK) * b->r;
@@ -1304,7 +1547,7 @@ rf_status rf_fir_batch_create(int impl, const float* taps, int num_taps, int cou
             RF_HIP_CHECK(hipMalloc(&b->taps, sizeof(float) * num_taps));
             RF_HIP_CHECK(hipMemcpyAsync(b->taps, taps, sizeof(float) * num_taps,
                                         hipMemcpyDeviceToDevice, s));
-            if (ola) {
+            if (points == rf::kOlaP) {
                 RF_HIP_CHECK(hipMalloc(&b->spec, sizeof(float2) * rf::kOlaP));
                 rf::launch_ola_taps(b->taps, num_taps, b->spec, s);
             }
@@ -1357,7 +1600,14 @@ rf_status rf_fir_batch_run(const rf_fir_batch* batch, rf_stream stream)
         const long long cap = 4LL * rf::cu_count();
         const unsigned grid = static_cast<unsigned>(
             rf::ceil_div(std::min<long long>(batch->total, cap), 8) * 8);
-        if (batch->ola) {
+        if (batch->points == rf::kOla512P) {
+            rf::fir_ola512_batch_kernel<<<rf::ola512_grid(batch->total), rf::k512Lanes, 0, s>>>(
+                batch->caps, batch->items, batch->total, batch->taps, batch->num_taps,
+                rf::twiddle_table_4096());
+            rf::check_launch("fir_ola512_batch_kernel");
+            return;
+        }
+        if (batch->points == rf::kOlaP) {
             rf::fir_ola_batch_kernel<<<grid, rf::k4096Lanes, 0, s>>>(
                 batch->caps, batch->items, batch->total, batch->spec, batch->num_taps,
                 rf::fft4096_stage_table());

@@ -36,7 +36,7 @@ rng = np.random.default_rng(3)
 x = (rng.standard_normal(30011) + 1j * rng.standard_normal(30011)).astype(np.complex64)
 cases = []
 for impl, num_taps in [("fft_ola", 1025), ("mfma_f16x3", 777), ("mfma_bf16x3", 2049),
-                       ("fft_ola", 66)]:
+                       ("fft_ola", 226), ("fft_ola", 63)]:
     h = kaiser(num_taps)
     cases.append((impl, h, rf.DeviceBuffer.from_array(h), rf.DeviceBuffer(x.size)))
 d_x = rf.DeviceBuffer.from_array(x)

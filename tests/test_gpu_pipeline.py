@@ -211,14 +211,18 @@ def test_fir_batched_matches_oracle(impl: str, taps_name: str) -> None:
         np.testing.assert_array_equal(got, rf.fir(x, h, impl))
 
 
-# Which kernel each variant runs: overlap-save and the matrix-core GEMMs cover 66..2049 taps,
-# the register kernel the rest (a single tap is then exact, and very short filters are cheaper
-# direct).
+# Which kernel each variant runs: overlap-save covers 17..225 taps with a 512-point transform and
+# 226..2049 with a 4096-point one, the matrix-core GEMMs 66..2049 taps, the register kernel the
+# rest (a single tap is then exact, and very short filters are cheaper direct).
 @pytest.mark.parametrize("num_taps,ola,mfma", [
     (1, "fir_register_kernel", "fir_register_kernel"),
-    (63, "fir_register_kernel", "fir_register_kernel"),
-    (65, "fir_register_kernel", "fir_register_kernel"),
-    (66, "fir_ola_kernel", "fir_mfma_kernel"),
+    (16, "fir_register_kernel", "fir_register_kernel"),
+    (17, "fir_ola512_kernel", "fir_register_kernel"),
+    (63, "fir_ola512_kernel", "fir_register_kernel"),
+    (65, "fir_ola512_kernel", "fir_register_kernel"),
+    (66, "fir_ola512_kernel", "fir_mfma_kernel"),
+    (225, "fir_ola512_kernel", "fir_mfma_kernel"),
+    (226, "fir_ola_kernel", "fir_mfma_kernel"),
     (1025, "fir_ola_kernel", "fir_mfma_kernel"),
     (2049, "fir_ola_kernel", "fir_mfma_kernel"),
     (2050, "fir_register_kernel", "fir_register_kernel"),
@@ -250,6 +254,23 @@ def test_fir_transform_variants_at_their_tap_counts(impl: str, num_taps: int) ->
     assert not np.any(z)
 
 
+@pytest.mark.parametrize("num_taps", [17, 33, 63, 65, 129, 225])
+@pytest.mark.parametrize("n", [1, 449, 512, 30011])
+def test_fir_short_overlap_save(num_taps: int, n: int) -> None:
+    # fft_ola's 512-point form (one wavefront per segment): both ends of its tap range, captures
+    # shorter than one segment, exactly one, and many; against the oracle, and different in the
+    # last bits from the register kernel (so the transform ran) wherever the outputs are not
+    # trivially exact. Zeros give exactly zero.
+    h = kaiser_lowpass(num_taps, 0.2, 8.0)
+    x = noise(60 + num_taps + n, n)
+    got = rf.fir(x, h, "fft_ola")
+    assert_fir_close("fft_ola", got, x, h)
+    if n > 64:
+        assert not np.array_equal(got, rf.fir(x, h, "register"))
+    z = rf.fir(np.zeros(max(n, 2000), np.complex64), h, "fft_ola")
+    assert not np.any(z)
+
+
 @pytest.mark.parametrize("impl", TRANSFORM_FIR)
 @pytest.mark.parametrize("num_taps,delay", [(129, 100), (1025, 777), (2049, 2048)])
 def test_fir_transform_variants_delayed_impulse(impl: str, num_taps: int, delay: int) -> None:
@@ -265,6 +286,21 @@ def test_fir_transform_variants_delayed_impulse(impl: str, num_taps: int, delay:
     tol = {"fft_ola": 2e-6, "mfma_f16x3": 2.0 ** -19, "mfma_bf16x3": 2.0 ** -14}[impl]
     err = np.abs(got - want)
     assert np.all(err <= tol * (np.abs(want) + 1.0)), float(err.max())
+
+
+@pytest.mark.parametrize("num_taps,delay", [(17, 16), (40, 3), (65, 64), (65, 0), (225, 224),
+                                            (160, 97)])
+def test_fir_short_overlap_save_delayed_impulse(num_taps: int, delay: int) -> None:
+    # The 512-point form with h = unit impulse at `delay`: y[n] = x[n - delay], to the
+    # transform's rounding, sample by sample (a shifted or mirrored index would miss by O(1)),
+    # across segment borders (20011 samples, 288..496 outputs per segment).
+    h = np.zeros(num_taps, np.float32)
+    h[delay] = 1.0
+    x = noise(70 + delay, 20011)
+    got = rf.fir(x, h, "fft_ola").astype(np.complex128)
+    want = np.concatenate([np.zeros(delay), x[:x.size - delay].astype(np.complex128)])
+    err = np.abs(got - want)
+    assert np.all(err <= 2e-6 * (np.abs(want) + 1.0)), float(err.max())
 
 
 @pytest.mark.parametrize("impl", ["tiled_const", "register"])

@@ -43,8 +43,10 @@ def test_noalloc_matches_rf_fir_on_two_calls(impl: str) -> None:
     lib = rf.lib()
     stream = rf.Stream()
     # first call: long capture and the longest transform filter; second: shorter capture, other
-    # taps (the scratch keeps the first call's contents and must be overwritten, not reused)
-    for seed, n, num_taps, cutoff in [(1, 300007, 2049, 0.1), (2, 40001, 777, 0.2)]:
+    # taps (the scratch keeps the first call's contents and must be overwritten, not reused);
+    # a third, shorter still, with a filter of the 512-point overlap-save range
+    for seed, n, num_taps, cutoff in [(1, 300007, 2049, 0.1), (2, 40001, 777, 0.2),
+                                      (3, 20011, 63, 0.2)]:
         x, h = noise(seed, n), kaiser(num_taps, cutoff)
         print(f"rf_fir_noalloc {impl} n={n} taps={num_taps}", flush=True)
         got = call(lib.rf_fir_noalloc, impl, x, h, stream.handle)
