@@ -7,16 +7,18 @@ phases, seeded), filtered by a 1025-tap Kaiser (beta 8) low-pass with cutoff 0.1
 gain. One step is one rf_fir call over the whole capture with inputs already in HBM.
 
   python bench.py [--steps K] [--warmup W] [--impl fft_ola] [--workload fir_long|fir_plain|
-                   channelize|gcc_batch|delay|copy|sigflow|stream] [--gpus 1]
+                   fir_short|channelize|gcc_batch|delay|copy|sigflow|stream] [--gpus 1]
 
-Input/output buffers rotate over enough sets (at least 320 MiB: three for the long FIR, 20 for
-the 2^20-sample FIR) that the 256 MB Infinity Cache cannot hold a step's input from the previous
-visit. Timing: device events on
-the stream the kernels run on, bracketing all K steps, after a device synchronisation. The
-roofline figure uses the algorithmic bytes of one step (input + output + taps) over the time of
-the dominant kernel alone, measured the same way over K launches of a prepared filter
-(rf_fir_plan_run, the same bits as rf_fir without the per-call tap transform). The CPU baseline
-is the float64 oracle (single thread) on a bounded prefix of the same input.
+fir_plain is 2^20 samples x 63-tap Hamming low-pass 0.2 fs (the CPU-runnable FIR), fir_short the
+same filter over 2^24 samples. Input/output buffers rotate over enough sets (at least 320 MiB:
+three at 2^24 samples, 20 at 2^20) that the 256 MB Infinity Cache cannot hold a step's input
+from the previous visit. Timing: device events on the stream the kernels run on, bracketing all
+K steps, after a device synchronisation. The roofline figure uses the algorithmic bytes of one
+step (input + output + taps) over the time of the dominant kernel alone, measured the same way
+over K launches of a prepared filter (rf_fir_plan_run, the same bits as rf_fir without the
+per-call tap transform). The CPU baseline is the multithreaded FP32 CPU path
+(oracle/cpu_fast.cpp) on the whole workload; the float64 oracle's own time is reported beside
+it and its output is the checker.
 """
 from __future__ import annotations
 
@@ -160,8 +162,12 @@ def time_steps(rf, step, steps: int, warmup: int, stream) -> float:
     return stop.ms_since(start)
 
 
-def bench_fir(rf, args, long_filter: bool) -> dict:
-    x, h = fir_long_input() if long_filter else fir_plain_input()
+def bench_fir(rf, args) -> dict:
+    long_filter = args.workload == "fir_long"
+    if long_filter:
+        x, h = fir_long_input()
+    else:
+        x, h = fir_plain_input(1 << 24 if args.workload == "fir_short" else 1 << 20)
     n, t = x.size, h.size
     # Enough rotating input/output sets that a step's buffers were evicted from the 256 MB
     # Infinity Cache since their last visit: at least 320 MiB in all (3 sets at 2^24 samples,
@@ -197,7 +203,7 @@ def bench_fir(rf, args, long_filter: bool) -> dict:
 
     kernel_ms = time_steps(rf, plan_step, args.steps, args.warmup, stream.handle) / args.steps
     graph = None
-    if not long_filter:
+    if n < 1 << 24:
         # A short capture is a few microseconds of GPU work per call, as long as a launch from
         # the host: the same K calls captured once into a HIP graph and replayed show the
         # GPU-side rate without per-call host gaps.
@@ -281,7 +287,8 @@ def bench_fir(rf, args, long_filter: bool) -> dict:
                    "buffer_sets": sets, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles(f"fir_{args.impl}") if long_filter else None},
+                     "traffic": traffic_from_profiles(
+                         f"fir_{args.impl}" if long_filter else f"{args.workload}_{args.impl}")},
         "roofline_kernel": {"kernel": rf.fir_kernel(args.impl, t),
                             "launches": "rf_fir_plan_run: the filtering kernel only, the kernel "
                                         "the timed rf_fir step runs after its tap transform",
@@ -901,8 +908,8 @@ def main() -> int:
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--impl", default="fft_ola")
     p.add_argument("--workload", default="fir_long",
-                   choices=["fir_long", "fir_plain", "channelize", "gcc_batch", "delay", "copy",
-                            "sigflow", "stream"])
+                   choices=["fir_long", "fir_plain", "fir_short", "channelize", "gcc_batch",
+                            "delay", "copy", "sigflow", "stream"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--sigflow-dir", default=None,
                    help="sigflow: write the 2^18-sample inputs and outputs here and keep them")
@@ -927,7 +934,7 @@ def main() -> int:
     elif args.workload == "stream":
         out = bench_stream(rf, args)
     else:
-        out = bench_fir(rf, args, args.workload == "fir_long")
+        out = bench_fir(rf, args)
     line = {"metric": out.pop("metric"), "value": out.pop("value"), "unit": out.pop("unit"),
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": out.pop("ms_per_step"), "higher_is_better": True, "scaling": "weak",
