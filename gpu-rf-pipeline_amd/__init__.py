@@ -572,8 +572,9 @@ def fir_impl_index(name: str) -> int:
 
 
 def fir_kernel(impl: str, num_taps: int) -> str:
-    """Name of the kernel rf_fir runs for this variant and tap count (fft_ola and the
-    matrix-core variants use the register kernel outside 66..2049 taps)."""
+    """Name of the kernel rf_fir runs for this variant and tap count (fft_ola: 512-point
+    segments for 17..225 taps, 4096-point for 226..2049; the matrix-core variants: 66..2049;
+    the register kernel outside those ranges)."""
     name = _lib.rf_fir_kernel_name(fir_impl_index(impl), num_taps)
     if name is None:
         raise RfError(RF_INVALID_ARG, f"no FIR kernel for {impl} with {num_taps} taps")

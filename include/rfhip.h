@@ -10,7 +10,8 @@
  *   - compute entry points only enqueue work on the given stream; they never synchronise,
  *     never copy to or from the host and, once a stream has been used, never allocate: the
  *     constant tables are compiled into the library, and the one-shot FIR forms that derive a
- *     table from the taps (fft_ola's 4096-point tap spectrum, the MFMA variants' split taps)
+ *     table from the taps (fft_ola's 4096-point tap spectrum from 226 taps, the MFMA variants'
+ *     split taps; fft_ola's 512-point form for 17..225 taps makes its spectrum in registers)
  *     keep it in a 72 KiB workspace block per stream, allocated by the first such call on that
  *     stream (or by rf_stream_create) and freed by rf_stream_destroy;
  *   - the caller owns every input and output buffer; plan handles own their workspaces;
@@ -111,7 +112,9 @@ enum {
     RF_FIR_TILED = 1,       /* LDS tile with a T-1 halo (src/fir.cu:100-144) */
     RF_FIR_TILED_CONST = 2, /* tiled, taps through the scalar (constant) cache (src/fir.cu:213-254) */
     RF_FIR_REGISTER = 3,    /* R outputs per lane from a register window (src/fir.cu:337-409) */
-    RF_FIR_FFT_OLA = 4,     /* overlap-save with the library's LDS FFT; no reference counterpart */
+    RF_FIR_FFT_OLA = 4,     /* overlap-save with the library's FFT: 512-point segments per wavefront
+                               for 17..225 taps, 4096-point for 226..2049, the register kernel
+                               otherwise; no reference counterpart */
     RF_FIR_MFMA_F16X3 = 5,  /* Toeplitz GEMM, FP16 matrix cores, hi/lo split; no counterpart */
     RF_FIR_MFMA_BF16X3 = 6, /* the same in BF16 (reduced precision, error ~2^-16 relative) */
     RF_FIR_IMPL_COUNT = 7
@@ -172,7 +175,8 @@ rf_status rf_fir_run_config(const rf_fir_config* config, const rf_cf32* x, const
 
 /* Prepared filter. No counterpart in the reference, whose host wrappers upload the taps on
  * every call (src/fir.cu:62-85): for one filter applied to many blocks, create copies the
- * device taps into memory the plan owns and, for fft_ola, makes their 4096-point spectrum once
+ * device taps into memory the plan owns and, for fft_ola from 226 taps, makes their 4096-point
+ * spectrum once
  * (then synchronises `stream`), so each run launches only the filtering kernel. Arguments and
  * errors as rf_fir (impl, taps, num_taps, block checked at create); a run gives the same bits
  * as rf_fir with that impl and block. Runs on one plan may overlap on different streams. */
@@ -186,8 +190,9 @@ rf_status rf_fir_plan_destroy(rf_fir_plan* plan);
 /* Persistent batched FIR: any number of captures of uneven length sharing one filter, all
  * filtered by ONE launch of a persistent kernel that walks {capture, segment} work items
  * (XCD-major). No reference counterpart beyond fir_batched_device (src/fused.cu:92-109), which
- * takes at most 8 channels as grid.y. impl is RF_FIR_FFT_OLA (overlap-save segments for
- * 66..2049 taps, the register kernel's blocks otherwise) or RF_FIR_REGISTER; each capture's
+ * takes at most 8 channels as grid.y. impl is RF_FIR_FFT_OLA (overlap-save segments of 512
+ * points for 17..225 taps and 4096 for 226..2049, the register kernel's blocks otherwise) or
+ * RF_FIR_REGISTER; each capture's
  * output is bit-identical to rf_fir with that impl. create copies the DEVICE taps, makes the
  * tap spectrum and the item map (synchronises `stream`); bind takes HOST arrays of `count`
  * device pointers (synchronous); run only enqueues one launch (no allocation: capturable). */
