@@ -379,20 +379,12 @@ __device__ __forceinline__ void buffer_store2(float2 v, __amdgpu_buffer_rsrc_t r
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voffset, soffset, 2);
 }
 
-// One segment: seg[i] = x[in0 + i], i < 4096 (zero outside the input), filtered by the
-// spectrum, outputs y[out0 + i - (T-1)] for T-1 <= i < 4096. Lane t (threadIdx.x, passed in)
-// holds positions t + 256 r. The twiddle factors are loaded after the samples, so they arrive
-// while the first pass runs, and serve both transforms.
-__device__ __forceinline__ void ola_segment(const float2* __restrict__ x,
-                                            const float2* __restrict__ spec,
-                                            float2* __restrict__ y, int n, int halo,
-                                            long long out0, float2* buf,
-                                            const float2* __restrict__ stages, int t)
+// A lane's 16 samples x[in0 + t + 256 r] of a segment (zero outside the input).
+__device__ __forceinline__ void ola_load(const float2* __restrict__ x, int n, long long in0, int t,
+                                         float2 (&u)[16])
 {
-    const long long in0 = out0 - halo;
     const unsigned lane_bytes = static_cast<unsigned>(t) * 8u;
     constexpr unsigned kBlockBytes = k4096Lanes * 8u;
-    float2 u[16];
     if (in0 >= 0 && in0 + kOlaP <= n
         && (reinterpret_cast<unsigned long long>(x + in0) & 15ULL) == 0) {
         // Blocks 2 q and 2 q + 1 with one 16-byte load per lane: an even lane t reads samples
@@ -427,24 +419,34 @@ __device__ __forceinline__ void ola_segment(const float2* __restrict__ x,
             u[r] = load_sample(x, in0 + t + r * k4096Lanes, n);
         }
     }
-    Fft4096Twiddles tw;
-    fft4096_load_twiddles(t, stages, tw);
-    fft4096<false>(buf, t, u, tw);
-    // the tap spectrum in bin pairs (ola_taps_spectrum): 8 loads of 16 bytes per lane
+}
+
+// u (the forward transform of a segment) times the tap spectrum, read in bin pairs
+// (ola_taps_spectrum): 8 loads of 16 bytes per lane.
+__device__ __forceinline__ void ola_spectrum_product(const float2* __restrict__ spec, int t,
+                                                     float2 (&u)[16])
+{
+    constexpr unsigned kBlockBytes = k4096Lanes * 8u;
     const __amdgpu_buffer_rsrc_t h = unchecked_buffer(spec);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         typedef float f32x4 __attribute__((ext_vector_type(4)));
         const f32x4 hv = __builtin_bit_cast(
-            f32x4, __builtin_amdgcn_raw_buffer_load_b128(h, 2u * lane_bytes, q * 2u * kBlockBytes, 0));
+            f32x4, __builtin_amdgcn_raw_buffer_load_b128(h, 2u * static_cast<unsigned>(t) * 8u,
+                                                         q * 2u * kBlockBytes, 0));
         u[2 * q] = cmul(u[2 * q], make_float2(hv.x, hv.y));
         u[2 * q + 1] = cmul(u[2 * q + 1], make_float2(hv.z, hv.w));
     }
-    __syncthreads();  // every lane has read the forward result before it is overwritten
-    fft4096<true>(buf, t, u, tw);
+}
+
+// Outputs of a segment: position p = t + 256 r goes to y[out0 + p - halo] when p >= halo (and
+// inside the output).
+__device__ __forceinline__ void ola_store(float2* __restrict__ y, int n, int halo, long long out0,
+                                          const float2 (&u)[16], int t)
+{
+    const unsigned lane_bytes = static_cast<unsigned>(t) * 8u;
     if (out0 + kOlaP - halo <= n) {
-        // Position p = t + 256 r goes to y[out0 + p - halo] when p >= halo; a block wholly at or
-        // above the halo stores without a per-lane test.
+        // a block wholly at or above the halo stores without a per-lane test
         const __amdgpu_buffer_rsrc_t dst = unchecked_buffer(y + out0);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
@@ -465,6 +467,27 @@ __device__ __forceinline__ void ola_segment(const float2* __restrict__ x,
             }
         }
     }
+}
+
+// One segment: seg[i] = x[in0 + i], i < 4096 (zero outside the input), filtered by the
+// spectrum, outputs y[out0 + i - (T-1)] for T-1 <= i < 4096. Lane t (threadIdx.x, passed in)
+// holds positions t + 256 r. The twiddle factors are loaded after the samples, so they arrive
+// while the first pass runs, and serve both transforms.
+__device__ __forceinline__ void ola_segment(const float2* __restrict__ x,
+                                            const float2* __restrict__ spec,
+                                            float2* __restrict__ y, int n, int halo,
+                                            long long out0, float2* buf,
+                                            const float2* __restrict__ stages, int t)
+{
+    float2 u[16];
+    ola_load(x, n, out0 - halo, t, u);
+    Fft4096Twiddles tw;
+    fft4096_load_twiddles(t, stages, tw);
+    fft4096<false>(buf, t, u, tw);
+    ola_spectrum_product(spec, t, u);
+    __syncthreads();  // every lane has read the forward result before it is overwritten
+    fft4096<true>(buf, t, u, tw);
+    ola_store(y, n, halo, out0, u, t);
 }
 
 // Segment of a workgroup, XCD-major: workgroups are dispatched round-robin over the 8 XCDs, so
@@ -561,11 +584,12 @@ __global__ void fir_register_batch_kernel(const FirCapture* __restrict__ caps,
 // per bin, over Q outputs), against about 110 for the 4096-point form, which overtakes it
 // only where Q has shrunk to about half a segment (measured on 2^24 samples: 63.6 against
 // 76.4 us at 66 taps, 62.4 against 66.7 at 193, 74.7 against 66.6 at 257). A workgroup is one
-// wave (64 lanes), so a segment's barriers hold only its own wave. Each persistent wave makes the tap spectrum itself before its first segment (one
-// transform more per wave, where a separate spectrum launch cost about 2.6 us per call on
-// 2^20-sample captures), so every call is one launch and needs no device memory beyond its
-// inputs and outputs; the wave then keeps its 14 transform factors and 8 spectrum bins in
-// registers across its segments.
+// wave (64 lanes), so a segment's barriers hold only its own wave. Each persistent wave makes
+// the tap spectrum itself before its first segment (one transform more per wave, where a
+// separate spectrum launch cost about 2.6 us per call on 2^20-sample captures), so every call
+// is one launch and needs no device memory beyond its inputs and outputs; the wave then keeps
+// its 14 transform factors and 8 spectrum bins in registers across its segments, and loads
+// its next segment's samples before transforming the current one.
 constexpr int kOla512P = k512;
 constexpr int kOla512MinTaps = 17;
 constexpr int kOla512MaxTaps = 225;
