@@ -273,6 +273,18 @@ def bench_fir(rf, args) -> dict:
         variants[name] = {"kernel": rf.fir_kernel(name, t),
                           "ms_per_step": round(vms, 4), "msamples_per_s": round(n / vms / 1e3, 1),
                           "max_abs_err": v_abs, "max_rel_err": v_rel}
+    tuned = None
+    if not args.no_variants:
+        # The library's autotuner over the variants that suit this filter (tile shapes of the
+        # register kernel, fft_ola's transform length, the split-precision GEMMs), timed on
+        # the same buffers: what it picks and its own timing of that configuration (one buffer
+        # set, so partly resident in the Infinity Cache: faster than the rotating-set figures).
+        names = ["register", "fft_ola", "mfma_f16x3", "mfma_bf16x3"]
+        best = rf.fir_tune(d_x[0].ptr, d_h.ptr, d_y[0].ptr, n, t, names, stream.handle)
+        tuned = {"impl": rf.fir_impls()[best.impl].name, "block": best.block,
+                 "outputs_per_lane": best.outputs_per_lane,
+                 "transform_points": best.transform_points, "ms": round(best.ms, 5),
+                 "candidates": names}
     release(rf, d_h, d_x, d_y, stream)
     workload = (f"FIR {n} complex64 samples x {t}-tap "
                 + ("Kaiser(beta=8) lowpass 0.1 fs" if long_filter else "Hamming lowpass 0.2 fs"))
@@ -301,6 +313,7 @@ def bench_fir(rf, args) -> dict:
         "graph_replay": graph,
         "accuracy": accuracy,
         "variants": variants,
+        "tuned": tuned,
         "cpu_baseline": cpu,
         "cpu_oracle": cpu_oracle,
     }

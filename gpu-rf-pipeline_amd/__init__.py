@@ -133,9 +133,11 @@ _proto("rf_magnitude", _st, _vp, _vp, _int, _vp)
 
 
 class FirConfig(ctypes.Structure):
-    """rf_fir_config: variant, lanes per workgroup, outputs per lane, measured ms."""
+    """rf_fir_config: variant, lanes per workgroup, outputs per lane, measured ms, overlap-save
+    transform length (fft_ola: 512 or 4096; 0 = the library's choice for the tap count)."""
     _fields_ = [("impl", ctypes.c_int32), ("block", ctypes.c_int32),
-                ("outputs_per_lane", ctypes.c_int32), ("ms", ctypes.c_float)]
+                ("outputs_per_lane", ctypes.c_int32), ("ms", ctypes.c_float),
+                ("transform_points", ctypes.c_int32)]
 
 
 _proto("rf_fir_tune", _st, _vp, _vp, _vp, _int, _int, ctypes.c_uint, _vp,

@@ -746,7 +746,8 @@ __global__ __launch_bounds__(k512Lanes) void fir_ola512_batched_kernel(
     const int q = kOla512P - (num_taps - 1);
     const float2* x = batch.x[c];
     float2* y = batch.y[c];
-    ola512_walk((n + q - 1) / q, taps, num_taps, table, buf, [&](int seg) {
+    const int segments = static_cast<int>((static_cast<long long>(n) + q - 1) / q);
+    ola512_walk(segments, taps, num_taps, table, buf, [&](int seg) {
         return Ola512Seg{x, y, n, static_cast<long long>(seg) * q};
     });
 }
@@ -964,6 +965,29 @@ void launch_ola(const float2* x, const float2* spec, float2* y, int n, int num_t
     check_launch("fir_ola_kernel");
 }
 
+
+// Largest tap count of each overlap-save length when a configuration forces it (at least 64
+// outputs per 512-point segment; the 4096-point form as rf_fir).
+constexpr int kOla512LimitTaps = 449;
+
+bool ola_points_fit(int points, int num_taps)
+{
+    return num_taps >= 1 && num_taps <= (points == kOla512P ? kOla512LimitTaps : kOlaMaxTaps);
+}
+
+// fft_ola with a given transform length (tuner configurations): the 512-point form in one
+// launch, or the 4096-point form with its tap spectrum in the stream's workspace block.
+void launch_ola_points(int points, const float2* x, const float* taps, float2* y, int n,
+                       int num_taps, hipStream_t stream)
+{
+    if (points == kOla512P) {
+        launch_ola512(x, taps, y, n, num_taps, stream);
+        return;
+    }
+    float2* h = static_cast<float2*>(stream_workspace(stream));
+    launch_ola_taps(taps, num_taps, h, stream);
+    launch_ola(x, h, y, n, num_taps, stream);
+}
 
 // `workspace`: at least kStreamWorkspaceBytes of device memory for the tap spectrum / split
 // taps of fft_ola and the matrix-core variants, or nullptr for the stream's own block.
@@ -1261,7 +1285,27 @@ rf_status rf_fir_run_config(const rf_fir_config* config, const rf_cf32* x, const
             rf::fail(RF_INVALID_ARG, "fir config: outputs_per_lane must be 0, 4, 8 or 16");
         }
         rf::check_fir_args(config->impl, x, taps, y, num_samples, num_taps, config->block);
+        const int points = config->transform_points;
+        if (points != 0 && points != rf::kOla512P && points != rf::kOlaP) {
+            rf::fail(RF_INVALID_ARG, "fir config: transform_points must be 0, 512 or 4096");
+        }
+        if (points != 0 && config->impl != RF_FIR_FFT_OLA) {
+            rf::fail(RF_INVALID_ARG, "fir config: transform_points applies to fft_ola only");
+        }
+        if (points != 0 && !rf::ola_points_fit(points, num_taps)) {
+            rf::fail(RF_LIMIT, "fir config: a " + std::to_string(points)
+                                   + "-point overlap-save takes 1.."
+                                   + std::to_string(points == rf::kOla512P ? rf::kOla512LimitTaps
+                                                                           : rf::kOlaMaxTaps)
+                                   + " taps, not " + std::to_string(num_taps));
+        }
         if (num_samples == 0) {
+            return;
+        }
+        if (points != 0) {
+            rf::launch_ola_points(points, reinterpret_cast<const float2*>(x), taps,
+                                  reinterpret_cast<float2*>(y), num_samples, num_taps,
+                                  rf::as_stream(stream));
             return;
         }
         if (config->impl == RF_FIR_REGISTER && r != 0) {
@@ -1299,16 +1343,32 @@ rf_status rf_fir_tune(const rf_cf32* x, const float* taps, rf_cf32* y, int num_s
                     for (int r : {4, 8, 16}) {
                         if (rf::register_bytes_for(r, num_taps, block)
                             <= static_cast<std::size_t>(rf::max_lds_per_block())) {
-                            candidates.push_back(rf_fir_config{impl, block, r, 0.0f});
+                            candidates.push_back(rf_fir_config{impl, block, r, 0.0f, 0});
                         }
                     }
                 }
             } else if (impl == RF_FIR_NAIVE || impl == RF_FIR_TILED || impl == RF_FIR_TILED_CONST) {
                 for (int block : {128, 256, 512}) {
-                    candidates.push_back(rf_fir_config{impl, block, 0, 0.0f});
+                    candidates.push_back(rf_fir_config{impl, block, 0, 0.0f, 0});
+                }
+            } else if (impl == RF_FIR_FFT_OLA) {
+                // the transform length: each form the tap count allows (a single tap, or more
+                // than 2049, leaves only the library's choice, the register kernel)
+                bool any = false;
+                if (num_taps >= 2) {
+                    for (int points : {rf::kOla512P, rf::kOlaP}) {
+                        if (rf::ola_points_fit(points, num_taps)) {
+                            candidates.push_back(
+                                rf_fir_config{impl, RF_DEFAULT_FIR_BLOCK, 0, 0.0f, points});
+                            any = true;
+                        }
+                    }
+                }
+                if (!any) {
+                    candidates.push_back(rf_fir_config{impl, RF_DEFAULT_FIR_BLOCK, 0, 0.0f, 0});
                 }
             } else {
-                candidates.push_back(rf_fir_config{impl, RF_DEFAULT_FIR_BLOCK, 0, 0.0f});
+                candidates.push_back(rf_fir_config{impl, RF_DEFAULT_FIR_BLOCK, 0, 0.0f, 0});
             }
         }
         hipStream_t s = rf::as_stream(stream);

@@ -159,17 +159,23 @@ rf_status rf_fir_batched(int impl, const rf_cf32* const* x, rf_cf32* const* y,
  * src/fir.cu:449-485): times every variant allowed by impl_mask (bit i = RF_FIR_* value i) on
  * the caller's buffers (y is overwritten) with its tile shapes, lanes per workgroup
  * {64, 128, 256, 512} x outputs per lane {4, 8, 16} for `register`, {128, 256, 512} for the
- * other direct variants, and reports the fastest; device events on `stream`, which it
+ * other direct variants, the overlap-save transform length {512, 4096} for fft_ola (each where
+ * the tap count allows it), and reports the fastest; device events on `stream`, which it
  * synchronises. The direct variants give identical bits for every shape. */
 typedef struct rf_fir_config {
     int32_t impl;             /* RF_FIR_* */
     int32_t block;            /* lanes per workgroup of the direct variants */
     int32_t outputs_per_lane; /* register variant: 4, 8 or 16; 0 = the library's choice */
     float ms;                 /* measured time of one call */
+    int32_t transform_points; /* fft_ola: 512 (2..449 taps) or 4096 (2..2049 taps) points per
+                                 overlap-save segment; 0 = the library's choice for the tap
+                                 count (rf_fir's bits); 0 for every other variant */
 } rf_fir_config;
 rf_status rf_fir_tune(const rf_cf32* x, const float* taps, rf_cf32* y, int num_samples,
                       int num_taps, unsigned impl_mask, rf_stream stream, rf_fir_config* best);
-/* rf_fir with a tuned configuration. */
+/* rf_fir with a tuned configuration. RF_INVALID_ARG for a transform length other than 0, 512
+ * or 4096 or one given to a variant other than fft_ola; RF_LIMIT when the tap count is outside
+ * the chosen length's range. */
 rf_status rf_fir_run_config(const rf_fir_config* config, const rf_cf32* x, const float* taps,
                             rf_cf32* y, int num_samples, int num_taps, rf_stream stream);
 

@@ -132,6 +132,15 @@ def test_channelize_mfma_and_fir_tune_arguments() -> None:
     assert L.rf_fir_tune(8, 8, 8, 1024, 63, 1 << 3, None, None) == rf.RF_INVALID_ARG
     cfg = rf.FirConfig(3, 256, 5, 0.0)
     assert L.rf_fir_run_config(ctypes.byref(cfg), 8, 8, 8, 1024, 63, None) == rf.RF_INVALID_ARG
+    # the overlap-save transform length: only 0, 512 or 4096, only for fft_ola, and only for
+    # the tap counts that length takes (1..449 and 1..2049), all refused before any launch
+    ola = rf.fir_impl_index("fft_ola")
+    for points, impl, num_taps, status in [(1000, ola, 63, rf.RF_INVALID_ARG),
+                                           (512, 3, 63, rf.RF_INVALID_ARG),
+                                           (512, ola, 450, rf.RF_LIMIT),
+                                           (4096, ola, 2050, rf.RF_LIMIT)]:
+        cfg = rf.FirConfig(impl, 256, 0, 0.0, points)
+        assert L.rf_fir_run_config(ctypes.byref(cfg), 8, 8, 8, 1024, num_taps, None) == status
 
 
 def test_spectrum_magnitude_arguments() -> None:

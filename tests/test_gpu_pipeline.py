@@ -504,6 +504,37 @@ def test_fir_autotuner_configs_give_identical_bits() -> None:
                           d_x.ptr, d_h.ptr, d_y.ptr, n, t)
 
 
+@pytest.mark.parametrize("num_taps", [17, 129, 300, 1025])
+def test_fir_autotuner_picks_the_transform_length(num_taps: int) -> None:
+    """fft_ola's tuner candidates are its 512- and 4096-point forms wherever the tap count
+    allows them: the tuned configuration and each forced length match the oracle, and the length
+    rf_fir picks for the tap count gives rf_fir's bits."""
+    n = 100003
+    x = noise(80 + num_taps, n)
+    h = kaiser_lowpass(num_taps, 0.1, 8.0)
+    d_x = rf.DeviceBuffer.from_array(x)
+    d_h = rf.DeviceBuffer.from_array(h)
+    d_y = rf.DeviceBuffer(n)
+    want = oracle.fir(x, h)
+    default = {"fir_ola512_kernel": 512, "fir_ola_kernel": 4096}[rf.fir_kernel("fft_ola", num_taps)]
+    ola = rf.fir_impl_index("fft_ola")
+    best = rf.fir_tune(d_x.ptr, d_h.ptr, d_y.ptr, n, num_taps, ["fft_ola"])
+    assert best.impl == ola and best.ms > 0
+    assert best.transform_points in ([512, 4096] if num_taps <= 449 else [4096])
+    for points in sorted({best.transform_points, 512, 4096}):
+        if points == 512 and num_taps > 449:
+            continue
+        d_y.zero()
+        rf.fir_run_config(rf.FirConfig(ola, 256, 0, 0.0, points), d_x.ptr, d_h.ptr, d_y.ptr, n,
+                          num_taps)
+        got = d_y.download()
+        np.testing.assert_allclose(got, want, rtol=FIR_RTOL, atol=FIR_ATOL)
+        if points == default:
+            assert got.tobytes() == rf.fir(x, h, "fft_ola").tobytes()
+    for b in (d_x, d_h, d_y):
+        b.free()
+
+
 @pytest.mark.parametrize("impl", ["register", "fft_ola", "mfma_f16x3", "mfma_bf16x3"])
 def test_fir_full_size_long_filter_sampled_outputs(impl: str) -> None:
     # 2^24 samples x 1025 Kaiser taps: 3000 outputs recomputed in float64 on the host.
