@@ -19,8 +19,10 @@ rf = load_rfhip()
 def captures(seed: int, count: int, longest: int) -> list[np.ndarray]:
     rng = np.random.default_rng(seed)
     lengths = np.exp(rng.uniform(0.0, np.log(longest), count)).astype(int)
-    # edge lengths: empty, one sample, shorter than the halo, one segment and a few
-    lengths[:6] = [0, 1, 62, 1024, 3072, 3073]
+    # edge lengths: empty, one sample, shorter than the halo, one 4096-point segment of a
+    # 1025-tap filter (3072 outputs) and around it, one 512-point segment of a 63-tap filter (450
+    # outputs) and around it
+    lengths[:9] = [0, 1, 62, 1024, 3072, 3073, 449, 450, 451]
     return [(rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
             for n in lengths]
 
@@ -47,8 +49,9 @@ def test_batch_equals_rf_fir_per_capture(impl: str, taps_name: str) -> None:
     batch.destroy()
 
 
-def test_batch_graph_replays() -> None:
-    h = taps("lowpass_long")
+@pytest.mark.parametrize("taps_name", ["lowpass_long", "lowpass"])
+def test_batch_graph_replays(taps_name: str) -> None:
+    h = taps(taps_name)
     xs = captures(8, 64, 1 << 16)
     stream = rf.Stream()
     d_h = rf.DeviceBuffer.from_array(h)
