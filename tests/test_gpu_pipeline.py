@@ -239,11 +239,12 @@ TRANSFORM_FIR = ["fft_ola", "mfma_f16x3", "mfma_bf16x3"]
 
 
 @pytest.mark.parametrize("impl", TRANSFORM_FIR)
-@pytest.mark.parametrize("num_taps", [66, 1025, 2049])
+@pytest.mark.parametrize("num_taps", [66, 226, 1025, 2049])
 def test_fir_transform_variants_at_their_tap_counts(impl: str, num_taps: int) -> None:
-    # The variants whose own kernels start at 66 taps, at both ends of their range and at the
-    # long-FIR size: against the oracle, and different in the last bits from the direct
-    # register kernel (so the variant's own arithmetic ran, not the direct fallback).
+    # The matrix-core variants at both ends of their range (66..2049 taps) and at the long-FIR
+    # size, fft_ola at the same counts (its 512-point form at 66, the 4096-point form from 226):
+    # against the oracle, and different in the last bits from the direct register kernel (so
+    # the variant's own arithmetic ran, not the direct fallback).
     h = kaiser_lowpass(num_taps, 0.1, 8.0)
     x = noise(40 + num_taps, 30011)
     got = rf.fir(x, h, impl)
