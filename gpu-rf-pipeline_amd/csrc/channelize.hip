@@ -72,16 +72,24 @@ static_assert(lds_pad(63) < 72, "a 64-point frame fits 72 slots");
 // sample going into every output that uses it, so only the eight accumulators and the taps
 // stay live, and each output still sums its taps in ascending order (p = 0, 1, ...), as
 // src/channelize.cu:40-48 does.
+// The lane's PB taps h[pM + m] (0 past the filter), loaded before the workgroup stages its
+// samples so that their latency hides behind the staging loads instead of following the
+// staging barrier.
 template <int PB>
-__device__ __forceinline__ void branch_window(const float2* lds, const float* __restrict__ taps,
-                                              int num_taps, int M, int m, int g, float2* acc)
+__device__ __forceinline__ void branch_taps(const float* __restrict__ taps, int num_taps, int M,
+                                            int m, float* h)
 {
-    float h[PB];
 #pragma unroll
     for (int p = 0; p < PB; ++p) {
         const int l = p * M + m;
         h[p] = l < num_taps ? taps[l] : 0.0f;
     }
+}
+
+template <int PB>
+__device__ __forceinline__ void branch_window(const float2* lds, const float* h, int M, int m,
+                                              int g, float2* acc)
+{
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         acc[j] = make_float2(0.0f, 0.0f);
@@ -144,6 +152,11 @@ __global__ __launch_bounds__(kChanThreads) void channelize_fused_kernel(
         return;  // the whole workgroup: padding of the grid to a multiple of 8
     }
     const long long t0 = group * F;
+    constexpr bool windowed = PB != 0;
+    float h[PB ? PB : 1];
+    if (windowed) {
+        branch_taps<PB ? PB : 1>(taps, num_taps, M, threadIdx.x & (M - 1), h);
+    }
     const long long base = t0 * M - static_cast<long long>(pb) * M + 1;
     const int len = (F + pb - 1) * M;
     if constexpr (LOG2M != 0 && PB != 0) {
@@ -211,11 +224,10 @@ __global__ __launch_bounds__(kChanThreads) void channelize_fused_kernel(
 
     // PB != 0 is only launched for M <= 512 (try_fused), where each lane owns one branch.
     float2 acc[8];
-    constexpr bool windowed = PB != 0;
     if (windowed) {
         const int m = threadIdx.x & (M - 1);
         const int g = threadIdx.x >> log2M;
-        branch_window<PB ? PB : 1>(st, taps, num_taps, M, m, g, acc);
+        branch_window<PB ? PB : 1>(st, h, M, m, g, acc);
     } else {
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
