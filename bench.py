@@ -857,7 +857,7 @@ def bench_sigflow(rf, args) -> dict:
 def bench_stream(rf, args) -> dict:
     """The streaming FIR (rfgpu stream: chunks uploaded, filtered and downloaded on three
     streams over a ring of page-locked slots) on 2^24 complex64 samples, PCIe included: chunk
-    2^20, depth 4, each memory mode, 63 taps (register kernel) and 1025 taps (register and
+    2^20, depth 4, each memory mode, 63 taps (register kernel and fft_ola) and 1025 taps (register and
     fft_ola). value = the 1025-tap fft_ola rate in copy mode, the mode with no help from the
     caller. The reference's best streaming times for 2^24 samples are 15.8 ms (63 taps) and
     21.6 ms (1025 taps) (its README.md:77-79, docs/PROFILING_LOG.md:464-471)."""
@@ -874,7 +874,8 @@ def bench_stream(rf, args) -> dict:
         x.tofile(work / "x.cf32")
         h63.tofile(work / "h63.f32")
         h1025.tofile(work / "h1025.f32")
-        for taps, impl in [(63, "register"), (1025, "register"), (1025, "fft_ola")]:
+        for taps, impl in [(63, "register"), (63, "fft_ola"), (1025, "register"),
+                           (1025, "fft_ola")]:
             for memory in ["copy", "registered", "zero-copy"]:
                 proc = subprocess.run(
                     [str(rfgpu), "stream", "--in", str(work / "x.cf32"), "--taps",
