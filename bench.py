@@ -857,8 +857,8 @@ def bench_sigflow(rf, args) -> dict:
 def bench_stream(rf, args) -> dict:
     """The streaming FIR (rfgpu stream: chunks uploaded, filtered and downloaded on three
     streams over a ring of page-locked slots) on 2^24 complex64 samples, PCIe included: chunk
-    2^20, depth 4, each memory mode, 63 taps (register kernel and fft_ola) and 1025 taps (register and
-    fft_ola). value = the 1025-tap fft_ola rate in copy mode, the mode with no help from the
+    2^20, depth 4, each memory mode, 63 and 1025 taps, each with the register kernel and
+    fft_ola. value = the 1025-tap fft_ola rate in copy mode, the mode with no help from the
     caller. The reference's best streaming times for 2^24 samples are 15.8 ms (63 taps) and
     21.6 ms (1025 taps) (its README.md:77-79, docs/PROFILING_LOG.md:464-471)."""
     import subprocess
