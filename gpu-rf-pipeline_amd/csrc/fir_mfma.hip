@@ -265,33 +265,49 @@ __global__ __launch_bounds__(kMfmaThreads) void fir_mfma_kernel(
     f32x16 acc_re1 = {};
     f32x16 acc_im1 = {};
     // Step s covers kk = 16 s .. 16 s + 15; even steps accumulate into acc*0, odd ones into
-    // acc*1, two steps per iteration with no branch, so the accumulators stay in place.
-    auto step = [&](int s, f32x16& acc_re, f32x16& acc_im) {
+    // acc*1, so the accumulators stay in place. A step's six fragments (sample hi/lo of re and
+    // im, tap hi/lo) are read from LDS one step ahead: step s + 1's while step s multiplies,
+    // so the LDS latency hides behind six matrix-core instructions instead of stalling each
+    // step (with the reads next to their MFMAs the loop alone ran at 43 % of the matrix-core
+    // rate, using under half the LDS bandwidth).
+    struct Frags {
+        V arh, arl, aih, ail, bh, bl;
+    };
+    auto load = [&](int s) {
+        Frags f;
         const int u0 = tile_slot(1024 * w + 32 * i - 16 * s - 8 * g + kp - 8);
-        const V arh = *reinterpret_cast<const V*>(rh + u0);
-        const V arl = *reinterpret_cast<const V*>(rl + u0);
-        const V aih = *reinterpret_cast<const V*>(ih + u0);
-        const V ail = *reinterpret_cast<const V*>(il + u0);
+        f.arh = *reinterpret_cast<const V*>(rh + u0);
+        f.arl = *reinterpret_cast<const V*>(rl + u0);
+        f.aih = *reinterpret_cast<const V*>(ih + u0);
+        f.ail = *reinterpret_cast<const V*>(il + u0);
         const int b = num_taps + 71 - 16 * s - 8 * g - i;  // b0 + 48 > 0
         const int r = b & 7;
         const int base = r * len + (b - r);
-        const V bh = *reinterpret_cast<const V*>(th + base);
-        const V bl = *reinterpret_cast<const V*>(tl + base);
-        acc_re = Split<BF16>::mma(arh, bh, acc_re);
-        acc_im = Split<BF16>::mma(aih, bh, acc_im);
-        acc_re = Split<BF16>::mma(arh, bl, acc_re);
-        acc_im = Split<BF16>::mma(aih, bl, acc_im);
-        acc_re = Split<BF16>::mma(arl, bh, acc_re);
-        acc_im = Split<BF16>::mma(ail, bh, acc_im);
+        f.bh = *reinterpret_cast<const V*>(th + base);
+        f.bl = *reinterpret_cast<const V*>(tl + base);
+        return f;
+    };
+    auto mma = [](const Frags& f, f32x16& acc_re, f32x16& acc_im) {
+        acc_re = Split<BF16>::mma(f.arh, f.bh, acc_re);
+        acc_im = Split<BF16>::mma(f.aih, f.bh, acc_im);
+        acc_re = Split<BF16>::mma(f.arh, f.bl, acc_re);
+        acc_im = Split<BF16>::mma(f.aih, f.bl, acc_im);
+        acc_re = Split<BF16>::mma(f.arl, f.bh, acc_re);
+        acc_im = Split<BF16>::mma(f.ail, f.bh, acc_im);
     };
     const int steps = kp / 16;
     int s = 0;
+    Frags f0 = load(0);
     for (; s + 1 < steps; s += 2) {
-        step(s, acc_re0, acc_im0);
-        step(s + 1, acc_re1, acc_im1);
+        const Frags f1 = load(s + 1);
+        mma(f0, acc_re0, acc_im0);
+        if (s + 2 < steps) {
+            f0 = load(s + 2);
+        }
+        mma(f1, acc_re1, acc_im1);
     }
     if (s < steps) {
-        step(s, acc_re0, acc_im0);
+        mma(f0, acc_re0, acc_im0);
     }
     const float scale = ldexpf(*tap_descale, -ex);
     const long long nw = n0 + 1024 * w;
