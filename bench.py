@@ -216,6 +216,16 @@ def bench_fir(rf, args) -> dict:
                  "hbm_frac": round(bytes_step / (gms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                  "calls_per_graph": args.steps}
     plan.destroy()
+    # The timed one-shot calls themselves (tap transform and filtering kernel per call, as
+    # `value`), K of them captured once into a HIP graph and replayed: the rate without the
+    # host's launch gaps between the calls. Reported beside `value`, which stays the eager rate.
+    g = rf.Graph(stream, lambda: [step_for(impl)(i) for i in range(args.steps)])
+    g.launch(stream)
+    stream.synchronize()
+    gms = time_steps(rf, lambda i: g.launch(stream), 1, 0, stream.handle) / args.steps
+    g.destroy()
+    graph_one_shot = {"ms_per_step": round(gms, 5), "msamples_per_s": round(n / gms / 1e3, 1),
+                      "calls_per_graph": args.steps}
     achieved = bytes_step / (kernel_ms * 1e-3) / 1e9
 
     # Arithmetic: the direct form costs 4 flop per tap and output (complex x real MAC); the
@@ -311,6 +321,7 @@ def bench_fir(rf, args) -> dict:
                           "frac": round(bytes_step / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "compute": compute,
         "graph_replay": graph,
+        "graph_replay_one_shot": graph_one_shot,
         "accuracy": accuracy,
         "variants": variants,
         "tuned": tuned,
