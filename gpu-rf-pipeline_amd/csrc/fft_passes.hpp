@@ -133,15 +133,111 @@ __device__ __forceinline__ void stockham_exchange(float2* col, float2 (&u)[8], i
     }
 }
 
+// stockham_exchange for a compile-time length P = 2^LOG2P after a radix-8 pass of span PP,
+// with every LDS index written as one per-lane base plus a compile-time offset, which the LDS
+// instructions carry in their offset field. lds_pad(j + c) = lds_pad(j) + c + c / 8 when c is
+// a multiple of 8, and lds_pad(j) + c when j is and c < 8: the writes go to
+// (t - k) 8 + k + b PP (k = t mod PP; PP = 1 or a multiple of 8), the reads come from
+// t + c_b with every c_b a multiple of P / 8 (a multiple of 8 from P = 64 on).
+template <int LR, bool TAIL, int LOG2P, int PP>
+__device__ __forceinline__ void fixed_exchange(float2* col, float2 (&u)[8], int t, bool active)
+{
+    constexpr int P = 1 << LOG2P;
+    constexpr int N8 = P >> 3;
+    constexpr int R = 1 << LR;
+    if (active) {
+        const int k = t & (PP - 1);
+        float2* dst = col + lds_pad((t - k) * 8 + k);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            dst[PP >= 8 ? b * PP + b * PP / 8 : b] = u[b];
+        }
+    }
+    __syncthreads();
+    if (active) {
+        if constexpr (P >= 64) {
+            const float2* src = col + lds_pad(t);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const int c =(b >> LR) * N8 + (b & (R - 1)) * (P >> LR);
+                u[b] = src[c + c / 8];
+            }
+        } else {
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                u[b] = col[lds_pad(stockham_in<LR>(t, N8, P, b))];
+            }
+        }
+    }
+    if (TAIL) {
+        __syncthreads();
+    }
+}
+
+// Passes PASS .. of stockham_column_fixed (span 8^PASS onwards); returns the radix log2 of the
+// last pass.
+template <bool INV, int LOG2P, int PASS>
+__device__ __forceinline__ int fixed_passes(float2* col, float2 (&u)[8], int t, bool active,
+                                            const float2* __restrict__ table)
+{
+    constexpr int N8 = (1 << LOG2P) >> 3;
+    constexpr int FULL8 = LOG2P / 3;
+    constexpr int REM = LOG2P - 3 * FULL8;
+    constexpr int p = 1 << (3 * PASS);
+    if constexpr (PASS < FULL8) {
+        fixed_exchange<3, (PASS + 1 < FULL8 || REM != 0), LOG2P, (p >> 3)>(col, u, t, active);
+        if (active) {
+            stockham_pass_regs<INV, 3>(u, t, N8, p, table);
+        }
+        return fixed_passes<INV, LOG2P, PASS + 1>(col, u, t, active, table);
+    } else if constexpr (REM != 0) {
+        fixed_exchange<REM, false, LOG2P, (p >> 3)>(col, u, t, active);
+        if (active) {
+            stockham_pass_regs<INV, REM>(u, t, N8, p, table);
+        }
+        return REM;
+    } else {
+        return 3;
+    }
+}
+
+// stockham_column for a compile-time length: the same passes, exchanges and arithmetic (so the
+// same results, bit for bit), with the index arithmetic folded into constants.
+template <bool INV, int LOG2P>
+__device__ __forceinline__ int stockham_column_fixed(float2* col, float2 (&u)[8], int t,
+                                                     bool active, const float2* __restrict__ table)
+{
+    if (active) {
+        stockham_pass_regs<INV, 3>(u, t, (1 << LOG2P) >> 3, 1, table);
+    }
+    return fixed_passes<INV, LOG2P, 1>(col, u, t, active, table);
+}
+
 // The transform of one column: on entry u[r] holds position t + n8 r, on exit u[b] holds
 // output position t + n8 stockham_order(lr, b), where lr (returned) is the radix log2 of the
 // last pass. Every lane of the workgroup calls it (it has barriers); `col` is the column's
 // padded LDS (lds_slots(P) values) and is only touched by active lanes. After it returns, the
-// caller puts a barrier before the LDS is written again.
+// caller puts a barrier before the LDS is written again. Each length 8 .. 4096 runs its own
+// compile-time instance (stockham_column_fixed): with the length known only at run time, the
+// LDS indices of the exchanges took a handful of instructions per access, and the column passes
+// spent about as many instructions on indices as on arithmetic.
 template <bool INV>
 __device__ __forceinline__ int stockham_column(float2* col, float2 (&u)[8], int log2P, int t,
                                                bool active, const float2* __restrict__ table)
 {
+    switch (log2P) {
+    case 3: return stockham_column_fixed<INV, 3>(col, u, t, active, table);
+    case 4: return stockham_column_fixed<INV, 4>(col, u, t, active, table);
+    case 5: return stockham_column_fixed<INV, 5>(col, u, t, active, table);
+    case 6: return stockham_column_fixed<INV, 6>(col, u, t, active, table);
+    case 7: return stockham_column_fixed<INV, 7>(col, u, t, active, table);
+    case 8: return stockham_column_fixed<INV, 8>(col, u, t, active, table);
+    case 9: return stockham_column_fixed<INV, 9>(col, u, t, active, table);
+    case 10: return stockham_column_fixed<INV, 10>(col, u, t, active, table);
+    case 11: return stockham_column_fixed<INV, 11>(col, u, t, active, table);
+    case 12: return stockham_column_fixed<INV, 12>(col, u, t, active, table);
+    default: break;
+    }
     const int P = 1 << log2P;
     const int n8 = P >> 3;
     const int full8 = log2P / 3;
