@@ -110,6 +110,31 @@ def test_spectrum_magnitude_fusion(nfft: int) -> None:
     np.testing.assert_allclose(fused, want, rtol=SPECTRUM_RTOL, atol=SPECTRUM_ATOL * scale)
 
 
+@pytest.mark.parametrize("log2n", list(range(13, 25)))
+def test_four_step_every_column_length(log2n: int) -> None:
+    """The four-step transform at every column length it has (n2 = 8 .. 4096 rows, one
+    compile-time column transform each) against numpy's float64 FFT: rms error relative to the
+    spectrum's rms within 2e-6 (float32 rounding over log2 n passes; a wrong factor or index
+    gives errors of order one), and at 2^23 and 2^24 points the inverse transform back to the
+    input."""
+    n = 1 << log2n
+    x = noise(log2n, n)
+    got = rf.spectrum(x, n)
+    want = np.fft.fft(x.astype(np.complex128))
+    rel = np.sqrt(np.mean(np.abs(got - want) ** 2) / np.mean(np.abs(want) ** 2))
+    assert rel < 2e-6, rel
+    if log2n >= 23:
+        plan = rf.FftPlan(n, 1)
+        d = rf.DeviceBuffer.from_array(x)
+        plan.exec(d.ptr, d.ptr, rf.RF_FFT_FORWARD)
+        plan.exec(d.ptr, d.ptr, rf.RF_FFT_INVERSE)
+        back = d.download() / n
+        rf.synchronize()
+        plan.destroy()
+        d.free()
+        np.testing.assert_allclose(back, x, atol=2e-5 * log2n)
+
+
 def test_fft_inverse_round_trip() -> None:
     for nfft, batch in [(4096, 3), (1 << 15, 2), (1 << 18, 1), (48, 5)]:
         x = noise(nfft, nfft * batch)
