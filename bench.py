@@ -294,7 +294,9 @@ def bench_fir(rf, args) -> dict:
         tuned = {"impl": rf.fir_impls()[best.impl].name, "block": best.block,
                  "outputs_per_lane": best.outputs_per_lane,
                  "transform_points": best.transform_points, "ms": round(best.ms, 5),
-                 "candidates": names}
+                 "candidates": names,
+                 "note": "the tuner's own timing on one buffer set (partly resident in the "
+                         "256 MB Infinity Cache), not comparable with ms_per_step"}
     release(rf, d_h, d_x, d_y, stream)
     workload = (f"FIR {n} complex64 samples x {t}-tap "
                 + ("Kaiser(beta=8) lowpass 0.1 fs" if long_filter else "Hamming lowpass 0.2 fs"))
