@@ -78,6 +78,34 @@ def test_fused_peak_is_the_first_maximum_and_the_oracle_lag(batch) -> None:
         assert pk.magnitude == pytest.approx(mags[pk.index], rel=1e-6)
 
 
+def test_longest_fft_lengths_match_the_oracle() -> None:
+    """Pairs whose FFT lengths are 2^22, 2^23 and 2^24 points (the batch's limit): column
+    transforms of 1024, 2048 and 4096 rows, which the pairs above do not reach. Full outputs
+    against the oracle, fused peak at the oracle's lag and the planted delay."""
+    shapes = [((1 << 21), (1 << 21) - 5, 123), (3_000_000, 2_500_000, -77),
+              (6_000_000, 5_000_000, 45)]
+    pairs = [delayed_pair(100 + i, na, nb, d, 0.3) for i, (na, nb, d) in enumerate(shapes)]
+    d_a = [rf.DeviceBuffer.from_array(a) for a, _ in pairs]
+    d_b = [rf.DeviceBuffer.from_array(b) for _, b in pairs]
+    d_r = [rf.DeviceBuffer(a.size + b.size - 1) for a, b in pairs]
+    h = rf.GccBatch([a.size for a, _ in pairs], [b.size for _, b in pairs])
+    h.bind([x.ptr for x in d_a], [x.ptr for x in d_b], [x.ptr for x in d_r])
+    res = rf.DeviceBuffer(16 * len(pairs), np.uint8)
+    print("gcc batch: FFT lengths 2^22, 2^23, 2^24", flush=True)
+    h.run(res.ptr)
+    peaks = rf.read_peaks(res, len(pairs))
+    for (a, b), (na, nb, delay), pk, d in zip(pairs, shapes, peaks, d_r):
+        got = d.download()
+        want = oracle.gcc_phat(a, b)
+        np.testing.assert_allclose(got, want, rtol=0, atol=GCC_ATOL)
+        ref_idx, _ = oracle.peak(want)
+        assert pk.index == ref_idx == int(np.argmax(np.abs(got)))
+        assert pk.index - (na - 1) == delay
+    for x in d_a + d_b + d_r + [res]:
+        x.free()
+    h.destroy()
+
+
 def test_single_pair_path_agrees(batch) -> None:
     pairs, _, _, _, _, _, peaks, outs = batch
     for (a, b), pk, got in list(zip(pairs, peaks, outs))[:6]:
