@@ -38,8 +38,9 @@ constexpr int kItemThreads = 512;
 // The forward column pass works on tiles of 8192 values (C = 8192 / n2 columns, or all n1 if
 // fewer) on 1024 lanes, twice the inverse pass's 4096, so that it reads and writes row segments
 // twice as long (256 B per row at L = 2^20, 128 B at 2^21): measured 1.43 -> 1.04 ms per
-// batch of 256 uneven pairs (2^12..2^20 samples). Its 80 KiB of LDS allow two workgroups per CU. The inverse pass, whose peak reduction
-// ends every item with a workgroup-wide barrier, measured slower with the wide tile
+// batch of 256 uneven pairs (2^12..2^20 samples). Its 80 KiB of LDS allow two workgroups per
+// CU. The inverse pass, whose peak reduction ends every item with a workgroup-wide barrier,
+// measured slower with the wide tile
 // (0.59 -> 0.69 ms) and keeps 4096; with its transforms run from registers the wide tile
 // (the waves' peaks then in the tile's LDS, so that two 80 KiB workgroups share a CU, and 64
 // registers) ran that batch in the same 2.05 ms. Each pass has its own item map.
