@@ -174,11 +174,12 @@ __global__ __launch_bounds__(kThreads) void channelize_mfma_kernel(
     const int col = 32 * w + i;
     if (t0 + kFramesPerBlock <= frames) {
         float* dst = out + t0 * (2 * kM) + col;
+        // non-temporal, as the fused kernel's stores: the outputs are not re-read here
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
             const int row = (q & 3) + 8 * (q >> 2) + 4 * (l >> 5);
-            dst[row * (2 * kM)] = acc0[q] * scale;
-            dst[(32 + row) * (2 * kM)] = acc1[q] * scale;
+            __builtin_nontemporal_store(acc0[q] * scale, dst + row * (2 * kM));
+            __builtin_nontemporal_store(acc1[q] * scale, dst + (32 + row) * (2 * kM));
         }
         return;
     }
